@@ -1,0 +1,126 @@
+/*
+ * recurgreen_hip.h -- C ABI of the MI355X (gfx950) recursive Green function library.
+ *
+ * Drop-in for the CPU/LAPACK transmission path of RecurGreen's `waveguide` module.  The
+ * reference has no FFI layer; its boundary is the Fortran type `waveguide_t` and its
+ * type-bound procedures (src/waveguide.f90:13-73).  Each entry point below names the
+ * reference procedure it replaces.  The Fortran binding is recurgreen_amd/fortran/
+ * recurgreen_hip.f90 (iso_c_binding); the Python binding is recurgreen_amd/binding.py
+ * (ctypes).
+ *
+ * Conventions
+ *   - Plain C types only.  Complex numbers are interleaved (re, im) doubles, the layout of
+ *     Fortran complex(real64) and C double _Complex.
+ *   - Matrices crossing the ABI are column-major (Fortran order): element (i, j) of an
+ *     m x n matrix at [2*(i + j*m)] / [2*(i + j*m) + 1].
+ *   - A potential batch is upot(ix, iy, r): ix fastest (src/waveguide.f90:43), in units of
+ *     U*dx/k, shape (nxdiso, ny, nreal).
+ *   - Transmission eigenvalues: realization r occupies tallvals[r*neigv .. r*neigv+neigv-1],
+ *     in decreasing order (src/waveguide.f90:448-454).
+ *   - Every function returns an rg_status; 0 is success.  Pointers documented as "host" are
+ *     ordinary host memory; "device" pointers are HIP device allocations on the plan's
+ *     device.  A plan is used by one caller at a time; plans are independent.
+ */
+#ifndef RECURGREEN_HIP_H
+#define RECURGREEN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rg_status {
+    RG_OK = 0,
+    RG_EINVAL = 1,       /* invalid argument; the reference stops with errmsg_invalid_arg */
+    RG_ESINGULAR = 2,    /* a slice matrix was numerically singular (zgetrf info > 0) */
+    RG_ENOCONV = 3,      /* the singular-value iteration did not converge (zgesvd info > 0) */
+    RG_ENOMEM = 4,       /* device or host allocation failed */
+    RG_EDEVICE = 5,      /* HIP runtime error */
+    RG_ENOTSUP = 6,      /* combination not supported by this build (see rg_status_string) */
+    RG_ENONFINITE = 7    /* non-finite value in an input potential or in G(L;0) */
+} rg_status;
+
+typedef enum rg_boundtype { RG_DIRICHLET = 0, RG_PERIODIC = 1 } rg_boundtype;
+
+typedef enum rg_precision {
+    RG_FP64 = 0,         /* every step in complex128 */
+    RG_MIXED_FP32 = 1,   /* slice GEMMs and inversion updates in FP32, FP64 pivot decisions */
+    RG_MIXED_BF16X3 = 2  /* GEMMs as 3-term split-BF16 products with FP32 accumulation */
+} rg_precision;
+
+/* Parameters of waveguide_t%init (src/waveguide.f90:81), plus two extensions:
+ *   ny        > 0 sets the transverse grid (config mode, ny >= nmu); 0 keeps ny = nmu
+ *             (src/waveguide.f90:162,202).
+ *   dos_model 0 = the reference's finite mode-sum DOS (:243); 1 = continuum 1/(4 pi). */
+typedef struct rg_params {
+    int32_t boundtype;
+    int32_t nxdiso, nxfree;
+    double wol, exfac, aratio, dscat, dabso, napera, naperb;
+    int32_t ny;
+    int32_t dos_model;
+} rg_params;
+
+typedef struct rg_plan rg_plan;
+
+/* Mirrors waveguide_t%init (src/waveguide.f90:81-255) and get_nmu/get_neigv/get_nx (:291-314).
+ * batch_max is the number of realizations one device pass holds (device memory is allocated
+ * here, never per call). */
+rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t device, rg_plan **plan);
+/* Mirrors waveguide_t%del (src/waveguide.f90:275-286). */
+rg_status rg_plan_destroy(rg_plan *plan);
+/* dims[0..10] = nmu, ny, nx, neigv, imin, imax, jmin, jmax, nprop, nxdiso, nxfree (1-based
+ * index ranges like the reference).  vals[0..3] = kL, k*dx, dos, stdu (may be NULL). */
+rg_status rg_plan_dims(const rg_plan *plan, int32_t *dims, double *vals);
+
+/* Mirrors compute_tdistrib (src/waveguide.f90:392-462) over a batch of seeds, the loop of
+ * src/main.f90:148-171: realization r uses seed seed0 + r; tallvals (host) receives
+ * nseed*neigv values, status (host, may be NULL) one rg_status per realization.  The disorder
+ * is generated on the device (Philox4x32-10 keyed by seed, ix, iy; Box-Muller cosine branch of
+ * src/random_normal.f90:64). */
+rg_status rg_tdistrib(rg_plan *plan, int64_t seed0, int32_t nseed, int32_t precision,
+                      double *tallvals, int32_t *status);
+/* Same with device-resident output (tallvals_dev: nseed*neigv doubles, status_dev: nseed
+ * int32, may be NULL); the call is asynchronous on the plan's stream. */
+rg_status rg_tdistrib_device(rg_plan *plan, int64_t seed0, int32_t nseed, int32_t precision,
+                             double *tallvals_dev, int32_t *status_dev);
+/* Explicit potentials (host upot, nxdiso*ny*nreal doubles): the path used for parity against
+ * a CPU implementation fed the same potentials. */
+rg_status rg_tdistrib_potential(rg_plan *plan, const double *upot, int32_t nreal, int32_t precision,
+                                double *tallvals, int32_t *status);
+/* Transmission matrices t (filtered ranges, (imax-imin+1) x (jmax-jmin+1) column-major
+ * complex each) for explicit host potentials; upot == NULL means no disorder. */
+rg_status rg_transmission_potential(rg_plan *plan, const double *upot, int32_t nreal, int32_t precision,
+                                    double *tmat, int32_t *status);
+/* Disorder potentials the device generator produces for seeds seed0.. (host upot out,
+ * nxdiso*ny*nreal doubles): lets a caller feed the identical disorder to another code. */
+rg_status rg_generate_potential(rg_plan *plan, int64_t seed0, int32_t nreal, double *upot);
+
+/* Histogram of samples (device or host pointer, flag `on_device`) on bins[0..nbins] (host),
+ * with the counting rules of src/base_utils.f90:491-516: out-of-range samples are counted in
+ * *missing, the last edge is inclusive, NaN is dropped and counted in *dropped.
+ * counts (host) receives nbins int64 values. */
+rg_status rg_histogram(rg_plan *plan, const double *samples, int64_t nsamp, int32_t on_device,
+                       const double *bins, int32_t nbins, int64_t *counts, int64_t *missing,
+                       int64_t *dropped);
+
+/* Stream the plan launches on (a hipStream_t), for callers that order their own work. */
+rg_status rg_plan_stream(rg_plan *plan, void **stream);
+rg_status rg_plan_synchronize(rg_plan *plan);
+/* Per-kernel timing with HIP events on the plan's stream.  enable != 0 starts recording;
+ * rg_plan_kernel_times fills up to `cap` entries of names (NUL-separated, 32 chars each),
+ * total milliseconds and launch counts since the last reset. */
+rg_status rg_plan_profile(rg_plan *plan, int32_t enable);
+rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *ms, int64_t *launches,
+                               int32_t *count);
+
+/* Human-readable text of a status code (static storage). */
+const char *rg_status_string(int32_t code);
+/* Library version string. */
+const char *rg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RECURGREEN_HIP_H */

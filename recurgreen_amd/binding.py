@@ -1,0 +1,234 @@
+"""ctypes binding of the C ABI declared in include/recurgreen_hip.h.
+
+The shared library is built in-tree (recurgreen_amd/librecurgreen_hip.so, see build.py).
+There is no fallback: if the library is missing or cannot be loaded, every entry point
+raises RecurGreenError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librecurgreen_hip.so")
+
+RG_OK, RG_EINVAL, RG_ESINGULAR, RG_ENOCONV, RG_ENOMEM, RG_EDEVICE, RG_ENOTSUP, RG_ENONFINITE = range(8)
+PRECISIONS = {"fp64": 0, "mixed_fp32": 1, "mixed_bf16x3": 2}
+BOUNDTYPES = {"dirichlet": 0, "periodic": 1}
+
+# Every symbol include/recurgreen_hip.h declares.
+EXPORTS = (
+    "rg_plan_create", "rg_plan_destroy", "rg_plan_dims", "rg_tdistrib", "rg_tdistrib_device",
+    "rg_tdistrib_potential", "rg_transmission_potential", "rg_generate_potential", "rg_histogram",
+    "rg_plan_stream", "rg_plan_synchronize", "rg_plan_profile", "rg_plan_kernel_times",
+    "rg_status_string", "rg_version",
+)
+
+
+class RecurGreenError(RuntimeError):
+    """Raised for a non-zero rg_status; .code holds the status."""
+
+    def __init__(self, code: int, msg: str | None = None):
+        self.code = int(code)
+        super().__init__(msg or status_string(code))
+
+
+class rg_params(C.Structure):
+    _fields_ = [
+        ("boundtype", C.c_int32), ("nxdiso", C.c_int32), ("nxfree", C.c_int32),
+        ("wol", C.c_double), ("exfac", C.c_double), ("aratio", C.c_double), ("dscat", C.c_double),
+        ("dabso", C.c_double), ("napera", C.c_double), ("naperb", C.c_double),
+        ("ny", C.c_int32), ("dos_model", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load librecurgreen_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RecurGreenError(RG_EDEVICE, f"{LIB_PATH} is not built; run recurgreen_amd.build.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, dp = C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double)
+    i32p, i64p = C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    sig = {
+        "rg_plan_create": [C.POINTER(rg_params), i32, i32, C.POINTER(vp)],
+        "rg_plan_destroy": [vp],
+        "rg_plan_dims": [vp, i32p, dp],
+        "rg_tdistrib": [vp, i64, i32, i32, vp, vp],
+        "rg_tdistrib_device": [vp, i64, i32, i32, vp, vp],
+        "rg_tdistrib_potential": [vp, vp, i32, i32, vp, vp],
+        "rg_transmission_potential": [vp, vp, i32, i32, vp, vp],
+        "rg_generate_potential": [vp, i64, i32, vp],
+        "rg_histogram": [vp, vp, i64, i32, dp, i32, i64p, i64p, i64p],
+        "rg_plan_stream": [vp, C.POINTER(vp)],
+        "rg_plan_synchronize": [vp],
+        "rg_plan_profile": [vp, i32],
+        "rg_plan_kernel_times": [vp, i32, C.c_char_p, dp, i64p, i32p],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.rg_status_string.argtypes = [C.c_int32]
+    L.rg_status_string.restype = C.c_char_p
+    L.rg_version.argtypes = []
+    L.rg_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def status_string(code: int) -> str:
+    try:
+        return lib().rg_status_string(int(code)).decode()
+    except RecurGreenError:
+        return f"RecurGreen status {code}"
+
+
+def check(code: int) -> None:
+    if code != RG_OK:
+        raise RecurGreenError(code)
+
+
+def ptr(a) -> C.c_void_p:
+    """Address of a numpy array or a torch tensor (device or host)."""
+    if a is None:
+        return C.c_void_p(0)
+    if isinstance(a, np.ndarray):
+        return C.c_void_p(a.ctypes.data)
+    return C.c_void_p(int(a.data_ptr()))
+
+
+class Plan:
+    """Owner of one rg_plan (device buffers, stream, mode constants)."""
+
+    def __init__(self, params: dict, batch_max: int = 256, device: int = 0):
+        L = lib()
+        p = rg_params(
+            boundtype=BOUNDTYPES[params.get("boundtype", "dirichlet")],
+            nxdiso=int(params["nxdiso"]), nxfree=int(params.get("nxfree", 0)),
+            wol=float(params["wol"]), exfac=float(params.get("exfac", 1.0)),
+            aratio=float(params.get("aratio", 1.0)), dscat=float(params.get("dscat", 0.0)),
+            dabso=float(params.get("dabso", 0.0)), napera=float(params.get("napera", 1.0)),
+            naperb=float(params.get("naperb", 1.0)), ny=int(params.get("ny") or 0),
+            dos_model=int(params.get("dos_model", 0)),
+        )
+        h = C.c_void_p()
+        check(L.rg_plan_create(C.byref(p), int(batch_max), int(device), C.byref(h)))
+        self._h = h
+        self.batch_max = int(batch_max)
+        d = (C.c_int32 * 11)()
+        v = (C.c_double * 4)()
+        check(L.rg_plan_dims(self._h, d, v))
+        (self.nmu, self.ny, self.nx, self.neigv, self.imin, self.imax, self.jmin, self.jmax,
+         self.nprop, self.nxdiso, self.nxfree) = list(d)
+        self.kl, self.kdx, self.dos, self.stdu = list(v)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rg_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- sweeps ---------------------------------------------------------------------------
+    def tdistrib(self, seed0: int, nseed: int, precision: str = "fp64"):
+        T = np.zeros(nseed * self.neigv)
+        st = np.zeros(nseed, np.int32)
+        code = lib().rg_tdistrib(self._h, int(seed0), int(nseed), PRECISIONS[precision], ptr(T), ptr(st))
+        if code not in (RG_OK, RG_ESINGULAR, RG_ENOCONV, RG_ENONFINITE):
+            check(code)
+        return T, st, code
+
+    def tdistrib_device(self, seed0: int, nseed: int, precision: str, out, status=None):
+        check(lib().rg_tdistrib_device(self._h, int(seed0), int(nseed), PRECISIONS[precision], ptr(out),
+                                       ptr(status)))
+
+    def tdistrib_potential(self, upot: np.ndarray, precision: str = "fp64"):
+        u = np.asfortranarray(upot, dtype=np.float64)
+        nreal = 1 if u.ndim == 2 else u.shape[2]
+        assert u.shape[:2] == (self.nxdiso, self.ny), u.shape
+        T = np.zeros(nreal * self.neigv)
+        st = np.zeros(nreal, np.int32)
+        code = lib().rg_tdistrib_potential(self._h, ptr(u), nreal, PRECISIONS[precision], ptr(T), ptr(st))
+        if code not in (RG_OK, RG_ESINGULAR, RG_ENOCONV, RG_ENONFINITE):
+            check(code)
+        return T, st, code
+
+    def transmission_potential(self, upot: np.ndarray | None, nreal: int = 1, precision: str = "fp64"):
+        m, n = self.imax - self.imin + 1, self.jmax - self.jmin + 1
+        if upot is not None:
+            u = np.asfortranarray(upot, dtype=np.float64)
+            nreal = 1 if u.ndim == 2 else u.shape[2]
+        else:
+            u = None
+        t = np.zeros((nreal, n, m), complex)  # column-major m x n per realization
+        st = np.zeros(nreal, np.int32)
+        code = lib().rg_transmission_potential(self._h, ptr(u), int(nreal), PRECISIONS[precision], ptr(t), ptr(st))
+        if code not in (RG_OK, RG_ESINGULAR, RG_ENONFINITE):
+            check(code)
+        return np.ascontiguousarray(t.transpose(0, 2, 1)), st, code
+
+    def generate_potential(self, seed0: int, nreal: int) -> np.ndarray:
+        """upot(ix, iy, r) as a Fortran-ordered (nxdiso, ny, nreal) array."""
+        u = np.zeros((nreal, self.ny, self.nxdiso))
+        check(lib().rg_generate_potential(self._h, int(seed0), int(nreal), ptr(u)))
+        return u.transpose(2, 1, 0)
+
+    def histogram(self, samples, bins: np.ndarray, on_device: bool | None = None):
+        b = np.ascontiguousarray(bins, dtype=np.float64)
+        nb = len(b) - 1
+        counts = np.zeros(nb, np.int64)
+        miss, drop = C.c_int64(0), C.c_int64(0)
+        if on_device is None:
+            on_device = not isinstance(samples, np.ndarray)
+        if isinstance(samples, np.ndarray):
+            s = np.ascontiguousarray(samples, dtype=np.float64)
+            n = s.size
+        else:
+            s = samples
+            n = samples.numel()
+        check(lib().rg_histogram(self._h, ptr(s), int(n), int(bool(on_device)),
+                                 b.ctypes.data_as(C.POINTER(C.c_double)), nb,
+                                 counts.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(miss), C.byref(drop)))
+        return counts, miss.value, drop.value
+
+    # -- stream / timing ------------------------------------------------------------------
+    def stream(self) -> int:
+        s = C.c_void_p()
+        check(lib().rg_plan_stream(self._h, C.byref(s)))
+        return int(s.value or 0)
+
+    def synchronize(self):
+        check(lib().rg_plan_synchronize(self._h))
+
+    def profile(self, enable: bool):
+        check(lib().rg_plan_profile(self._h, int(bool(enable))))
+
+    def kernel_times(self) -> dict:
+        cap = 16
+        names = C.create_string_buffer(32 * cap)
+        ms = (C.c_double * cap)()
+        nl = (C.c_int64 * cap)()
+        cnt = C.c_int32(0)
+        check(lib().rg_plan_kernel_times(self._h, cap, names, ms, nl, C.byref(cnt)))
+        out = {}
+        raw = names.raw
+        for k in range(cnt.value):
+            nm = raw[32 * k:32 * k + 32].split(b"\0", 1)[0].decode()
+            out[nm] = {"ms": ms[k], "launches": nl[k]}
+        return out

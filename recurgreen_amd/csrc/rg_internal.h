@@ -1,0 +1,142 @@
+// rg_internal.h -- shared declarations of the gfx950 RGF library (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/recurgreen_hip.h"
+
+// ------------------------------------------------------------------------------------------------
+// Complex double helpers (interleaved double2 = Fortran complex(real64)).
+// ------------------------------------------------------------------------------------------------
+__host__ __device__ inline double2 cmk(double x, double y) { return make_double2(x, y); }
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return cmk(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return cmk(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cneg(double2 a) { return cmk(-a.x, -a.y); }
+__host__ __device__ inline double2 cconj(double2 a) { return cmk(a.x, -a.y); }
+__host__ __device__ inline double2 cmul(double2 a, double2 b)
+{
+    return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// acc - a*b with fused multiply-adds
+__device__ inline double2 cfnms(double2 acc, double2 a, double2 b)
+{
+    acc.x = fma(-a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(-a.x, b.y, acc.y);
+    acc.y = fma(-a.y, b.x, acc.y);
+    return acc;
+}
+// acc + a*b
+__device__ inline double2 cfma(double2 acc, double2 a, double2 b)
+{
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+    return acc;
+}
+// 1/a with Smith's range reduction
+__host__ __device__ inline double2 crecip(double2 a)
+{
+    if (fabs(a.x) >= fabs(a.y)) {
+        double r = a.y / a.x, den = a.x + a.y * r;
+        return cmk(1.0 / den, -r / den);
+    }
+    double r = a.x / a.y, den = a.x * r + a.y;
+    return cmk(r / den, -1.0 / den);
+}
+__host__ __device__ inline double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
+
+// ------------------------------------------------------------------------------------------------
+// Plan.
+// ------------------------------------------------------------------------------------------------
+struct rg_timer {
+    std::string name;
+    double ms = 0.0;
+    int64_t launches = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+};
+
+enum rg_kernel_id { RGK_PROJECT = 0, RGK_INVERSE, RGK_GEMM, RGK_SVD, RGK_HIST, RGK_COPY, RGK_COUNT };
+
+struct rg_plan {
+    rg_params p{};
+    int nmu = 0, ny = 0, nx = 0, neigv = 0, imin = 0, imax = 0, jmin = 0, jmax = 0, nprop = 0;
+    int nxdiso = 0, nxfree = 0;
+    double kl = 0, kdx = 0, dos = 0, stdu = 0;
+    int np = 0;        // padded matrix order (multiple of 16)
+    int ntab = 0;      // projection coefficients per slice (doubles)
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int batch_max = 0;
+    int upot_cap = 0;  // realizations the explicit-potential staging buffer holds
+
+    // host copies of the mode constants (np entries, padded)
+    std::vector<double2> h_mu, h_b, h_abound, h_abulk, h_sqrtmu;
+
+    // device constants
+    double2 *d_b = nullptr, *d_adiag = nullptr, *d_sqrtmu = nullptr;  // d_adiag: 3*np (bound, bulk, last)
+    double *d_tab = nullptr;                                           // ny x ntab
+    // batch buffers
+    double2 *d_gss = nullptr, *d_gs1[2] = {nullptr, nullptr}, *d_tmat = nullptr;
+    double *d_cq = nullptr, *d_teig = nullptr, *d_upot = nullptr;
+    int32_t *d_status = nullptr;
+    unsigned long long *d_hist = nullptr;  // counts + missing + dropped
+    double *d_bins = nullptr;
+    int hist_cap = 0;
+
+    bool profiling = false;
+    rg_timer timers[RGK_COUNT];
+};
+
+// Kernel launchers (rg_kernels_f64.hip).
+void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double *d_upot, int upot_nreal);
+void rg_launch_inverse_f64(rg_plan *P, int nb, int slot, int first, bool with_disorder);
+void rg_launch_gemm_f64(rg_plan *P, int nb, const double2 *gss, const double2 *gs1_old, double2 *gs1_new);
+void rg_launch_fisher_lee_svd_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status,
+                                  double2 *tmat_out);
+void rg_launch_histogram(rg_plan *P, const double *d_samples, int64_t n, const double *d_bins, int nbins,
+                         unsigned long long *d_counts);
+void rg_launch_potential(rg_plan *P, int nb, int64_t seed0, double *d_upot);
+
+// Timing helpers (rg_plan.hip).
+void rg_timer_begin(rg_plan *P, int id, hipEvent_t *stop_out);
+void rg_timer_end(rg_plan *P, int id, hipEvent_t stop);
+
+// Host-side Philox4x32-10 (shared with the device generator: same constants and rounds).
+__host__ __device__ inline void rg_philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Standard normal of disorder cell (ix, iy) of realization `seed`: two 53-bit uniforms from one
+// Philox block, u1 in (0,1], u2 in [0,1), then sqrt(-2 log u1) cos(2 pi u2) -- the cosine branch
+// of the Box-Muller transform used by src/random_normal.f90:64.
+__host__ __device__ inline double rg_normal_cell(uint64_t seed, uint32_t stream, int ix, int iy)
+{
+    uint32_t c[4] = {(uint32_t)iy, (uint32_t)ix, stream, 0u};
+    rg_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32) ^ 0x52474E31u);
+    uint64_t a = ((uint64_t)(c[0] >> 5) << 26) | (uint64_t)(c[1] >> 6);
+    uint64_t b = ((uint64_t)(c[2] >> 5) << 26) | (uint64_t)(c[3] >> 6);
+    double u1 = (double)(a + 1) * 0x1.0p-53;
+    double u2 = (double)b * 0x1.0p-53;
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+}

@@ -1,0 +1,527 @@
+// rg_plan.hip -- C ABI of the gfx950 RGF library: plans, batched sweeps, histogram, timing.
+//
+// A plan is the device-side counterpart of waveguide_t (src/waveguide.f90:13-73): the mode
+// constants of init (src/waveguide.f90:81-255) are computed once on the host in FP64, uploaded,
+// and every batch buffer is allocated at creation, so the sweep entry points never allocate.
+#include <algorithm>
+#include <complex>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "rg_internal.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+using zc = std::complex<double>;
+
+inline double2 to2(zc z) { return make_double2(z.real(), z.imag()); }
+
+#define HIPCHK(expr)                                  \
+    do {                                              \
+        hipError_t e_ = (expr);                       \
+        if (e_ != hipSuccess) return RG_EDEVICE;      \
+    } while (0)
+
+template <class T>
+rg_status dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) return RG_OK;
+    if (hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * count) != hipSuccess) {
+        *p = nullptr;
+        return RG_ENOMEM;
+    }
+    return RG_OK;
+}
+
+void free_plan(rg_plan *P)
+{
+    if (!P) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(P->device);
+    void *bufs[] = {P->d_b, P->d_adiag, P->d_sqrtmu, P->d_tab, P->d_gss, P->d_gs1[0], P->d_gs1[1],
+                    P->d_tmat, P->d_cq, P->d_teig, P->d_upot, P->d_status, P->d_hist, P->d_bins};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (auto &t : P->timers) {
+        for (auto &e : t.pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        for (auto &e : t.pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    }
+    if (P->stream) (void)hipStreamDestroy(P->stream);
+    (void)hipSetDevice(cur);
+    delete P;
+}
+
+// Mode set-up, src/waveguide.f90:81-255 (validation :91-140, Dirichlet :191-229, periodic
+// :151-189, mu/DOS/B :240-248, slice diagonals :358-363, disorder strength :370).
+rg_status init_modes(rg_plan *P)
+{
+    const rg_params &p = P->p;
+    if (!(p.wol >= 2.220446049250313e-16) || !(p.aratio >= 2.220446049250313e-16)) return RG_EINVAL;
+    if (p.exfac < 1.0 || p.exfac > 1e4) return RG_EINVAL;
+    if (p.nxfree < 0 || p.nxdiso < 1 || p.nxdiso + 2 * p.nxfree < 2) return RG_EINVAL;
+    if (p.dscat < 0.0 || p.dabso < 0.0) return RG_EINVAL;
+    if (p.napera <= 0.0 || p.napera > 1.0 || p.naperb <= 0.0 || p.naperb > 1.0) return RG_EINVAL;
+    P->nxdiso = p.nxdiso;
+    P->nxfree = p.nxfree;
+    P->nx = p.nxdiso + 2 * p.nxfree;
+    P->kl = 2.0 * kPi * p.wol * p.aratio;
+    P->kdx = P->kl / p.nxdiso;
+    std::vector<double> kp;
+    if (p.boundtype == RG_DIRICHLET) {
+        if (std::fmod(p.wol, 0.5) == 0.0) return RG_EINVAL;
+        P->nmu = (int)std::floor(2.0 * p.exfac * p.wol);
+        P->imin = 1;
+        P->imax = (int)std::floor(2.0 * p.naperb * p.wol);
+        P->jmin = 1;
+        P->jmax = (int)std::floor(2.0 * p.napera * p.wol);
+        P->neigv = (int)std::floor(2.0 * std::min(p.napera, p.naperb) * p.wol);
+        P->nprop = (int)std::floor(2.0 * p.wol);
+        for (int m = 1; m <= P->nmu; m++) kp.push_back(0.5 * m / p.wol);
+    } else if (p.boundtype == RG_PERIODIC) {
+        if (std::fmod(p.wol, 1.0) == 0.0) return RG_EINVAL;
+        P->nmu = 2 * (int)std::floor(p.exfac * p.wol) + 1;
+        P->imin = P->nmu / 2 + 1 - (int)std::floor(p.naperb * p.wol);
+        P->imax = P->nmu / 2 + 1 + (int)std::floor(p.naperb * p.wol);
+        P->jmin = P->nmu / 2 + 1 - (int)std::floor(p.napera * p.wol);
+        P->jmax = P->nmu / 2 + 1 + (int)std::floor(p.napera * p.wol);
+        P->neigv = 2 * (int)std::floor(std::min(p.napera, p.naperb) * p.wol) + 1;
+        P->nprop = 2 * (int)std::floor(p.wol) + 1;
+        for (int m = 1; m <= P->nmu; m++) kp.push_back((double)(m - P->nmu / 2 - 1) / p.wol);
+    } else {
+        return RG_EINVAL;
+    }
+    P->ny = p.ny > 0 ? p.ny : P->nmu;
+    if (P->nmu < 1 || P->ny < P->nmu) return RG_EINVAL;
+    if (P->neigv < 1) return RG_EINVAL;
+    const int n = P->nmu;
+    P->np = ((n + 15) / 16) * 16;
+    double dsum = 0.0;
+    P->h_mu.assign(P->np, make_double2(0, 0));
+    P->h_b.assign(P->np, make_double2(0, 0));
+    P->h_abound.assign(P->np, make_double2(0, 0));
+    P->h_abulk.assign(P->np, make_double2(0, 0));
+    P->h_sqrtmu.assign(P->np, make_double2(0, 0));
+    for (int m = 0; m < n; m++) {
+        const double kp2 = kp[m] * kp[m];
+        const zc mu = std::sqrt(zc(1.0 - kp2, p.dabso / P->kl));
+        dsum += (1.0 / std::sqrt(zc(1.0 - kp2, 0.0))).real();
+        const zc th = mu * P->kdx;
+        P->h_mu[m] = to2(mu);
+        P->h_b[m] = to2(mu / std::sin(th));
+        P->h_abound[m] = to2(zc(0.0, 1.0) * mu - mu / std::tan(th));
+        P->h_abulk[m] = to2(-2.0 * mu / std::tan(th));
+        P->h_sqrtmu[m] = to2(std::sqrt(mu));
+    }
+    P->dos = p.dos_model == 1 ? 1.0 / (4.0 * kPi) : dsum / (p.wol * (2.0 * kPi) * (2.0 * kPi));
+    P->stdu = std::sqrt(p.dscat * P->ny / (2.0 * kPi * kPi * P->dos * p.wol * p.nxdiso));
+    P->ntab = p.boundtype == RG_DIRICHLET ? 2 * n + 1 : 2 * n;
+    return RG_OK;
+}
+
+rg_status upload_constants(rg_plan *P)
+{
+    const int np = P->np, ny = P->ny, nt = P->ntab;
+    std::vector<double2> ad(3 * (size_t)np, make_double2(0, 0));
+    for (int m = 0; m < P->nmu; m++) {
+        ad[m] = P->h_abound[m];
+        ad[np + m] = P->h_abulk[m];
+        // last slice: boundary diagonal; when nx == 2 the reference adds the bulk diagonal to
+        // slice 2 after copying the boundary (src/waveguide.f90:359-360).
+        ad[2 * np + m] = P->nx == 2 ? cadd(P->h_abound[m], P->h_abulk[m]) : P->h_abound[m];
+    }
+    std::vector<double> tab((size_t)ny * nt);
+    for (int i = 0; i < ny; i++) {
+        const double y = (i + 1 - 0.5) / ny;
+        if (P->p.boundtype == RG_DIRICHLET) {
+            for (int q = 0; q < nt; q++) tab[(size_t)i * nt + q] = std::cos(q * kPi * y);
+        } else {
+            for (int q = 0; q < nt / 2; q++) {
+                tab[(size_t)i * nt + 2 * q] = std::cos(2.0 * kPi * q * y);
+                tab[(size_t)i * nt + 2 * q + 1] = std::sin(2.0 * kPi * q * y);
+            }
+        }
+    }
+    rg_status st;
+    if ((st = dalloc(&P->d_b, np)) || (st = dalloc(&P->d_adiag, 3 * (size_t)np)) || (st = dalloc(&P->d_sqrtmu, np)) ||
+        (st = dalloc(&P->d_tab, (size_t)ny * nt)))
+        return st;
+    HIPCHK(hipMemcpy(P->d_b, P->h_b.data(), sizeof(double2) * np, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P->d_adiag, ad.data(), sizeof(double2) * ad.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P->d_sqrtmu, P->h_sqrtmu.data(), sizeof(double2) * np, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P->d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    return RG_OK;
+}
+
+rg_status alloc_batch(rg_plan *P)
+{
+    const size_t B = (size_t)P->batch_max, mat = (size_t)P->np * P->np;
+    rg_status st;
+    if ((st = dalloc(&P->d_gss, B * mat)) || (st = dalloc(&P->d_gs1[0], B * mat)) || (st = dalloc(&P->d_gs1[1], B * mat)) ||
+        (st = dalloc(&P->d_tmat, B * mat)) || (st = dalloc(&P->d_cq, B * P->ntab)) ||
+        (st = dalloc(&P->d_teig, B * P->neigv)) || (st = dalloc(&P->d_status, B)))
+        return st;
+    const size_t per = (size_t)P->nxdiso * P->ny;
+    const size_t cap_bytes = (size_t)256 << 20;
+    P->upot_cap = (int)std::max<size_t>(1, std::min<size_t>(B, cap_bytes / (per * sizeof(double))));
+    if ((st = dalloc(&P->d_upot, (size_t)P->upot_cap * per))) return st;
+    P->hist_cap = 4096;
+    if ((st = dalloc(&P->d_hist, (size_t)P->hist_cap + 2)) || (st = dalloc(&P->d_bins, (size_t)P->hist_cap + 1))) return st;
+    return RG_OK;
+}
+
+struct DeviceGuard {
+    int prev = 0;
+    explicit DeviceGuard(int dev) { (void)hipGetDevice(&prev); (void)hipSetDevice(dev); }
+    ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+// One pass of the sweep over nb realizations (src/waveguide.f90:424-440): the final G(L;0)
+// is left in the returned buffer.  Exactly one of seed0-based generation or d_upot is used.
+double2 *run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot)
+{
+    int cur = 0;
+    const int nx = P->nx;
+    for (int ix = 1; ix <= nx; ix++) {
+        const int id = ix - P->nxfree;  // 1-based index in the disordered region
+        const bool dis = id >= 1 && id <= P->nxdiso;
+        if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
+        const int slot = ix == 1 ? 0 : (ix == nx ? 2 : 1);
+        rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
+        if (ix == 1) {
+            hipEvent_t stop = nullptr;
+            rg_timer_begin(P, RGK_COPY, &stop);
+            (void)hipMemcpyAsync(P->d_gs1[0], P->d_gss, sizeof(double2) * (size_t)nb * P->np * P->np,
+                                 hipMemcpyDeviceToDevice, P->stream);
+            rg_timer_end(P, RGK_COPY, stop);
+        } else {
+            rg_launch_gemm_f64(P, nb, P->d_gss, P->d_gs1[cur], P->d_gs1[cur ^ 1]);
+            cur ^= 1;
+        }
+    }
+    return P->d_gs1[cur];
+}
+
+rg_status check_common(rg_plan *P, int32_t n, int32_t precision)
+{
+    if (!P || n < 0) return RG_EINVAL;
+    if (precision != RG_FP64) return RG_ENOTSUP;
+    if (P->np > 128) return RG_ENOTSUP;
+    return RG_OK;
+}
+
+rg_status harvest_status(rg_plan *P, int nb, int32_t *status_host, int32_t *worst)
+{
+    std::vector<int32_t> st(nb);
+    HIPCHK(hipMemcpyAsync(st.data(), P->d_status, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    for (int i = 0; i < nb; i++) {
+        if (status_host) status_host[i] = st[i];
+        if (st[i] && !*worst) *worst = st[i];
+    }
+    return RG_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// Timing.
+// ------------------------------------------------------------------------------------------------
+static const char *kTimerNames[RGK_COUNT] = {"project", "slice_inverse", "dyson_gemm", "fisher_lee_svd",
+                                             "histogram", "copy"};
+
+void rg_timer_begin(rg_plan *P, int id, hipEvent_t *stop_out)
+{
+    *stop_out = nullptr;
+    if (!P->profiling) return;
+    rg_timer &t = P->timers[id];
+    std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+    if (!t.pool.empty()) {
+        ev = t.pool.back();
+        t.pool.pop_back();
+    } else {
+        if (hipEventCreate(&ev.first) != hipSuccess) return;
+        if (hipEventCreate(&ev.second) != hipSuccess) { (void)hipEventDestroy(ev.first); return; }
+    }
+    (void)hipEventRecord(ev.first, P->stream);
+    t.pending.push_back(ev);
+    *stop_out = ev.second;
+}
+
+void rg_timer_end(rg_plan *P, int id, hipEvent_t stop)
+{
+    if (!P->profiling || !stop) return;
+    (void)hipEventRecord(stop, P->stream);
+    (void)id;
+}
+
+static void harvest_timers(rg_plan *P)
+{
+    for (auto &t : P->timers) {
+        for (auto &e : t.pending) {
+            float ms = 0.f;
+            if (hipEventSynchronize(e.second) == hipSuccess && hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+                t.ms += ms;
+                t.launches++;
+            }
+            t.pool.push_back(e);
+        }
+        t.pending.clear();
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// C ABI.
+// ------------------------------------------------------------------------------------------------
+extern "C" {
+
+rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t device, rg_plan **plan)
+{
+    if (!params || !plan || batch_max < 1) return RG_EINVAL;
+    *plan = nullptr;
+    rg_plan *P = new (std::nothrow) rg_plan();
+    if (!P) return RG_ENOMEM;
+    P->p = *params;
+    P->device = device;
+    P->batch_max = batch_max;
+    for (int i = 0; i < RGK_COUNT; i++) P->timers[i].name = kTimerNames[i];
+    rg_status st = init_modes(P);  // argument checks first, like init_waveguide
+    if (st) { delete P; return st; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) { delete P; return RG_EDEVICE; }
+    DeviceGuard g(device);
+    if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess) { free_plan(P); return RG_EDEVICE; }
+    if ((st = upload_constants(P)) || (st = alloc_batch(P))) { free_plan(P); return st; }
+    *plan = P;
+    return RG_OK;
+}
+
+rg_status rg_plan_destroy(rg_plan *plan)
+{
+    if (!plan) return RG_EINVAL;
+    (void)hipStreamSynchronize(plan->stream);
+    free_plan(plan);
+    return RG_OK;
+}
+
+rg_status rg_plan_dims(const rg_plan *P, int32_t *dims, double *vals)
+{
+    if (!P || !dims) return RG_EINVAL;
+    const int d[11] = {P->nmu, P->ny, P->nx, P->neigv, P->imin, P->imax, P->jmin, P->jmax, P->nprop, P->nxdiso, P->nxfree};
+    for (int i = 0; i < 11; i++) dims[i] = d[i];
+    if (vals) { vals[0] = P->kl; vals[1] = P->kdx; vals[2] = P->dos; vals[3] = P->stdu; }
+    return RG_OK;
+}
+
+rg_status rg_tdistrib_device(rg_plan *P, int64_t seed0, int32_t nseed, int32_t precision, double *tallvals_dev,
+                             int32_t *status_dev)
+{
+    rg_status st = check_common(P, nseed, precision);
+    if (st) return st;
+    if (!tallvals_dev && nseed > 0) return RG_EINVAL;
+    DeviceGuard g(P->device);
+    for (int32_t off = 0; off < nseed; off += P->batch_max) {
+        const int nb = std::min(P->batch_max, nseed - off);
+        HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
+        double2 *gl0 = run_sweep(P, nb, seed0 + off, nullptr);
+        rg_launch_fisher_lee_svd_f64(P, nb, gl0, tallvals_dev + (size_t)off * P->neigv, P->d_status, nullptr);
+        if (status_dev)
+            HIPCHK(hipMemcpyAsync(status_dev + off, P->d_status, sizeof(int32_t) * nb, hipMemcpyDeviceToDevice, P->stream));
+    }
+    HIPCHK(hipGetLastError());
+    return RG_OK;
+}
+
+rg_status rg_tdistrib(rg_plan *P, int64_t seed0, int32_t nseed, int32_t precision, double *tallvals, int32_t *status)
+{
+    rg_status st = check_common(P, nseed, precision);
+    if (st) return st;
+    if (!tallvals && nseed > 0) return RG_EINVAL;
+    DeviceGuard g(P->device);
+    int32_t worst = 0;
+    for (int32_t off = 0; off < nseed; off += P->batch_max) {
+        const int nb = std::min(P->batch_max, nseed - off);
+        HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
+        double2 *gl0 = run_sweep(P, nb, seed0 + off, nullptr);
+        rg_launch_fisher_lee_svd_f64(P, nb, gl0, P->d_teig, P->d_status, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(tallvals + (size_t)off * P->neigv, P->d_teig, sizeof(double) * (size_t)nb * P->neigv,
+                              hipMemcpyDeviceToHost, P->stream));
+        if ((st = harvest_status(P, nb, status ? status + off : nullptr, &worst))) return st;
+    }
+    if (P->profiling) harvest_timers(P);
+    return worst ? (rg_status)worst : RG_OK;
+}
+
+static rg_status potential_pass(rg_plan *P, const double *upot, int32_t nreal, int32_t precision, double *out,
+                                int32_t *status, bool want_t)
+{
+    rg_status st = check_common(P, nreal, precision);
+    if (st) return st;
+    if (!out && nreal > 0) return RG_EINVAL;
+    DeviceGuard g(P->device);
+    const size_t per = (size_t)P->nxdiso * P->ny;
+    const int m = P->imax - P->imin + 1, nn = P->jmax - P->jmin + 1;
+    const int cap = std::min(P->upot_cap, P->batch_max);
+    int32_t worst = 0;
+    for (int32_t off = 0; off < nreal; off += cap) {
+        const int nb = std::min(cap, nreal - off);
+        HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
+        const double *dup = nullptr;
+        if (upot) {
+            for (size_t e = 0; e < per * nb; e++)
+                if (!std::isfinite(upot[per * off + e])) return RG_ENONFINITE;
+            HIPCHK(hipMemcpyAsync(P->d_upot, upot + per * off, sizeof(double) * per * nb, hipMemcpyHostToDevice, P->stream));
+            dup = P->d_upot;
+        } else {
+            HIPCHK(hipMemsetAsync(P->d_upot, 0, sizeof(double) * per * nb, P->stream));
+            dup = P->d_upot;
+        }
+        double2 *gl0 = run_sweep(P, nb, 0, dup);
+        if (want_t) {
+            rg_launch_fisher_lee_svd_f64(P, nb, gl0, nullptr, P->d_status, P->d_tmat);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(out + (size_t)off * m * nn * 2, P->d_tmat, sizeof(double2) * (size_t)nb * m * nn,
+                                  hipMemcpyDeviceToHost, P->stream));
+        } else {
+            rg_launch_fisher_lee_svd_f64(P, nb, gl0, P->d_teig, P->d_status, nullptr);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(out + (size_t)off * P->neigv, P->d_teig, sizeof(double) * (size_t)nb * P->neigv,
+                                  hipMemcpyDeviceToHost, P->stream));
+        }
+        if ((st = harvest_status(P, nb, status ? status + off : nullptr, &worst))) return st;
+    }
+    if (P->profiling) harvest_timers(P);
+    return worst ? (rg_status)worst : RG_OK;
+}
+
+rg_status rg_tdistrib_potential(rg_plan *P, const double *upot, int32_t nreal, int32_t precision, double *tallvals,
+                                int32_t *status)
+{
+    if (!upot && nreal > 0) return RG_EINVAL;
+    return potential_pass(P, upot, nreal, precision, tallvals, status, false);
+}
+
+rg_status rg_transmission_potential(rg_plan *P, const double *upot, int32_t nreal, int32_t precision, double *tmat,
+                                    int32_t *status)
+{
+    return potential_pass(P, upot, nreal, precision, tmat, status, true);
+}
+
+rg_status rg_generate_potential(rg_plan *P, int64_t seed0, int32_t nreal, double *upot)
+{
+    if (!P || nreal < 0 || (!upot && nreal > 0)) return RG_EINVAL;
+    DeviceGuard g(P->device);
+    const size_t per = (size_t)P->nxdiso * P->ny;
+    for (int32_t off = 0; off < nreal; off += P->upot_cap) {
+        const int nb = std::min(P->upot_cap, nreal - off);
+        rg_launch_potential(P, nb, seed0 + off, P->d_upot);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(upot + per * off, P->d_upot, sizeof(double) * per * nb, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+    }
+    return RG_OK;
+}
+
+rg_status rg_histogram(rg_plan *P, const double *samples, int64_t nsamp, int32_t on_device, const double *bins,
+                       int32_t nbins, int64_t *counts, int64_t *missing, int64_t *dropped)
+{
+    if (!P || !bins || !counts || nbins < 1 || nsamp < 0 || (!samples && nsamp > 0)) return RG_EINVAL;
+    if (nbins > P->hist_cap) return RG_ENOTSUP;
+    for (int i = 0; i < nbins; i++)
+        if (!(bins[i] <= bins[i + 1])) return RG_EINVAL;  // check_histogram, src/base_utils.f90:468-482
+    DeviceGuard g(P->device);
+    HIPCHK(hipMemcpyAsync(P->d_bins, bins, sizeof(double) * (nbins + 1), hipMemcpyHostToDevice, P->stream));
+    HIPCHK(hipMemsetAsync(P->d_hist, 0, sizeof(unsigned long long) * (nbins + 2), P->stream));
+    if (on_device) {
+        if (nsamp > 0) rg_launch_histogram(P, samples, nsamp, P->d_bins, nbins, P->d_hist);
+    } else {
+        // host samples are staged through the explicit-potential buffer in chunks
+        const int64_t chunk = (int64_t)P->upot_cap * P->nxdiso * P->ny;
+        for (int64_t off = 0; off < nsamp; off += chunk) {
+            const int64_t nc = std::min(chunk, nsamp - off);
+            HIPCHK(hipMemcpyAsync(P->d_upot, samples + off, sizeof(double) * nc, hipMemcpyHostToDevice, P->stream));
+            rg_launch_histogram(P, P->d_upot, nc, P->d_bins, nbins, P->d_hist);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<unsigned long long> h(nbins + 2);
+    HIPCHK(hipMemcpyAsync(h.data(), P->d_hist, sizeof(unsigned long long) * (nbins + 2), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    for (int i = 0; i < nbins; i++) counts[i] = (int64_t)h[i];
+    if (missing) *missing = (int64_t)h[nbins];
+    if (dropped) *dropped = (int64_t)h[nbins + 1];
+    if (P->profiling) harvest_timers(P);
+    return RG_OK;
+}
+
+rg_status rg_plan_stream(rg_plan *P, void **stream)
+{
+    if (!P || !stream) return RG_EINVAL;
+    *stream = (void *)P->stream;
+    return RG_OK;
+}
+
+rg_status rg_plan_synchronize(rg_plan *P)
+{
+    if (!P) return RG_EINVAL;
+    DeviceGuard g(P->device);
+    HIPCHK(hipStreamSynchronize(P->stream));
+    HIPCHK(hipGetLastError());
+    if (P->profiling) harvest_timers(P);
+    return RG_OK;
+}
+
+rg_status rg_plan_profile(rg_plan *P, int32_t enable)
+{
+    if (!P) return RG_EINVAL;
+    DeviceGuard g(P->device);
+    (void)hipStreamSynchronize(P->stream);
+    harvest_timers(P);
+    for (auto &t : P->timers) { t.ms = 0.0; t.launches = 0; }
+    P->profiling = enable != 0;
+    return RG_OK;
+}
+
+rg_status rg_plan_kernel_times(rg_plan *P, int32_t cap, char *names, double *ms, int64_t *launches, int32_t *count)
+{
+    if (!P || !count) return RG_EINVAL;
+    DeviceGuard g(P->device);
+    (void)hipStreamSynchronize(P->stream);
+    harvest_timers(P);
+    int k = 0;
+    for (int i = 0; i < RGK_COUNT && k < cap; i++, k++) {
+        if (names) {
+            std::memset(names + 32 * k, 0, 32);
+            std::strncpy(names + 32 * k, P->timers[i].name.c_str(), 31);
+        }
+        if (ms) ms[k] = P->timers[i].ms;
+        if (launches) launches[k] = P->timers[i].launches;
+    }
+    *count = k;
+    return RG_OK;
+}
+
+const char *rg_status_string(int32_t code)
+{
+    switch (code) {
+        case RG_OK: return "RecurGreen: success";
+        case RG_EINVAL: return "RecurGreen: Invalid arguments, aborting...";
+        case RG_ESINGULAR: return "RecurGreen: zgetrf: Matrix is numerically singular, aborting...";
+        case RG_ENOCONV: return "RecurGreen: zgesvd: Failed to find the singular values, aborting...";
+        case RG_ENOMEM: return "RecurGreen: device or host allocation failed";
+        case RG_EDEVICE: return "RecurGreen: HIP runtime error";
+        case RG_ENOTSUP: return "RecurGreen: combination not supported by this build";
+        case RG_ENONFINITE: return "RecurGreen: non-finite value in the potential or in G(L;0)";
+        default: return "RecurGreen: unknown status";
+    }
+}
+
+const char *rg_version(void) { return "recurgreen-hip 0.1 gfx950"; }
+
+}  // extern "C"

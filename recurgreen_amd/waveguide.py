@@ -1,0 +1,98 @@
+"""Host-side mirror of RecurGreen's `waveguide` module over the gfx950 library.
+
+`Waveguide` keeps the names, argument meaning and error behaviour of waveguide_t
+(src/waveguide.f90:13-73): init (:81), get_nmu/get_neigv/get_nx (:291-314),
+compute_tdistrib (:392), del (:275).  The reference computes one realization per call
+(chunk `iseed` of `tallvals`) inside an OpenMP loop (src/main.f90:148-171); here
+compute_tdistrib_batch runs the whole loop as one device pass, and compute_tdistrib keeps the
+single-chunk form for callers written against the reference.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import binding
+from .binding import RecurGreenError
+
+ERRMSG_INVALID_ARG = "RecurGreen: Invalid arguments, aborting..."  # src/base_utils.f90:56
+
+
+class Waveguide:
+    def __init__(self, boundtype: str = "dirichlet", nxdiso: int = 100, nxfree: int = 0, wol: float = 4.3,
+                 exfac: float = 1.0, aratio: float = 1.0, dscat: float = 0.2, dabso: float = 0.0,
+                 napera: float = 1.0, naperb: float = 1.0, *, ny: int | None = None, dos_model: int = 0,
+                 batch_max: int = 256, device: int = 0):
+        self.params = dict(boundtype=boundtype, nxdiso=nxdiso, nxfree=nxfree, wol=wol, exfac=exfac,
+                           aratio=aratio, dscat=dscat, dabso=dabso, napera=napera, naperb=naperb,
+                           ny=ny, dos_model=dos_model)
+        if boundtype not in binding.BOUNDTYPES:
+            raise RecurGreenError(binding.RG_EINVAL,
+                                  f"Unknown boundary condition type '{boundtype}', aborting...")
+        self.plan = binding.Plan(self.params, batch_max=batch_max, device=device)
+
+    # -- accessors (src/waveguide.f90:291-314) ---------------------------------------------
+    def get_nmu(self) -> int:
+        return self.plan.nmu
+
+    def get_neigv(self) -> int:
+        return self.plan.neigv
+
+    def get_nx(self) -> int:
+        return self.plan.nx
+
+    def get_xmesh(self) -> np.ndarray:
+        """x_i/L (src/waveguide.f90:146)."""
+        p = self.params
+        i = np.arange(1, self.plan.nx + 1)
+        return p["aratio"] * (i - 0.5 - p["nxfree"]) / p["nxdiso"]
+
+    def get_parameters(self) -> dict:
+        return dict(self.params)
+
+    # -- transmission eigenvalues -------------------------------------------------------------
+    def compute_tdistrib(self, iseed: int, tallvals: np.ndarray, precision: str = "fp64") -> None:
+        """Chunk `iseed` (1-based) of tallvals <- transmission eigenvalues of realization iseed,
+        with the argument checks of src/waveguide.f90:407-421."""
+        neigv = self.plan.neigv
+        nchunk = tallvals.size // neigv
+        if nchunk == 0:
+            raise RecurGreenError(binding.RG_EINVAL,
+                                  f"Invalid eigenvalue array length, received {tallvals.size}, expected at least {neigv}...")
+        if nchunk * neigv != tallvals.size:
+            raise RecurGreenError(binding.RG_EINVAL,
+                                  f"Invalid eigenvalue array length, received {tallvals.size}, expected {nchunk * neigv}...")
+        if iseed < 1 or nchunk < iseed:
+            raise RecurGreenError(binding.RG_EINVAL,
+                                  f"Invalid chunk index, received {iseed}, expected in [1, {nchunk}]...")
+        T, st, code = self.plan.tdistrib(iseed, 1, precision)
+        if code:
+            raise RecurGreenError(code)
+        tallvals[(iseed - 1) * neigv: iseed * neigv] = T
+
+    def compute_tdistrib_batch(self, seed0: int, nseed: int, precision: str = "fp64"):
+        """All realizations seed0 .. seed0+nseed-1 in one device pass (the OpenMP loop of
+        src/main.f90:153-166).  Returns (tallvals, per-realization status)."""
+        T, st, code = self.plan.tdistrib(seed0, nseed, precision)
+        if code and code not in (binding.RG_ESINGULAR, binding.RG_ENOCONV, binding.RG_ENONFINITE):
+            raise RecurGreenError(code)
+        return T, st
+
+    def compute_tdistrib_potential(self, upot: np.ndarray, precision: str = "fp64"):
+        """Explicit potentials upot(ix, iy[, r]) in U*dx/k units (src/waveguide.f90:43 layout)."""
+        T, st, code = self.plan.tdistrib_potential(upot, precision)
+        return T, st
+
+    def transmission_matrix(self, upot: np.ndarray | None, precision: str = "fp64") -> np.ndarray:
+        t, st, code = self.plan.transmission_potential(upot, precision=precision)
+        if code:
+            raise RecurGreenError(code)
+        return t[0] if t.shape[0] == 1 else t
+
+    def generate_potential(self, seed0: int, nreal: int = 1) -> np.ndarray:
+        return self.plan.generate_potential(seed0, nreal)
+
+    def delete(self) -> None:
+        """waveguide_t%del (src/waveguide.f90:275)."""
+        self.plan.close()
+
+    del_ = delete

@@ -1,0 +1,145 @@
+"""GPU parity through the C ABI against the FP64 CPU oracle on identical inputs.
+
+Tolerances (complex128 path): transmission eigenvalues agree to 1e-10 relative, with an
+absolute floor of 1e-13 for T < 1e-3 (eigenvalues far below 1 carry absolute, not relative,
+rounding from the sweep); t matrices to 1e-10 relative to max|t|; generated potentials to
+4 ulp; histogram counts bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from recurgreen_amd import binding, distrib
+
+pytestmark = pytest.mark.gpu
+
+RTOL, FLOOR = 1e-10, 1e-13
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def assert_T_close(Tg, To, rtol=RTOL, floor=FLOOR):
+    Tg, To = np.asarray(Tg), np.asarray(To)
+    err = np.abs(Tg - To)
+    tol = np.maximum(rtol * np.abs(To), floor)
+    bad = err > tol
+    assert not bad.any(), f"max rel err {np.max(err / np.maximum(np.abs(To), 1e-300)):.3e}, {bad.sum()} bad of {bad.size}"
+
+
+def upot_batch(ow, seeds):
+    """Oracle potentials stacked as upot(ix, iy, r) (Fortran order)."""
+    return np.stack([ow.potential(s).T for s in seeds], axis=2)
+
+
+CASES = {
+    # name: params (reference mode unless ny is given)
+    "ref_N10_free": dict(boundtype="dirichlet", nxdiso=100, nxfree=20, wol=4.3, exfac=1.2, dscat=0.2),
+    "c1_N20_ny205": dict(boundtype="dirichlet", nxdiso=200, wol=10.25, aratio=10 / 10.25, dscat=2.0, ny=205,
+                         dos_model=1),
+    "N37_absorb": dict(boundtype="dirichlet", nxdiso=60, wol=18.7, dscat=3.0, dabso=0.5, ny=80),
+    "N64": dict(boundtype="dirichlet", nxdiso=40, wol=32.25, aratio=20 / 32.25, dscat=1.0, ny=645, dos_model=1),
+    "N100_filtered": dict(boundtype="dirichlet", nxdiso=30, wol=50.25, dscat=1.0, napera=0.6, naperb=0.9),
+    "N128": dict(boundtype="dirichlet", nxdiso=30, wol=64.25, aratio=50 / 64.25, dscat=5.0, ny=1285, dos_model=1),
+    "periodic_N11": dict(boundtype="periodic", nxdiso=80, nxfree=5, wol=4.3, exfac=1.2, dscat=0.5),
+    "periodic_N41_ny60": dict(boundtype="periodic", nxdiso=50, wol=20.3, dscat=2.0, ny=60, napera=0.5),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_explicit_potential_parity(name):
+    p = CASES[name]
+    say(f"explicit potential parity {name}")
+    ow = O.Waveguide(**p)
+    plan = binding.Plan(p, batch_max=4)
+    seeds = [1, 2, 3]
+    U = upot_batch(ow, seeds)
+    Tg, st, code = plan.tdistrib_potential(U)
+    assert code == 0 and not st.any()
+    Tg = Tg.reshape(len(seeds), -1)
+    for k, s in enumerate(seeds):
+        assert_T_close(Tg[k], ow.tdistrib_potential(U[:, :, k].T))
+    plan.close()
+
+
+@pytest.mark.parametrize("name", ["ref_N10_free", "c1_N20_ny205", "N37_absorb", "periodic_N41_ny60"])
+def test_transmission_matrix_parity(name):
+    p = CASES[name]
+    say(f"t-matrix parity {name}")
+    ow = O.Waveguide(**p)
+    plan = binding.Plan(p, batch_max=2)
+    U = upot_batch(ow, [7, 8])
+    tg, st, code = plan.transmission_potential(U)
+    assert code == 0
+    for k in range(2):
+        to = ow.transmission_matrix(U[:, :, k].T)
+        assert np.max(np.abs(tg[k] - to)) <= RTOL * np.max(np.abs(to))
+    plan.close()
+
+
+@pytest.mark.parametrize("name", ["ref_N10_free", "c1_N20_ny205", "periodic_N11"])
+def test_seeded_generator_and_batch_parity(name):
+    """The device generator reproduces the oracle's Philox/Box-Muller field, and the seeded batch
+    (rg_tdistrib, spanning several device passes) equals the oracle's OpenMP loop."""
+    p = CASES[name]
+    say(f"seeded parity {name}")
+    ow = O.Waveguide(**p)
+    plan = binding.Plan(p, batch_max=5)
+    Ug = plan.generate_potential(11, 3)
+    for k in range(3):
+        Uo = ow.potential(11 + k).T
+        np.testing.assert_allclose(Ug[:, :, k], Uo, rtol=4e-16 * 4, atol=4e-16 * 4 * ow.stdu)
+    nseed = 12
+    Tg, st, code = plan.tdistrib(11, nseed)
+    assert code == 0 and not st.any()
+    To = ow.tdistrib_batch(11, nseed, nthreads=4)
+    assert_T_close(Tg, To)
+    plan.close()
+
+
+@pytest.mark.parametrize("boundtype", ["dirichlet", "periodic"])
+def test_zero_disorder_closed_form(boundtype):
+    p = dict(boundtype=boundtype, nxdiso=40, nxfree=3, wol=12.3, dscat=0.0, dabso=0.7)
+    say(f"zero disorder {boundtype}")
+    plan = binding.Plan(p, batch_max=1)
+    ow = O.Waveguide(**p)
+    T, st, code = plan.transmission_potential(None)
+    t = T[0]
+    th = ow.mu * ow.kdx
+    ref = np.exp(1j * th * (ow.nx - 1)) * 2j * ow.mu / (2j * ow.mu)
+    np.testing.assert_allclose(np.diag(t), ref[ow.imin - 1:ow.imax], rtol=1e-11)
+    plan.close()
+
+
+def test_histogram_counts_bit_exact():
+    say("histogram counts")
+    plan = binding.Plan(CASES["ref_N10_free"], batch_max=1)
+    rng = np.random.default_rng(3)
+    s = np.concatenate([rng.random(200000), [0.0, 1.0, -1e-9, 1.0 + 1e-12, np.nan, np.nan, 0.5]])
+    for bt, nb in (("chebyshev", 256), ("linear", 50), ("chebyshev", 7)):
+        e = distrib.bins(bt, nb)
+        e[0] = 0.0
+        c, miss, drop = plan.histogram(s, e)
+        h, m = O.histogram(s, e)
+        np.testing.assert_array_equal(c, h.astype(np.int64))
+        assert miss == m and drop == 2
+        # exact edge values land where the reference's linear scan puts them
+        c2, _, _ = plan.histogram(e.copy(), e)
+        h2, _ = O.histogram(e, e)
+        np.testing.assert_array_equal(c2, h2.astype(np.int64))
+    plan.close()
+
+
+def test_nonfinite_potential_rejected():
+    say("non-finite potential")
+    p = CASES["ref_N10_free"]
+    plan = binding.Plan(p, batch_max=2)
+    U = np.zeros((plan.nxdiso, plan.ny, 1), order="F")
+    U[3, 2, 0] = np.nan
+    T, st, code = plan.tdistrib_potential(U)
+    assert code == binding.RG_ENONFINITE
+    plan.close()

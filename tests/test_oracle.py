@@ -1,0 +1,340 @@
+"""Pins of the FP64 CPU oracle (oracle/rg_oracle.c) -- CPU only.
+
+Each check is chosen so that a plausible mistake in the restatement (dropped term, wrong sign
+or index, transposed operand) fails it: the reference's own known-answer vectors
+(src/base_utils.test.f90), published Philox4x32-10 test vectors, LAPACK (numpy/scipy)
+cross-checks, closed forms and physical invariants (src/waveguide.test.f90:54-122), and a
+brute-force dense solve of the block-tridiagonal system.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def cplx(rows):
+    a = np.array(rows, dtype=float)
+    return a[:, 0] + 1j * a[:, 1]
+
+
+# src/base_utils.test.f90:393-406 (order=[2,1]: row-major)
+A_SVD = cplx([(-0.01283288021173989, 0.9377239102900901), (-0.95242616135770230, -0.8994963544167294),
+              (-0.06886775632719067, 0.6636594333362238), (0.70466932341279700, 0.7651115855726442),
+              (-0.89922794465691420, -0.5377498780142429), (0.13723225079543510, -0.5763274633716735),
+              (-0.22856244103566636, 0.1867127015776933), (-0.70086940334971310, -0.8956086270489085),
+              (-0.98866683961216850, 0.9166435318614940), (0.89492167606273610, -0.4782826178810580),
+              (0.82361640475279610, 0.3462092784006585), (-0.84349089333020190, -0.8929737044010584)]).reshape(4, 3)
+
+# src/base_utils.test.f90:264-283
+A_INV = cplx([(-0.28059215604538010, -0.3461434824696745), (0.18473331224680223, 0.6318812387085075),
+              (-0.40746040147950290, 0.1382727026518729), (0.87389608120820170, -0.2661422854657700),
+              (0.95491046318197360, -0.2105937134153009), (0.16813499164823710, 0.0418637468832905),
+              (0.28781284790753325, -0.4760898733399750), (0.85653516925394910, 0.4349539392523778),
+              (0.01216264260162925, 0.3239628778542527), (-0.19292937387052378, -0.9451826160100225),
+              (-0.28724113783282945, -0.7265218262414725), (0.03149998011019228, 0.4644297650159581),
+              (-0.82541754199947800, 0.9077152529671042), (-0.83414745798494080, -0.3596418431929087),
+              (0.51455340226931460, -0.6296860149451851), (0.14143895745785207, 0.9369393180454373)]).reshape(4, 4)
+
+
+# ---------------------------------------------------------------------------------------------
+# Random numbers
+# ---------------------------------------------------------------------------------------------
+def test_philox_known_answers():
+    # Salmon et al. (SC'11) Random123 known-answer vectors for philox4x32-10
+    assert O.philox4x32_10([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert O.philox4x32_10([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert O.philox4x32_10([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_normal_moments_and_correlation():
+    """Statistics of src/random_normal.test.f90:30-78 on 1e6 draws."""
+    w = O.Waveguide("dirichlet", nxdiso=1000, wol=4.3, exfac=1.0, dscat=1.0, ny=1000)
+    x = (w.potential(1) / w.stdu).ravel()
+    n = x.size
+    dev = 1 / math.sqrt(n)
+    assert abs(x.mean()) < 5 * dev
+    assert abs(x.std() - 1) < 5 * dev
+    assert abs(np.mean(x ** 3)) < 10 * dev
+    assert abs(np.mean(x ** 4) - 3) < 30 * dev
+    for lag in range(1, 6):
+        assert abs(np.mean(np.roll(x, lag) * x)) < 5 * dev
+    # distinct seeds give distinct, uncorrelated fields
+    y = (w.potential(2) / w.stdu).ravel()
+    assert abs(np.mean(x * y)) < 5 * dev
+
+
+# ---------------------------------------------------------------------------------------------
+# Dense kernels vs the reference's known answers and LAPACK
+# ---------------------------------------------------------------------------------------------
+def test_singular_values_known_answer():
+    S = O.singular_values(A_SVD)
+    np.testing.assert_allclose(S, [2.9024423080155306, 1.860396504848104, 0.47034257779202826], rtol=1e-14)
+    np.testing.assert_allclose(S ** 2, [8.42417135135852, 3.461075155251041, 0.22122214048405014], rtol=1e-13)
+    # wide matrix path (t^H) gives the same values
+    np.testing.assert_allclose(O.singular_values(A_SVD.conj().T), S, rtol=1e-14)
+
+
+def test_norm_known_answer():
+    A = cplx([(0.56822371494419420, 0.25427911592041810), (0.92842629910986170, 0.41627873454887654),
+              (0.28171163164123714, -0.45353926993390250), (-0.24135095684205732, -0.32814750945176563),
+              (0.34256543681552910, -0.45025306988077496), (-0.30110268435099340, 0.03998748001092878),
+              (0.63042919841714260, -0.46963146547886137), (0.90685286456350810, 0.12375768017873767),
+              (0.10100461913328873, -0.93995673371119360), (-0.13628841781638990, 0.94900245310786870),
+              (-0.52576497526710990, 0.97769098014051450), (-0.05815140897705184, 0.45950148578633376)]).reshape(4, 3)
+    # Sum T = ||A||_F^2 (the transmission sum rule the sweep tests rely on)
+    assert abs(math.sqrt(np.sum(O.singular_values(A) ** 2)) - 2.646045307275178) < 1e-14
+
+
+def test_inverse_known_answer_and_lapack():
+    X = O.inverse(A_INV)
+    np.testing.assert_allclose(X @ A_INV, np.eye(4), atol=1e-14)
+    np.testing.assert_allclose(X, np.linalg.inv(A_INV), rtol=1e-13)
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 7, 33):
+        A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        np.testing.assert_allclose(O.inverse(A), scipy.linalg.inv(A), rtol=1e-10, atol=1e-12)
+    # pivoting is exercised: a zero leading element needs a row interchange
+    B = np.array([[0, 1], [1, 0]], complex) + 0.0
+    np.testing.assert_allclose(O.inverse(B), B)
+    with pytest.raises(O.OracleError):
+        O.inverse(np.zeros((3, 3), complex))
+
+
+def test_singular_values_vs_lapack_gesvd():
+    rng = np.random.default_rng(7)
+    for m, n in ((5, 5), (12, 7), (7, 12), (40, 40)):
+        A = rng.normal(size=(m, n)) + 1j * rng.normal(size=(m, n))
+        ref = scipy.linalg.svd(A, compute_uv=False, lapack_driver="gesvd")
+        np.testing.assert_allclose(O.singular_values(A), ref, rtol=1e-12)
+    # graded matrix: small singular values keep relative accuracy
+    Q1 = np.linalg.qr(rng.normal(size=(20, 20)) + 1j * rng.normal(size=(20, 20)))[0]
+    Q2 = np.linalg.qr(rng.normal(size=(20, 20)) + 1j * rng.normal(size=(20, 20)))[0]
+    s = np.logspace(0, -9, 20)
+    A = (Q1 * s) @ Q2
+    np.testing.assert_allclose(O.singular_values(A), s, rtol=1e-6)
+
+
+def test_block_tridiag_known_answer():
+    """src/base_utils.test.f90:295-380: reshape(..., shape(amat)) is column-major, amat(i, m, n)."""
+    av = cplx([(-0.3450721935254197, 0.5392304997496717), (0.6897836443224770, 0.5412262828699856),
+               (0.3231770298367444, 0.0785420140112380), (0.3128024256862249, 0.0550528538949643),
+               (0.3334441501416232, 0.8645183075981557), (-0.7888526093217032, -0.8561361200407882),
+               (0.9215704435800176, -0.1313107808488722), (-0.9883756365893341, 0.7297537414237123),
+               (-0.8048043239128400, 0.9277252313541604), (0.9113402180562988, -0.2620342537740332),
+               (0.5449654877189336, 0.1398942224192785), (-0.2826158268607601, -0.4251008049348050)])
+    iv = cplx([(-0.1832660166300050, -0.9263008028015078), (-0.9377593996701266, 0.9927737116226631),
+               (0.8876721334230475, -0.0677127726588384), (0.9551250020205959, 0.9161932570770022),
+               (0.4131234542792588, -0.3505411051103899), (0.0751291445949542, 0.8756862438708004)])
+    amat = av.reshape((3, 2, 2), order="F")
+    imat = iv.reshape((3, 2, 1), order="F")
+    bdiag = np.array([2.0, 1.0], complex)
+    X = O.solve_block_tridiag(amat, bdiag, imat)
+    H = np.zeros((6, 6), complex)
+    for i in range(3):
+        H[2 * i:2 * i + 2, 2 * i:2 * i + 2] = amat[i]
+        if i < 2:
+            H[2 * i:2 * i + 2, 2 * i + 2:2 * i + 4] = np.diag(bdiag)
+            H[2 * i + 2:2 * i + 4, 2 * i:2 * i + 2] = np.diag(bdiag)
+    ref = np.linalg.solve(H, imat.reshape(6, 1))
+    np.testing.assert_allclose(X.reshape(6, 1), ref, rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------------------------
+# Waveguide set-up (src/waveguide.f90:81-255)
+# ---------------------------------------------------------------------------------------------
+def test_init_dims_dirichlet_and_periodic():
+    w = O.Waveguide("dirichlet", nxdiso=100, nxfree=20, wol=4.3, exfac=1.2, dscat=0.2)
+    assert (w.nmu, w.ny, w.nx, w.neigv, w.nprop) == (10, 10, 140, 8, 8)
+    assert (w.imin, w.imax, w.jmin, w.jmax) == (1, 8, 1, 8)
+    w = O.Waveguide("dirichlet", nxdiso=50, wol=4.3, napera=0.5, naperb=1.0)
+    assert (w.imax, w.jmax, w.neigv) == (8, 4, 4)
+    w = O.Waveguide("periodic", nxdiso=50, wol=4.3, exfac=1.2)
+    assert (w.nmu, w.neigv, w.imin, w.imax) == (11, 9, 2, 10)
+    # mode functions: chi_n(y) = sqrt(2/ny) sin(n pi y_i), y_i = (i - 1/2)/ny
+    w = O.Waveguide("dirichlet", nxdiso=10, wol=10.25, ny=205)
+    y = (np.arange(1, 206) - 0.5) / 205
+    ref = np.sqrt(2 / 205) * np.sin(np.outer(y, np.arange(1, 21)) * np.pi)
+    np.testing.assert_allclose(w.modal.real, ref, atol=1e-15)
+    np.testing.assert_allclose(w.modal.T @ w.modal, np.eye(20), atol=1e-13)  # DST-II orthogonality
+    kp = np.arange(1, 21) / (2 * 10.25)
+    np.testing.assert_allclose(w.mu, np.sqrt(1 - kp ** 2 + 0j))
+    np.testing.assert_allclose(w.b, w.mu / np.sin(w.mu * w.kdx))
+    np.testing.assert_allclose(w.abulk, -2 * w.mu / np.tan(w.mu * w.kdx))
+
+
+def test_disorder_strength_formulas():
+    """sigma(U dx/k) of the config W=10.25, L=10, dx=dy=0.05, ls=5: alpha/(dx dy) with
+    alpha = k/(pi nu ls) gives sqrt(dx/(2 pi^2 nu ls dy)); the reference's mode-sum DOS
+    (src/waveguide.f90:243) is a 4 % larger nu."""
+    dx = dy = 0.05
+    w1 = O.Waveguide("dirichlet", nxdiso=200, wol=10.25, aratio=10 / 10.25, dscat=2.0, ny=205, dos_model=1)
+    nu = 1 / (4 * np.pi)
+    assert abs(w1.stdu - math.sqrt(dx / (2 * np.pi ** 2 * nu * 5.0 * dy))) < 1e-12
+    w0 = O.Waveguide("dirichlet", nxdiso=200, wol=10.25, aratio=10 / 10.25, dscat=2.0, ny=205, dos_model=0)
+    kp = np.arange(1, 21) / 20.5
+    dos = np.sum(1 / np.sqrt(1 - kp ** 2)) / (10.25 * (2 * np.pi) ** 2)
+    assert abs(w0.dos - dos) < 1e-15
+    assert abs(w0.stdu - math.sqrt(dx / (2 * np.pi ** 2 * dos * 5.0 * dy))) < 1e-12
+
+
+def test_invalid_parameters_rejected():
+    for kw in (dict(wol=4.5), dict(wol=0.0), dict(exfac=0.5), dict(dscat=-1.0), dict(napera=0.0),
+               dict(naperb=1.5), dict(nxfree=-1)):
+        args = dict(boundtype="dirichlet", nxdiso=50, wol=4.3)
+        args.update(kw)
+        with pytest.raises(O.OracleError):
+            O.Waveguide(**args)
+    with pytest.raises(O.OracleError):
+        O.Waveguide("periodic", nxdiso=50, wol=4.0)
+
+
+# ---------------------------------------------------------------------------------------------
+# Projection shortcut and slice matrices
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("boundtype,ny", [("dirichlet", 10), ("dirichlet", 37), ("periodic", 11), ("periodic", 40)])
+def test_projection_transform_equals_direct(boundtype, ny):
+    w = O.Waveguide(boundtype, nxdiso=20, wol=4.3, exfac=1.2, dscat=0.5, ny=ny)
+    u = w.potential(9)
+    for ix in (0, 7, 19):
+        V0 = w.project(u[:, ix], 0)
+        V1 = w.project(u[:, ix], 1)
+        np.testing.assert_allclose(V1, V0, atol=1e-15)
+        Vb = w.modal.conj().T @ np.diag(u[:, ix]) @ w.modal  # src/waveguide.f90:379
+        np.testing.assert_allclose(V0, Vb, atol=1e-14)
+        np.testing.assert_allclose(V0, V0.conj().T, atol=1e-15)  # Hermitian
+
+
+def test_slice_matrices():
+    w = O.Waveguide("dirichlet", nxdiso=6, nxfree=2, wol=4.3, dscat=0.5)
+    u = w.potential(3)
+    A1 = w.slice_matrix(1, u)
+    np.testing.assert_allclose(A1, np.diag(w.abound))          # free boundary slice
+    A3 = w.slice_matrix(3, u)                                    # first disordered slice
+    np.testing.assert_allclose(A3, np.diag(w.abulk) - w.project(u[:, 0], 0), atol=1e-14)
+    An = w.slice_matrix(w.nx, u)
+    np.testing.assert_allclose(An, np.diag(w.abound))
+
+
+# ---------------------------------------------------------------------------------------------
+# Sweep: closed forms, invariants, brute force
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("boundtype", ["dirichlet", "periodic"])
+def test_zero_disorder_closed_form(boundtype):
+    w = O.Waveguide(boundtype, nxdiso=40, nxfree=3, wol=4.3, dscat=0.0, dabso=0.0)
+    np.testing.assert_allclose(w.tdistrib_potential(np.zeros((w.ny, w.nxdiso))), 1.0, atol=1e-12)
+    # with absorption: T_n = exp(-2 (nx-1) k dx Im mu_n) since G(L;0) = e^{i theta (nx-1)}/(2 i mu)
+    wa = O.Waveguide(boundtype, nxdiso=40, nxfree=3, wol=4.3, dscat=0.0, dabso=0.7)
+    T = wa.tdistrib_potential(np.zeros((wa.ny, wa.nxdiso)))
+    ref = np.sort(np.exp(-2 * (wa.nx - 1) * wa.kdx * wa.mu.imag)[wa.imin - 1:wa.imax])[::-1]
+    np.testing.assert_allclose(T, ref[:wa.neigv], rtol=1e-11)
+    G = wa.green_l0(None)
+    th = wa.mu * wa.kdx
+    np.testing.assert_allclose(np.diag(G), np.exp(1j * th * (wa.nx - 1)) / (2j * wa.mu), rtol=1e-11)
+    np.testing.assert_allclose(G - np.diag(np.diag(G)), 0, atol=1e-13)
+
+
+@pytest.mark.parametrize("boundtype", ["dirichlet", "periodic"])
+def test_unitarity_and_v1_v2(boundtype):
+    """src/waveguide.test.f90:54-122 (norm(t't + r'r - 1) and r/t of v1 vs v2, expected 0)."""
+    w = O.Waveguide(boundtype, nxdiso=100, nxfree=20, wol=4.3, exfac=1.2, dscat=0.2)
+    for seed in range(1, 4):
+        u = w.potential(seed)
+        r1, t1 = w.refl_trans(u, 1)
+        r2, t2 = w.refl_trans(u, 2)
+        assert np.linalg.norm(r1 - r2) < 1e-12
+        assert np.linalg.norm(t1 - t2) < 1e-12
+        # unitarity holds on the propagating channels (sqrt(Re mu) removes evanescent ones)
+        prop = w.mu.real > 1e-12
+        S = np.block([[r1[np.ix_(prop, prop)], t1[np.ix_(prop, prop)].T], [t1[np.ix_(prop, prop)], r1[np.ix_(prop, prop)].T]])
+        tt, rr = t1[np.ix_(prop, prop)], r1[np.ix_(prop, prop)]
+        assert np.linalg.norm(tt.conj().T @ tt + rr.conj().T @ rr - np.eye(prop.sum())) < 1e-11
+        del S
+
+
+def test_green_l0_vs_dense_block_tridiagonal_solve():
+    """Brute force: G(L;0) is the (nx, 1) block of inv(H), H = tridiag(B, A_x, B)."""
+    w = O.Waveguide("dirichlet", nxdiso=30, nxfree=2, wol=4.3, exfac=1.2, dscat=1.5, ny=23)
+    u = w.potential(11)
+    n, nx = w.nmu, w.nx
+    H = np.zeros((n * nx, n * nx), complex)
+    for ix in range(1, nx + 1):
+        s = slice((ix - 1) * n, ix * n)
+        H[s, s] = w.slice_matrix(ix, u)
+        if ix < nx:
+            H[s, ix * n:(ix + 1) * n] = np.diag(w.b)
+            H[ix * n:(ix + 1) * n, s] = np.diag(w.b)
+    E = np.zeros((n * nx, n), complex)
+    E[:n] = np.eye(n)
+    X = np.linalg.solve(H, E)
+    np.testing.assert_allclose(w.green_l0(u), X[(nx - 1) * n:], rtol=1e-10, atol=1e-13)
+    t = w.transmission_matrix(u)
+    D = np.sqrt(w.mu)
+    ref = 2j * D[:, None] * X[(nx - 1) * n:] * D[None, :]
+    np.testing.assert_allclose(t, ref[:w.imax, :w.jmax], rtol=1e-10, atol=1e-13)
+
+
+def test_mirror_invariance_and_sum_rule():
+    w = O.Waveguide("dirichlet", nxdiso=80, wol=6.3, dscat=2.0, dabso=0.3, ny=31)
+    u = w.potential(21)
+    T = w.tdistrib_potential(u)
+    Tm = w.tdistrib_potential(u[:, ::-1])
+    np.testing.assert_allclose(Tm, T, rtol=1e-10, atol=1e-14)
+    t = w.transmission_matrix(u)
+    assert abs(T.sum() - np.sum(np.abs(t) ** 2)) < 1e-12 * T.sum()
+    assert np.all(np.diff(T) <= 0)
+    np.testing.assert_allclose(T, scipy.linalg.svd(t, compute_uv=False, lapack_driver="gesvd") ** 2, rtol=1e-11,
+                               atol=1e-15)
+
+
+def test_filtered_apertures():
+    w = O.Waveguide("dirichlet", nxdiso=60, wol=6.3, dscat=1.0, napera=0.5, naperb=0.8)
+    u = w.potential(4)
+    t = w.transmission_matrix(u)
+    assert t.shape == (w.imax, w.jmax)
+    full = O.Waveguide("dirichlet", nxdiso=60, wol=6.3, dscat=1.0)
+    tf = full.transmission_matrix(u)
+    np.testing.assert_allclose(t, tf[:w.imax, :w.jmax], rtol=1e-12)
+    T = w.tdistrib_potential(u)
+    assert T.size == w.neigv == min(w.imax, w.jmax)
+
+
+def test_batch_matches_single_and_is_order_free():
+    w = O.Waveguide("dirichlet", nxdiso=50, wol=4.3, dscat=1.0, ny=17)
+    Tb = w.tdistrib_batch(5, 6, nthreads=3).reshape(6, -1)
+    for k in range(6):
+        np.testing.assert_array_equal(Tb[k], w.tdistrib_seed(5 + k))
+    Tb2 = w.tdistrib_batch(7, 2, nthreads=1).reshape(2, -1)
+    np.testing.assert_array_equal(Tb2, Tb[2:4])
+
+
+# ---------------------------------------------------------------------------------------------
+# Histogram chain (src/base_utils.f90:491-570, src/main.f90:223-251)
+# ---------------------------------------------------------------------------------------------
+def test_histogram_known_answer():
+    samples = [0.21, 0.00, 0.26, -0.22, -1.00, 0.77, 1.00, -0.37, -0.43, -0.80, -0.17, 0.50]
+    h, miss = O.histogram(samples, [-1.0, -0.5, 0.0, 0.5, 1.0])
+    assert list(h) == [2, 4, 3, 3] and miss == 0
+    np.testing.assert_array_equal(O.moving_average([1.0, 5.0, -3.0, 0.0, 10.0, -4.0], 2), [3.0, 1.0, -1.5, 5.0, 3.0])
+
+
+def test_replot_fixture():
+    """test/replot_1.distrib.csv = plothisto(chebyshev, 256 bins, [0,1]) of test/test_data.csv."""
+    from recurgreen_amd import distrib
+    samples = distrib.read_real_array(os.path.join(GOLD, "test_data.csv"))
+    assert samples.size == 15 and np.isnan(samples).sum() == 1
+    ref = distrib.load_distrib(os.path.join(GOLD, "replot_1.distrib.csv"))
+    rho, miss = O.prepare_distrib(samples, 256, "chebyshev")
+    assert miss == 6
+    np.testing.assert_allclose(rho, ref, rtol=1e-15, atol=0)
+    # host chain of the package on the oracle's counts gives the same rows
+    e = distrib.bins("chebyshev", 256)
+    np.testing.assert_array_equal(e, O.bins("chebyshev", 256))
+    h, _ = O.histogram(samples, e)
+    np.testing.assert_allclose(distrib.rho_from_counts(h, e), ref, rtol=1e-15, atol=0)
