@@ -36,5 +36,32 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+FEXE = os.path.join(FDIR, "recurgreen_gpu")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Fortran host (iso_c_binding module + namelist driver) linked against the HIP library."""
+    bdir = os.path.join(FDIR, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    steps = [
+        [FC, "-O2", "-J", bdir, "-c", os.path.join(FDIR, "recurgreen_hip.f90"), "-o", os.path.join(bdir, "rh.o")],
+        [FC, "-O2", "-J", bdir, "-c", os.path.join(FDIR, "recurgreen_gpu.f90"), "-o", os.path.join(bdir, "rg.o")],
+        [FC, "-o", FEXE + ".tmp", os.path.join(bdir, "rh.o"), os.path.join(bdir, "rg.o"), "-L", HERE,
+         "-lrecurgreen_hip", "-Wl,-rpath,$ORIGIN/.."],
+    ]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("amdflang failed building the Fortran host")
+    os.replace(FEXE + ".tmp", FEXE)
+    return FEXE
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

@@ -18,40 +18,64 @@ namespace {
 // periodic  tab[i][2q] = cos(2 pi q y_i), tab[i][2q+1] = sin(2 pi q y_i), q = 0..N-1
 // (V_mn = D(n-m), D(q) = C(2q) + i C(2q+1)).  The slice u_i is either generated (Philox,
 // realization seed seed0 + b) or read from an explicit potential batch upot(ix, iy, b).
-// Work items (q, s) split the i range in KSPLIT fixed parts; partials are summed in a fixed
-// order, so the result is deterministic.
 // ------------------------------------------------------------------------------------------------
-constexpr int KSPLIT = 4;
+// Tiling: a workgroup owns PRB realizations and all table columns; the i range advances in
+// chunks of PKI staged through LDS (the generated or loaded slice values and the table rows), and
+// each thread accumulates 4 realizations x 5 columns in registers, in a fixed order.
+constexpr int PRB = 16, PKI = 16, PNQ = 320;
 
 __global__ __launch_bounds__(256) void k_project(int ixd, int64_t seed0, const double *__restrict__ upot,
                                                  int nxdiso, int ny, double stdu,
                                                  const double *__restrict__ tab, int ntab,
-                                                 double *__restrict__ cq)
+                                                 double *__restrict__ cq, int nb)
 {
-    extern __shared__ double smem[];
-    double *s_u = smem;            // ny
-    double *s_part = smem + ny;    // KSPLIT * ntab
-    const int b = blockIdx.x;
-    for (int iy = threadIdx.x; iy < ny; iy += blockDim.x) {
-        double u;
-        if (upot) u = upot[(size_t)b * nxdiso * ny + (size_t)iy * nxdiso + ixd];
-        else u = stdu * rg_normal_cell((uint64_t)(seed0 + b), 0u, ixd, iy);
-        s_u[iy] = u;
+    __shared__ double s_u[PKI][PRB + 1];
+    __shared__ double s_t[PKI][PNQ];
+    const int tid = threadIdx.x, b0 = blockIdx.x * PRB;
+    const int tq = tid & 63, tr = tid >> 6;
+    double acc[4][5];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int j = 0; j < 5; j++) acc[a][j] = 0.0;
+    for (int i0 = 0; i0 < ny; i0 += PKI) {
+        {
+            const int rr = tid >> 4, ii = tid & 15, bb = b0 + rr, iy = i0 + ii;
+            double u = 0.0;
+            if (bb < nb && iy < ny) {
+                if (upot) u = upot[(size_t)bb * nxdiso * ny + (size_t)iy * nxdiso + ixd];
+                else u = stdu * rg_normal_cell((uint64_t)(seed0 + bb), 0u, ixd, iy);
+            }
+            s_u[ii][rr] = u;
+        }
+        for (int e = tid; e < PKI * PNQ; e += 256) {
+            const int ii = e / PNQ, q = e % PNQ, iy = i0 + ii;
+            s_t[ii][q] = (iy < ny && q < ntab) ? tab[(size_t)iy * ntab + q] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int ii = 0; ii < PKI; ii++) {
+            double uu[4], tt[5];
+#pragma unroll
+            for (int a = 0; a < 4; a++) uu[a] = s_u[ii][4 * tr + a];
+#pragma unroll
+            for (int j = 0; j < 5; j++) tt[j] = s_t[ii][tq + 64 * j];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int j = 0; j < 5; j++) acc[a][j] = fma(uu[a], tt[j], acc[a][j]);
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    const int chunk = (ny + KSPLIT - 1) / KSPLIT;
-    for (int w = threadIdx.x; w < ntab * KSPLIT; w += blockDim.x) {
-        const int q = w % ntab, s = w / ntab;
-        const int i0 = s * chunk, i1 = min(ny, i0 + chunk);
-        double acc = 0.0;
-        for (int i = i0; i < i1; i++) acc = fma(s_u[i], tab[(size_t)i * ntab + q], acc);
-        s_part[s * ntab + q] = acc;
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < ntab; q += blockDim.x) {
-        double acc = 0.0;
-        for (int s = 0; s < KSPLIT; s++) acc += s_part[s * ntab + q];
-        cq[(size_t)b * ntab + q] = acc / ny;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const int bb = b0 + 4 * tr + a;
+        if (bb >= nb) continue;
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            const int q = tq + 64 * j;
+            if (q < ntab) cq[(size_t)bb * ntab + q] = acc[a][j] / ny;
+        }
     }
 }
 
@@ -77,89 +101,102 @@ __device__ inline double2 slice_v(const double *s_cq, int boundtype, int m, int 
 // ------------------------------------------------------------------------------------------------
 // Slice inversion by Gauss-Jordan elimination with partial pivoting, register resident.
 //
-// The np x np matrix (np = 16*TR) is spread over 4*np threads: thread (r, c) holds rows
-// [r*TR, r*TR+TR) x columns [4c, 4c+4), r = tid % 16, c = tid / 16, so the 16 owners of a column
-// sit in one wavefront and the pivot search is a 16-lane shuffle reduction.  Pivots are chosen
-// like LAPACK's zgetrf (max |Re|+|Im| over the rows not yet used, first index on ties), rows are
-// never swapped; the permutation is undone when the result is written: with p_k the pivot row of
-// column k, inv(A)[i][p_l] = X[p_i][l].  Padded rows/columns (>= n) hold the identity and are
-// skipped.  Two barriers per elimination step; the broadcast buffers alternate between steps.
+// The np x np matrix (np = 16*TR) is spread over 4*np threads: thread (r, c), r = tid % 16,
+// c = tid / 16, holds rows {r + 16 i, i < TR} x columns [4c, 4c+4), so the 16 owners of a column
+// are 16 consecutive lanes of one wavefront (one DPP row) and the pivot search is a 4-stage DPP
+// butterfly.  Pivots are chosen like LAPACK's zgetrf: max |Re|+|Im| over the rows not used yet,
+// lowest row index on ties.  Rows are never swapped; with p_k the pivot row of column k the result
+// is written as inv(A)[i][p_l] = X[p_i][l].
+//
+// Each step is one uniform rank-1 update a <- a - m rv^T with no per-element special cases:
+// column k is first replaced by e_p, the scaled pivot row is rv = a[p,:]/pv (so rv_k = 1/pv), and
+// the pivot row's multiplier is pv - 1, which turns a[p,:] into a[p,:]/pv.  Two barriers per step;
+// the broadcast buffers alternate between steps.  Padded rows/columns (>= n) hold the identity and
+// their steps are skipped.
 // ------------------------------------------------------------------------------------------------
-// One Gauss-Jordan elimination step on column k (column jj of the owning tile).
+template <int CTRL>
+__device__ inline int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+// One butterfly stage of the (max value, min index) reduction over a 16-lane DPP row.
+template <int CTRL>
+__device__ inline void argmax_stage(double &best, int &bidx)
+{
+    const long long bits = __double_as_longlong(best);
+    const int lo = dpp_i<CTRL>((int)(bits & 0xffffffffll));
+    const int hi = dpp_i<CTRL>((int)(bits >> 32));
+    const int oi = dpp_i<CTRL>(bidx);
+    const double ob = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
+}
+
+// dst[0..TC) <- a[ip][0..TC) for a uniform ip, as a chain of scalar branches (no indexed register
+// access, which would move the array to scratch).
+template <int I, int TR, int TC>
+__device__ inline void store_row(const double2 (&a)[TR][TC], const int ip, double2 *dst)
+{
+    if constexpr (I < TR) {
+        if (ip == I) {
+#pragma unroll
+            for (int j = 0; j < TC; j++) dst[j] = a[I][j];
+        } else {
+            store_row<I + 1, TR, TC>(a, ip, dst);
+        }
+    }
+}
+
 template <int TR, int TC>
 __device__ inline void gj_step(double2 (&a)[TR][TC], const int k, const int jj, const int c, const int r,
-                               const int row0, const int col0, unsigned &pivoted, double2 *s_row, double2 *s_col,
-                               double2 &s_pv, int *s_perm, int &s_sing)
+                               unsigned &pivoted, double2 *s_col, double2 *s_rt, double2 &s_pd, int *s_perm,
+                               int &s_sing, const int np)
 {
-    if (c == k / TC) {  // the 16 owners of column k: one wavefront, 16 consecutive lanes
+    if (c == k / TC) {  // the 16 owners of column k
         double best = -1.0;
         int bidx = 0x7fffffff;
-        double2 bval = cmk(0.0, 0.0);
 #pragma unroll
         for (int i = 0; i < TR; i++) {
             if (!((pivoted >> i) & 1u)) {
                 const double v = cabs1(a[i][jj]);
-                if (v > best) { best = v; bidx = row0 + i; bval = a[i][jj]; }
+                if (v > best) { best = v; bidx = r + 16 * i; }
             }
         }
+        argmax_stage<0xB1>(best, bidx);   // quad_perm [1,0,3,2]
+        argmax_stage<0x4E>(best, bidx);   // quad_perm [2,3,0,1]
+        argmax_stage<0x141>(best, bidx);  // row_half_mirror
+        argmax_stage<0x140>(best, bidx);  // row_mirror
 #pragma unroll
-        for (int off = 8; off >= 1; off >>= 1) {
-            const double ob = __shfl_xor(best, off, 16);
-            const int oi = __shfl_xor(bidx, off, 16);
-            const double ox = __shfl_xor(bval.x, off, 16);
-            const double oy = __shfl_xor(bval.y, off, 16);
-            if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; bval = cmk(ox, oy); }
-        }
-        if (r == 0) {
-            s_perm[k] = bidx;
-            s_pv = bval;
-            if (!(best > 0.0)) s_sing = 1;
+        for (int i = 0; i < TR; i++) {
+            const int gi = r + 16 * i;
+            const double2 x = a[i][jj];
+            const bool isp = gi == bidx;
+            s_col[gi] = isp ? cmk(x.x - 1.0, x.y) : x;
+            if (isp) {
+                s_perm[k] = gi;
+                const bool sing = !(best > 0.0);
+                s_pd = sing ? cmk(1.0, 0.0) : crecip(x);
+                if (sing) s_sing = 1;
+            }
+            a[i][jj] = cmk(isp ? 1.0 : 0.0, 0.0);
         }
     }
     __syncthreads();
     const int p = s_perm[k];
-    double2 pv = s_pv;
-    if (pv.x == 0.0 && pv.y == 0.0) pv = cmk(1.0, 0.0);
-    const double2 d = crecip(pv);
-    if (p / TR == r) {
-        const int ip = p % TR;
-        double2 rowv[TC];
-#pragma unroll
-        for (int j = 0; j < TC; j++) rowv[j] = a[0][j];
-#pragma unroll
-        for (int i = 1; i < TR; i++)
-#pragma unroll
-            for (int j = 0; j < TC; j++) {
-                const bool sel = i == ip;
-                rowv[j].x = sel ? a[i][j].x : rowv[j].x;
-                rowv[j].y = sel ? a[i][j].y : rowv[j].y;
-            }
-#pragma unroll
-        for (int j = 0; j < TC; j++) s_row[col0 + j] = (col0 + j == k) ? d : cmul(rowv[j], d);
-    }
-    if (c == k / TC) {
-#pragma unroll
-        for (int i = 0; i < TR; i++) s_col[row0 + i] = a[i][jj];
-    }
+    const double2 d = s_pd;
+    const int ip = p >> 4;  // wave-uniform: the pivot row is register row ip of row group p % 16
+    if (r == (p & 15)) store_row<0, TR, TC>(a, ip, s_rt + TC * c);
     __syncthreads();
     double2 rv[TC];
 #pragma unroll
-    for (int j = 0; j < TC; j++) rv[j] = s_row[col0 + j];
-    const bool own_col = c == k / TC;
+    for (int j = 0; j < TC; j++) rv[j] = cmul(s_rt[TC * c + j], d);
 #pragma unroll
     for (int i = 0; i < TR; i++) {
-        const int gi = row0 + i;
-        const double2 m = s_col[gi];
-        const bool is_p = gi == p;
+        const double2 m = s_col[r + 16 * i];
 #pragma unroll
-        for (int j = 0; j < TC; j++) {
-            double2 v = cfnms(a[i][j], m, rv[j]);
-            if (own_col && j == jj) v = cneg(cmul(m, d));
-            a[i][j].x = is_p ? rv[j].x : v.x;
-            a[i][j].y = is_p ? rv[j].y : v.y;
-        }
+        for (int j = 0; j < TC; j++) a[i][j] = cfnms(a[i][j], m, rv[j]);
     }
-    if (p / TR == r) pivoted |= 1u << (p % TR);
+    if (r == (p & 15)) pivoted |= 1u << ip;
 }
 
 template <int TR>
@@ -170,20 +207,21 @@ __global__ __launch_bounds__(64 * TR) void k_slice_inverse(double2 *__restrict__
                                                              int32_t *__restrict__ status)
 {
     constexpr int NP = 16 * TR, TC = 4, NT = 64 * TR;
-    // staging height: a power of two with STAGE_ROWS * NP <= 2048 (32 KiB); NP is a multiple of 16
+    // staging height for the write-out: a power of two with STAGE_ROWS * NP <= 2048 (32 KiB)
     constexpr int STAGE_ROWS = NP <= 16 ? 16 : (NP <= 32 ? 64 : (NP <= 64 ? 32 : 16));
+    constexpr int RT = 2 * NP;  // two pivot-row buffers
+    constexpr int SHARED = RT > STAGE_ROWS * NP ? RT : STAGE_ROWS * NP;
     __shared__ double s_cq[2 * 256 + 8];
-    __shared__ double2 s_row[2][NP];
+    __shared__ double2 s_buf[SHARED];  // pivot row groups during elimination, staging afterwards
     __shared__ double2 s_col[2][NP];
-    __shared__ double2 s_pv[2];
+    __shared__ double2 s_pd[2];
     __shared__ int s_perm[NP];
     __shared__ int s_iperm[NP];
     __shared__ int s_sing;
-    __shared__ double2 s_stage[STAGE_ROWS * NP];
 
     const int b = blockIdx.x, tid = threadIdx.x;
     const int r = tid & 15, c = tid >> 4;
-    const int row0 = r * TR, col0 = c * TC;
+    const int col0 = c * TC;
     double2 *G = gss + (size_t)b * NP * NP;
 
     if (cq)
@@ -195,7 +233,7 @@ __global__ __launch_bounds__(64 * TR) void k_slice_inverse(double2 *__restrict__
     double2 a[TR][TC];
 #pragma unroll
     for (int i = 0; i < TR; i++) {
-        const int gi = row0 + i;
+        const int gi = r + 16 * i;
 #pragma unroll
         for (int j = 0; j < TC; j++) {
             const int gj = col0 + j;
@@ -219,7 +257,9 @@ __global__ __launch_bounds__(64 * TR) void k_slice_inverse(double2 *__restrict__
         for (int jj = 0; jj < TC; jj++) {
             const int k = kb + jj;
             if (k >= n) break;
-            gj_step<TR, TC>(a, k, jj, c, r, row0, col0, pivoted, s_row[k & 1], s_col[k & 1], s_pv[k & 1], s_perm, s_sing);
+            const int buf = k & 1;
+            gj_step<TR, TC>(a, k, jj, c, r, pivoted, s_col[buf], s_buf + buf * NP, s_pd[buf], s_perm, s_sing,
+                            NP);
         }
     }
     for (int k = n + tid; k < NP; k += NT) s_perm[k] = k;
@@ -232,21 +272,20 @@ __global__ __launch_bounds__(64 * TR) void k_slice_inverse(double2 *__restrict__
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < TR; i++) {
-            const int gi = row0 + i;
+            const int gi = r + 16 * i;
             if (gi >= ch && gi < ch + STAGE_ROWS)
 #pragma unroll
-                for (int j = 0; j < TC; j++) s_stage[(gi - ch) * NP + col0 + j] = a[i][j];
+                for (int j = 0; j < TC; j++) s_buf[(gi - ch) * NP + col0 + j] = a[i][j];
         }
         __syncthreads();
         for (int e = tid; e < STAGE_ROWS * NP; e += NT) {
             const int rr = e / NP, j = e % NP;
             if (ch + rr >= NP) break;
             const int dst = s_iperm[ch + rr];
-            G[(size_t)dst * NP + j] = s_stage[rr * NP + s_iperm[j]];
+            G[(size_t)dst * NP + j] = s_buf[rr * NP + s_iperm[j]];
         }
     }
 }
-
 // ------------------------------------------------------------------------------------------------
 // Dyson update Gs1_new = -Gss diag(b) Gs1_old, batched, LDS-tiled 64x64 output per workgroup,
 // 4x4 complex outputs per thread, k-chunks of 16 (src/waveguide.f90:438).
@@ -308,6 +347,96 @@ __global__ __launch_bounds__(256) void k_dyson_gemm(const double2 *__restrict__ 
             if (gr < NP && gc < NP) C[(size_t)gr * NP + gc] = cneg(acc[i][j]);
         }
     }
+}
+
+// ------------------------------------------------------------------------------------------------
+// The same Dyson update on the FP64 matrix cores (v_mfma_f64_16x16x4_f64).  Lane layout of one
+// 16x16x4 product (measured on gfx950): A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// D register v of lane l holds D[4v + l/16][l%16].  A complex product is four real ones:
+// C_re += A_re B_re - A_im B_im, C_im += A_re B_im + A_im B_re.  A workgroup of 4 wavefronts
+// computes a 64x64 output tile, each wavefront a 32x32 block (2x2 MFMA tiles, re and im
+// accumulators); k advances in chunks of 16 staged through LDS as separate re/im planes, with
+// diag(b) applied while staging Gs1.
+// ------------------------------------------------------------------------------------------------
+typedef double rg_d4 __attribute__((ext_vector_type(4)));
+
+template <int NP>
+__global__ __launch_bounds__(256) void k_dyson_gemm_mfma(const double2 *__restrict__ gss, const double2 *__restrict__ gs1,
+                                                          double2 *__restrict__ out, const double2 *__restrict__ bvec, int n)
+{
+    constexpr int TT = (NP + 63) / 64, LD = 64 + 2;  // padded LDS rows (bank spread)
+    __shared__ double AsR[16][LD], AsI[16][LD];      // A^T chunk: [k][row]
+    __shared__ double BsR[16][LD], BsI[16][LD];      // (b B) chunk: [k][col]
+    const int b = blockIdx.y, ti0 = (blockIdx.x / TT) * 64, tj0 = (blockIdx.x % TT) * 64;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    const double2 *A = gss + (size_t)b * NP * NP;
+    const double2 *B = gs1 + (size_t)b * NP * NP;
+
+    rg_d4 cr[2][2], ci[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) { cr[x][y] = (rg_d4){0, 0, 0, 0}; ci[x][y] = (rg_d4){0, 0, 0, 0}; }
+
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        {   // A chunk: thread -> row tid/4, k (tid%4)*4 .. +3
+            const int rr = tid >> 2, kq = (tid & 3) * 4, gr = ti0 + rr;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int gk = k0 + kq + e;
+                const double2 v = (gr < NP && gk < n) ? A[(size_t)gr * NP + gk] : cmk(0.0, 0.0);
+                AsR[kq + e][rr] = v.x;
+                AsI[kq + e][rr] = v.y;
+            }
+        }
+        {   // B chunk: thread -> k tid/16, cols (tid%16)*4 .. +3, scaled by b[k]
+            const int kk = tid >> 4, cq = (tid & 15) * 4, gk = k0 + kk;
+            const double2 bk = gk < n ? bvec[gk] : cmk(0.0, 0.0);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int gc = tj0 + cq + e;
+                const double2 v = (gk < n && gc < NP) ? cmul(bk, B[(size_t)gk * NP + gc]) : cmk(0.0, 0.0);
+                BsR[kk][cq + e] = v.x;
+                BsI[kk][cq + e] = v.y;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            const int kr = ks * 4 + lk;
+            double ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+            for (int x = 0; x < 2; x++) {
+                ar[x] = AsR[kr][wr + 16 * x + li];
+                ai[x] = AsI[kr][wr + 16 * x + li];
+                nai[x] = -ai[x];
+                br[x] = BsR[kr][wc + 16 * x + li];
+                bi[x] = BsI[kr][wc + 16 * x + li];
+            }
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int y = 0; y < 2; y++) {
+                    cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], cr[x][y], 0, 0, 0);
+                    cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[x], bi[y], cr[x][y], 0, 0, 0);
+                    ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], ci[x][y], 0, 0, 0);
+                    ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], ci[x][y], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    double2 *C = out + (size_t)b * NP * NP;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int gr = ti0 + wr + 16 * x + 4 * v + lk, gc = tj0 + wc + 16 * y + li;
+                if (gr < NP && gc < NP) C[(size_t)gr * NP + gc] = cmk(-cr[x][y][v], -ci[x][y][v]);
+            }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -485,9 +614,8 @@ void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double 
     (void)upot_nreal;
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_PROJECT, &stop);
-    const size_t shm = sizeof(double) * ((size_t)P->ny + (size_t)KSPLIT * P->ntab);
-    hipLaunchKernelGGL(k_project, dim3(nb), dim3(256), shm, P->stream, ixd, seed0, d_upot, P->nxdiso, P->ny,
-                       P->stdu, P->d_tab, P->ntab, P->d_cq);
+    hipLaunchKernelGGL(k_project, dim3((nb + PRB - 1) / PRB), dim3(256), 0, P->stream, ixd, seed0, d_upot, P->nxdiso,
+                       P->ny, P->stdu, P->d_tab, P->ntab, P->d_cq, nb);
     rg_timer_end(P, RGK_PROJECT, stop);
 }
 
@@ -534,7 +662,7 @@ void rg_launch_gemm_f64(rg_plan *P, int nb, const double2 *gss, const double2 *g
     const int tt = (np + 63) / 64;
 #define RG_GEMM_CASE(T)                                                                                     \
     case 16 * T:                                                                                            \
-        hipLaunchKernelGGL(k_dyson_gemm<16 * T>, dim3(tt * tt, nb), dim3(256), 0, P->stream, gss, gs1_old, gs1_new, \
+        hipLaunchKernelGGL(k_dyson_gemm_mfma<16 * T>, dim3(tt * tt, nb), dim3(256), 0, P->stream, gss, gs1_old, gs1_new, \
                            P->d_b, P->nmu);                                                                 \
         break;
     switch (np) {

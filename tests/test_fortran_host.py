@@ -1,0 +1,83 @@
+"""The Fortran host (iso_c_binding module + namelist tdistrib driver) over the C ABI."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from recurgreen_amd import build as rgbuild
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+NML = """&settings
+    task = 'tdistrib'
+/
+&waveguide
+    boundtype = 'dirichlet'
+    wol = {wol}
+    exfac = 1.0
+    aratio = {aratio}
+    dscat = 2.0
+    dabso = 0.0
+    nxdiso = 200
+    nxfree = 0
+    napera = 1.0
+    naperb = 1.0
+/
+&tdistrib
+    nseed = 16
+    nbins = 64
+    nthreads = 4
+    save_eigvals = .true.
+/
+"""
+GPU_GROUP = """&gpu
+    device = 0
+    precision = 'fp64'
+    batch = 16
+    ny = 205
+    dos_model = 1
+    bintype = 'chebyshev'
+/
+"""
+
+
+def run(tmp_path, text):
+    f = tmp_path / "settings.nml"
+    f.write_text(text)
+    return subprocess.run([rgbuild.FEXE, str(f)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+
+
+def test_invalid_width_stops_like_the_reference(tmp_path):
+    """Half-integer wol with Dirichlet boundaries stops before any device call
+    (src/waveguide.f90:194-198)."""
+    assert os.path.exists(rgbuild.FEXE), "Fortran host not built"
+    r = run(tmp_path, NML.format(wol=10.5, aratio=1.0))
+    # `stop "<message>"` exits with status 0, as the reference's `stop errmsg_invalid_arg` does
+    assert "Invalid arguments" in r.stderr + r.stdout
+    assert not (tmp_path / "out_gpu").exists()
+
+
+def test_unknown_task_rejected(tmp_path):
+    r = run(tmp_path, "&settings\n    task = 'nonsense'\n/\n")
+    assert "Unknown task 'nonsense'" in r.stderr
+    assert "Invalid arguments" in r.stderr
+
+
+@pytest.mark.gpu
+def test_fortran_tdistrib_matches_python_path(tmp_path):
+    from oracle import oracle as O
+    from recurgreen_amd import distrib
+
+    print("[gpu-case] fortran host tdistrib", flush=True)
+    r = run(tmp_path, NML.format(wol=10.25, aratio=10 / 10.25) + GPU_GROUP)
+    assert r.returncode == 0 and "Result written" in r.stdout, r.stdout + r.stderr
+    eig = tmp_path / "out_gpu" / "distrib" / "result_1.eigvals.csv"
+    dis = tmp_path / "out_gpu" / "distrib" / "result_1.distrib.csv"
+    T = distrib.read_real_array(str(eig))
+    ow = O.Waveguide("dirichlet", nxdiso=200, wol=10.25, aratio=10 / 10.25, dscat=2.0, ny=205, dos_model=1)
+    To = ow.tdistrib_batch(1, 16, nthreads=4)
+    np.testing.assert_allclose(T, To, rtol=1e-10, atol=1e-13)
+    rho = distrib.load_distrib(str(dis))
+    ref, miss = O.prepare_distrib(T, 64, "chebyshev")
+    np.testing.assert_allclose(rho, ref, rtol=1e-12)
