@@ -85,6 +85,7 @@ struct rg_plan {
     double *d_tab = nullptr;                                           // ny x ntab
     // batch buffers
     double2 *d_gss = nullptr, *d_gs1[2] = {nullptr, nullptr}, *d_tmat = nullptr;
+    float2 *d_gss32 = nullptr, *d_gs1_32[2] = {nullptr, nullptr};  // FP32 state of the mixed sweeps
     double *d_cq = nullptr, *d_teig = nullptr, *d_upot = nullptr;
     int32_t *d_status = nullptr;
     unsigned long long *d_hist = nullptr;  // counts + missing + dropped
@@ -101,6 +102,12 @@ void rg_launch_inverse_f64(rg_plan *P, int nb, int slot, int first, bool with_di
 void rg_launch_gemm_f64(rg_plan *P, int nb, const double2 *gss, const double2 *gs1_old, double2 *gs1_new);
 void rg_launch_fisher_lee_svd_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status,
                                   double2 *tmat_out);
+void rg_launch_fisher_lee_svd_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status,
+                                  double2 *tmat_out);
+// rg_blocked.hip
+void rg_launch_inverse_blocked(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
+void rg_launch_gemm_sp(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new, int precision);
+void rg_launch_copy_f2(rg_plan *P, const float2 *src, float2 *dst, size_t n);
 void rg_launch_histogram(rg_plan *P, const double *d_samples, int64_t n, const double *d_bins, int nbins,
                          unsigned long long *d_counts);
 void rg_launch_potential(rg_plan *P, int nb, int64_t seed0, double *d_upot);

@@ -454,7 +454,11 @@ __device__ inline void rr_pair(int round, int pr, int P, int &p, int &q)
     if (p > q) { int t = p; p = q; q = t; }
 }
 
-__global__ __launch_bounds__(256) void k_fisher_lee_svd(const double2 *__restrict__ gs1, int NP, int imin, int imax,
+__device__ inline double2 ldc(const double2 *p) { return *p; }
+__device__ inline double2 ldc(const float2 *p) { const float2 v = *p; return cmk((double)v.x, (double)v.y); }
+
+template <class E>
+__global__ __launch_bounds__(256) void k_fisher_lee_svd(const E *__restrict__ gs1, int NP, int imin, int imax,
                                                          int jmin, int jmax, const double2 *__restrict__ sqrtmu,
                                                          double2 *__restrict__ work, double *__restrict__ teig,
                                                          int neigv, int32_t *__restrict__ status, int max_sweeps)
@@ -466,13 +470,13 @@ __global__ __launch_bounds__(256) void k_fisher_lee_svd(const double2 *__restric
     const bool tr = nn > m;
     const int rows = tr ? nn : m, cols = tr ? m : nn;
     const int cols_e = cols + (cols & 1);
-    const double2 *G = gs1 + (size_t)b * NP * NP;
+    const E *G = gs1 + (size_t)b * NP * NP;
     double2 *X = work + (size_t)b * NP * NP;
 
     for (int e = tid; e < m * nn; e += blockDim.x) {
         const int ar = e / nn, ac = e % nn;
         const int i = imin - 1 + ar, j = jmin - 1 + ac;
-        const double2 v = cmul(cmul(sqrtmu[i], G[(size_t)i * NP + j]), sqrtmu[j]);
+        const double2 v = cmul(cmul(sqrtmu[i], ldc(G + (size_t)i * NP + j)), sqrtmu[j]);
         const double2 t = cmk(-2.0 * v.y, 2.0 * v.x);  // 2i * v
         if (!tr) X[(size_t)ac * NP + ar] = t;
         else X[(size_t)ar * NP + ac] = cconj(t);
@@ -551,17 +555,18 @@ __global__ __launch_bounds__(256) void k_fisher_lee_svd(const double2 *__restric
 }
 
 // t written in the ABI layout: column-major m x nn per realization.
-__global__ __launch_bounds__(256) void k_fisher_lee_out(const double2 *__restrict__ gs1, int NP, int imin, int imax,
+template <class E>
+__global__ __launch_bounds__(256) void k_fisher_lee_out(const E *__restrict__ gs1, int NP, int imin, int imax,
                                                         int jmin, int jmax, const double2 *__restrict__ sqrtmu,
                                                         double2 *__restrict__ tout)
 {
     const int b = blockIdx.x;
     const int m = imax - imin + 1, nn = jmax - jmin + 1;
-    const double2 *G = gs1 + (size_t)b * NP * NP;
+    const E *G = gs1 + (size_t)b * NP * NP;
     for (int e = threadIdx.x; e < m * nn; e += blockDim.x) {
         const int ar = e % m, ac = e / m;
         const int i = imin - 1 + ar, j = jmin - 1 + ac;
-        const double2 v = cmul(cmul(sqrtmu[i], G[(size_t)i * NP + j]), sqrtmu[j]);
+        const double2 v = cmul(cmul(sqrtmu[i], ldc(G + (size_t)i * NP + j)), sqrtmu[j]);
         tout[(size_t)b * m * nn + e] = cmk(-2.0 * v.y, 2.0 * v.x);
     }
 }
@@ -680,18 +685,30 @@ void rg_launch_gemm_f64(rg_plan *P, int nb, const double2 *gss, const double2 *g
     rg_timer_end(P, RGK_GEMM, stop);
 }
 
-void rg_launch_fisher_lee_svd_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status,
-                                  double2 *tmat_out)
+template <class E>
+static void launch_fisher_lee_svd(rg_plan *P, int nb, const E *gs1, double *teig, int32_t *status, double2 *tmat_out)
 {
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_SVD, &stop);
     if (tmat_out)
-        hipLaunchKernelGGL(k_fisher_lee_out, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->imin, P->imax,
+        hipLaunchKernelGGL(k_fisher_lee_out<E>, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->imin, P->imax,
                            P->jmin, P->jmax, P->d_sqrtmu, tmat_out);
     else
-        hipLaunchKernelGGL(k_fisher_lee_svd, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->imin, P->imax,
+        hipLaunchKernelGGL(k_fisher_lee_svd<E>, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->imin, P->imax,
                            P->jmin, P->jmax, P->d_sqrtmu, P->d_tmat, teig, P->neigv, status, 60);
     rg_timer_end(P, RGK_SVD, stop);
+}
+
+void rg_launch_fisher_lee_svd_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status,
+                                  double2 *tmat_out)
+{
+    launch_fisher_lee_svd<double2>(P, nb, gs1, teig, status, tmat_out);
+}
+
+void rg_launch_fisher_lee_svd_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status,
+                                  double2 *tmat_out)
+{
+    launch_fisher_lee_svd<float2>(P, nb, gs1, teig, status, tmat_out);
 }
 
 void rg_launch_histogram(rg_plan *P, const double *d_samples, int64_t n, const double *d_bins, int nbins,

@@ -45,7 +45,8 @@ void free_plan(rg_plan *P)
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(P->device);
     void *bufs[] = {P->d_b, P->d_adiag, P->d_sqrtmu, P->d_tab, P->d_gss, P->d_gs1[0], P->d_gs1[1],
-                    P->d_tmat, P->d_cq, P->d_teig, P->d_upot, P->d_status, P->d_hist, P->d_bins};
+                    P->d_tmat, P->d_cq, P->d_teig, P->d_upot, P->d_status, P->d_hist, P->d_bins,
+                    P->d_gss32, P->d_gs1_32[0], P->d_gs1_32[1]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &t : P->timers) {
@@ -164,7 +165,9 @@ rg_status alloc_batch(rg_plan *P)
     rg_status st;
     if ((st = dalloc(&P->d_gss, B * mat)) || (st = dalloc(&P->d_gs1[0], B * mat)) || (st = dalloc(&P->d_gs1[1], B * mat)) ||
         (st = dalloc(&P->d_tmat, B * mat)) || (st = dalloc(&P->d_cq, B * P->ntab)) ||
-        (st = dalloc(&P->d_teig, B * P->neigv)) || (st = dalloc(&P->d_status, B)))
+        (st = dalloc(&P->d_teig, B * P->neigv)) || (st = dalloc(&P->d_status, B)) ||
+        (st = dalloc(&P->d_gss32, B * mat)) || (st = dalloc(&P->d_gs1_32[0], B * mat)) ||
+        (st = dalloc(&P->d_gs1_32[1], B * mat)))
         return st;
     const size_t per = (size_t)P->nxdiso * P->ny;
     const size_t cap_bytes = (size_t)256 << 20;
@@ -181,36 +184,51 @@ struct DeviceGuard {
     ~DeviceGuard() { (void)hipSetDevice(prev); }
 };
 
-// One pass of the sweep over nb realizations (src/waveguide.f90:424-440): the final G(L;0)
-// is left in the returned buffer.  Exactly one of seed0-based generation or d_upot is used.
-double2 *run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot)
+// One pass of the sweep over nb realizations (src/waveguide.f90:424-440) followed by the
+// Fisher-Lee relation and the singular values (:443-449) into teig / tmat_out.  Exactly one of
+// seed0-based generation or d_upot is used.  FP64 keeps complex128 state; the mixed modes keep
+// complex64 state with FP64 projection, pivoting and singular values.
+void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int precision, double *teig,
+               double2 *tmat_out)
 {
     int cur = 0;
     const int nx = P->nx;
+    const size_t mat = (size_t)nb * P->np * P->np;
     for (int ix = 1; ix <= nx; ix++) {
         const int id = ix - P->nxfree;  // 1-based index in the disordered region
         const bool dis = id >= 1 && id <= P->nxdiso;
         if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
         const int slot = ix == 1 ? 0 : (ix == nx ? 2 : 1);
-        rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
-        if (ix == 1) {
-            hipEvent_t stop = nullptr;
-            rg_timer_begin(P, RGK_COPY, &stop);
-            (void)hipMemcpyAsync(P->d_gs1[0], P->d_gss, sizeof(double2) * (size_t)nb * P->np * P->np,
-                                 hipMemcpyDeviceToDevice, P->stream);
-            rg_timer_end(P, RGK_COPY, stop);
+        rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, precision);
+        hipEvent_t stop = nullptr;
+        if (precision == RG_FP64) {
+            if (ix == 1) {
+                rg_timer_begin(P, RGK_COPY, &stop);
+                (void)hipMemcpyAsync(P->d_gs1[0], P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+                rg_timer_end(P, RGK_COPY, stop);
+            } else {
+                rg_launch_gemm_f64(P, nb, P->d_gss, P->d_gs1[cur], P->d_gs1[cur ^ 1]);
+                cur ^= 1;
+            }
         } else {
-            rg_launch_gemm_f64(P, nb, P->d_gss, P->d_gs1[cur], P->d_gs1[cur ^ 1]);
-            cur ^= 1;
+            if (ix == 1) {
+                rg_timer_begin(P, RGK_COPY, &stop);
+                rg_launch_copy_f2(P, P->d_gss32, P->d_gs1_32[0], mat);
+                rg_timer_end(P, RGK_COPY, stop);
+            } else {
+                rg_launch_gemm_sp(P, nb, P->d_gss32, P->d_gs1_32[cur], P->d_gs1_32[cur ^ 1], precision);
+                cur ^= 1;
+            }
         }
     }
-    return P->d_gs1[cur];
+    if (precision == RG_FP64) rg_launch_fisher_lee_svd_f64(P, nb, P->d_gs1[cur], teig, P->d_status, tmat_out);
+    else rg_launch_fisher_lee_svd_f32(P, nb, P->d_gs1_32[cur], teig, P->d_status, tmat_out);
 }
 
 rg_status check_common(rg_plan *P, int32_t n, int32_t precision)
 {
     if (!P || n < 0) return RG_EINVAL;
-    if (precision != RG_FP64) return RG_ENOTSUP;
+    if (precision != RG_FP64 && precision != RG_MIXED_FP32 && precision != RG_MIXED_BF16X3) return RG_EINVAL;
     if (P->np > 128) return RG_ENOTSUP;
     return RG_OK;
 }
@@ -328,8 +346,7 @@ rg_status rg_tdistrib_device(rg_plan *P, int64_t seed0, int32_t nseed, int32_t p
     for (int32_t off = 0; off < nseed; off += P->batch_max) {
         const int nb = std::min(P->batch_max, nseed - off);
         HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
-        double2 *gl0 = run_sweep(P, nb, seed0 + off, nullptr);
-        rg_launch_fisher_lee_svd_f64(P, nb, gl0, tallvals_dev + (size_t)off * P->neigv, P->d_status, nullptr);
+        run_sweep(P, nb, seed0 + off, nullptr, precision, tallvals_dev + (size_t)off * P->neigv, nullptr);
         if (status_dev)
             HIPCHK(hipMemcpyAsync(status_dev + off, P->d_status, sizeof(int32_t) * nb, hipMemcpyDeviceToDevice, P->stream));
     }
@@ -347,8 +364,7 @@ rg_status rg_tdistrib(rg_plan *P, int64_t seed0, int32_t nseed, int32_t precisio
     for (int32_t off = 0; off < nseed; off += P->batch_max) {
         const int nb = std::min(P->batch_max, nseed - off);
         HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
-        double2 *gl0 = run_sweep(P, nb, seed0 + off, nullptr);
-        rg_launch_fisher_lee_svd_f64(P, nb, gl0, P->d_teig, P->d_status, nullptr);
+        run_sweep(P, nb, seed0 + off, nullptr, precision, P->d_teig, nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(tallvals + (size_t)off * P->neigv, P->d_teig, sizeof(double) * (size_t)nb * P->neigv,
                               hipMemcpyDeviceToHost, P->stream));
@@ -382,14 +398,13 @@ static rg_status potential_pass(rg_plan *P, const double *upot, int32_t nreal, i
             HIPCHK(hipMemsetAsync(P->d_upot, 0, sizeof(double) * per * nb, P->stream));
             dup = P->d_upot;
         }
-        double2 *gl0 = run_sweep(P, nb, 0, dup);
         if (want_t) {
-            rg_launch_fisher_lee_svd_f64(P, nb, gl0, nullptr, P->d_status, P->d_tmat);
+            run_sweep(P, nb, 0, dup, precision, nullptr, P->d_tmat);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(out + (size_t)off * m * nn * 2, P->d_tmat, sizeof(double2) * (size_t)nb * m * nn,
                                   hipMemcpyDeviceToHost, P->stream));
         } else {
-            rg_launch_fisher_lee_svd_f64(P, nb, gl0, P->d_teig, P->d_status, nullptr);
+            run_sweep(P, nb, 0, dup, precision, P->d_teig, nullptr);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(out + (size_t)off * P->neigv, P->d_teig, sizeof(double) * (size_t)nb * P->neigv,
                                   hipMemcpyDeviceToHost, P->stream));

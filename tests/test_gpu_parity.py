@@ -143,3 +143,28 @@ def test_nonfinite_potential_rejected():
     T, st, code = plan.tdistrib_potential(U)
     assert code == binding.RG_ENONFINITE
     plan.close()
+
+
+# ---------------------------------------------------------------------------------------------
+# Mixed precision (complex64 state, FP64 pivoting / projection / singular values).
+# Errors are stated against the FP64 oracle on identical potentials; bounds are absolute on T.
+# ---------------------------------------------------------------------------------------------
+MIXED_TOL = {"mixed_fp32": 2e-4, "mixed_bf16x3": 5e-3}
+
+
+@pytest.mark.parametrize("precision", ["mixed_fp32", "mixed_bf16x3"])
+@pytest.mark.parametrize("name", ["ref_N10_free", "c1_N20_ny205", "N64", "N128", "periodic_N41_ny60"])
+def test_mixed_precision_against_oracle(name, precision):
+    p = CASES[name]
+    say(f"mixed {precision} {name}")
+    ow = O.Waveguide(**p)
+    plan = binding.Plan(p, batch_max=4)
+    U = upot_batch(ow, [1, 2, 3])
+    Tg, st, code = plan.tdistrib_potential(U, precision)
+    assert code == 0 and not st.any()
+    Tg = Tg.reshape(3, -1)
+    To = np.stack([ow.tdistrib_potential(U[:, :, k].T) for k in range(3)])
+    err = np.max(np.abs(Tg - To))
+    print(f"[gpu-case] {precision} {name}: max |dT| = {err:.3e}, mean dT = {np.mean(Tg - To):.3e}", flush=True)
+    assert err <= MIXED_TOL[precision]
+    plan.close()
