@@ -26,10 +26,12 @@ DTYPE = {"fp64": "f64", "mixed_fp32": "f32+f64", "mixed_bf16x3": "bf16x3+f64"}
 
 
 def workload_params(args):
-    W, L, dx = 64.25, 50.0, 0.05
-    nxdiso = int(round(L / dx)) if args.nx is None else args.nx
+    """W=64.25, dx=dy=0.05, l_s=10; L=50 (nx=1000) unless --nx shortens the guide at fixed dx."""
+    W, dx, ls = 64.25, 0.05, 10.0
+    nxdiso = 1000 if args.nx is None else args.nx
+    L = nxdiso * dx
     return dict(boundtype="dirichlet", nxdiso=nxdiso, nxfree=0, wol=W, exfac=1.0, aratio=L / W,
-                dscat=L / 10.0, dabso=0.0, napera=1.0, naperb=1.0, ny=int(round(W / 0.05)), dos_model=1)
+                dscat=L / ls, dabso=0.0, napera=1.0, naperb=1.0, ny=int(round(W / dx)), dos_model=1)
 
 
 def cpu_baseline(params, seconds):
@@ -54,6 +56,43 @@ def cpu_baseline(params, seconds):
                       f"(FP64 oracle, OpenMP over realizations, {dt:.1f} s wall)"}
 
 
+def timed_sweeps(plan, precision, B, steps, warmup, out, status):
+    """Warmup + timed batched sweeps; the same seeds for every precision.  Returns (elapsed s,
+    per-kernel times, failed realizations); `out` holds the last step's eigenvalues."""
+    import torch
+
+    for w in range(warmup):
+        plan.tdistrib_device(1 + (w % max(1, steps)) * B, B, precision, out, status)
+    plan.synchronize()
+    torch.cuda.synchronize()
+    plan.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        plan.tdistrib_device(1 + s * B, B, precision, out, status)
+    plan.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    times = plan.kernel_times()
+    plan.profile(False)
+    return elapsed, times, int((status != 0).sum().item())
+
+
+def accuracy(plan, t_ref, t_mix):
+    """Errors of a mixed sweep against the FP64 sweep on identical seeds: mean T, max |dT|, and
+    the max bin deviation of rho(T) (50 linear bins and 256 Chebyshev bins on [0, 1])."""
+    from recurgreen_amd import distrib
+
+    out = {"mean_T_fp64": float(t_ref.mean().item()), "mean_T_err": float((t_mix.mean() - t_ref.mean()).abs().item()),
+           "max_abs_dT": float((t_mix - t_ref).abs().max().item())}
+    for bt, nb in (("linear", 50), ("chebyshev", 256)):
+        e = distrib.bins(bt, nb)
+        rr = distrib.rho_from_counts(plan.histogram(t_ref, e)[0], e)[:, 1]
+        rm = distrib.rho_from_counts(plan.histogram(t_mix, e)[0], e)[:, 1]
+        out[f"max_rho_bin_err_{bt}{nb}"] = float(abs(rm - rr).max())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,7 +100,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--nx", type=int, default=None, help="override slice count (profiling runs)")
-    ap.add_argument("--precision", default="fp64", choices=list(PEAK_TFLOPS))
+    ap.add_argument("--precision", default="mixed_fp32", choices=list(PEAK_TFLOPS),
+                    help="sweep reported as `value`; the FP64 sweep is always measured beside it")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -77,57 +117,51 @@ def main():
     params = workload_params(args)
     plan = binding.Plan(params, batch_max=args.batch, device=0)
     n, nx, B = plan.nmu, plan.nx, args.batch
-    out = torch.empty(B * plan.neigv, dtype=torch.float64, device="cuda:0")
+    flops_real = 16.0 * n ** 3 * (nx - 1) + 8.0 * n ** 3
     status = torch.zeros(B, dtype=torch.int32, device="cuda:0")
-
-    for w in range(args.warmup):
-        plan.tdistrib_device(1 + w * B, B, args.precision, out, status)
-    plan.synchronize()
-    torch.cuda.synchronize()
-
-    plan.profile(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        plan.tdistrib_device(1 + (args.warmup + s) * B, B, args.precision, out, status)
-    plan.synchronize()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    times = plan.kernel_times()
-    plan.profile(False)
-    bad = int((status != 0).sum().item())
-
-    flops_slice = 16.0 * n ** 3
-    flops_real = flops_slice * (nx - 1) + 8.0 * n ** 3
-    value = args.steps * B / elapsed
-    # dominant kernel and its roofline: algorithmic flops per launch / mean launch duration
-    dom = max(("slice_inverse", "dyson_gemm"), key=lambda k: times.get(k, {}).get("ms", 0.0))
-    launches = max(1, times[dom]["launches"])
-    avg_ms = times[dom]["ms"] / launches
-    per_launch = 8.0 * n ** 3 * B
-    achieved = per_launch / (avg_ms * 1e-3) / 1e12
-    peak = PEAK_TFLOPS[args.precision]
+    runs = {}
+    order = ["fp64"] + ([args.precision] if args.precision != "fp64" else [])
+    outs = {}
+    for prec in order:
+        outs[prec] = torch.empty(B * plan.neigv, dtype=torch.float64, device="cuda:0")
+        el, times, bad = timed_sweeps(plan, prec, B, args.steps, args.warmup, outs[prec], status)
+        dom = max(("slice_inverse", "dyson_gemm"), key=lambda k: times.get(k, {}).get("ms", 0.0))
+        launches = max(1, times[dom]["launches"])
+        avg_ms = times[dom]["ms"] / launches
+        per_launch = 8.0 * n ** 3 * B
+        runs[prec] = {"value": args.steps * B / el, "ms_per_step": el / args.steps * 1e3, "failed_realizations": bad,
+                      "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in times.items() if v["launches"]},
+                      "dominant": dom, "avg_launch_ms": avg_ms,
+                      "achieved_tflops": per_launch / (avg_ms * 1e-3) / 1e12}
+        print(f"[bench] {prec}: {runs[prec]['value']:.1f} realizations/s, {runs[prec]['ms_per_step']:.0f} ms/step",
+              file=sys.stderr, flush=True)
+    main_prec = order[-1]
+    r = runs[main_prec]
+    peak = PEAK_TFLOPS[main_prec]
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(f"{dom}:{args.precision}")
-
+        traffic = json.load(open(tpath)).get(f"{r['dominant']}:{main_prec}")
     line = {
         "metric": "realizations/s at N=128 modes, mixed vs FP64 sweep; max rho(T) bin error",
-        "value": value, "unit": "realizations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": DTYPE[args.precision], "data": "synthetic",
-        "config": {"workload": f"dirichlet W=64.25 N={n} L=50 nx={nx} ny={plan.ny} ls=10 eps=0, "
-                               f"batch {B} realizations, precision {args.precision}",
+        "value": r["value"], "unit": "realizations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": DTYPE[main_prec], "data": "synthetic",
+        "config": {"workload": f"dirichlet W=64.25 N={n} L={nx * 0.05:g} nx={nx} ny={plan.ny} ls=10 eps=0, "
+                               f"batch {B} realizations, precision {main_prec}",
                    "global_batch": B, "parallelism": "realizations"},
-        "slice_updates_per_s": value * (nx - 1),
-        "algorithmic_tflops": value * flops_real / 1e12,
-        "failed_realizations": bad,
-        "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in times.items() if v["launches"]},
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic,
-                     "per_launch_flops": per_launch, "avg_launch_ms": avg_ms},
+        "slice_updates_per_s": r["value"] * (nx - 1),
+        "algorithmic_tflops": r["value"] * flops_real / 1e12,
+        "fp64_sweep": runs["fp64"],
+        "speedup_vs_fp64": r["value"] / runs["fp64"]["value"],
+        "roofline": {"bound": "mfma", "kernel": r["dominant"], "achieved": r["achieved_tflops"], "peak": peak,
+                     "unit": "TFLOP/s", "frac": r["achieved_tflops"] / peak, "traffic": traffic,
+                     "per_launch_flops": 8.0 * n ** 3 * B, "avg_launch_ms": r["avg_launch_ms"]},
+        "kernel_ms_per_step": r["kernel_ms_per_step"],
+        "failed_realizations": r["failed_realizations"],
     }
+    if main_prec != "fp64":
+        line["accuracy_vs_fp64"] = accuracy(plan, outs["fp64"], outs[main_prec])
     plan.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(params, args.cpu_seconds)

@@ -93,6 +93,7 @@ struct rg_plan {
     int hist_cap = 0;
 
     bool profiling = false;
+    bool fp64_blocked = false;  // FP64 inversion on the blocked MFMA kernel
     rg_timer timers[RGK_COUNT];
 };
 

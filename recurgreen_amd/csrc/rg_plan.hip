@@ -199,7 +199,8 @@ void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int prec
         const bool dis = id >= 1 && id <= P->nxdiso;
         if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
         const int slot = ix == 1 ? 0 : (ix == nx ? 2 : 1);
-        rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, precision);
+        if (precision == RG_FP64 && !P->fp64_blocked) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
+        else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, precision);
         hipEvent_t stop = nullptr;
         if (precision == RG_FP64) {
             if (ix == 1) {
@@ -308,6 +309,9 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
     P->device = device;
     P->batch_max = batch_max;
     for (int i = 0; i < RGK_COUNT; i++) P->timers[i].name = kTimerNames[i];
+    // FP64 slice-inversion kernel: register Gauss-Jordan (default) or blocked MFMA ("blocked").
+    const char *inv = std::getenv("RG_FP64_INVERSE");
+    P->fp64_blocked = inv && std::strcmp(inv, "blocked") == 0;
     rg_status st = init_modes(P);  // argument checks first, like init_waveguide
     if (st) { delete P; return st; }
     int ndev = 0;

@@ -44,7 +44,10 @@ CASES = {
     "N37_absorb": dict(boundtype="dirichlet", nxdiso=60, wol=18.7, dscat=3.0, dabso=0.5, ny=80),
     "N64": dict(boundtype="dirichlet", nxdiso=40, wol=32.25, aratio=20 / 32.25, dscat=1.0, ny=645, dos_model=1),
     "N100_filtered": dict(boundtype="dirichlet", nxdiso=30, wol=50.25, dscat=1.0, napera=0.6, naperb=0.9),
-    "N128": dict(boundtype="dirichlet", nxdiso=30, wol=64.25, aratio=50 / 64.25, dscat=5.0, ny=1285, dos_model=1),
+    "N128": dict(boundtype="dirichlet", nxdiso=40, wol=64.25, aratio=2.0 / 64.25, dscat=1.0, ny=1285, dos_model=1),
+    # fewer slices than the Shannon limit (k dx > pi): B = mu / sin(mu k dx) is large, slices ill-conditioned
+    "N128_coarse": dict(boundtype="dirichlet", nxdiso=30, wol=64.25, aratio=50 / 64.25, dscat=5.0, ny=1285,
+                        dos_model=1),
     "periodic_N11": dict(boundtype="periodic", nxdiso=80, nxfree=5, wol=4.3, exfac=1.2, dscat=0.5),
     "periodic_N41_ny60": dict(boundtype="periodic", nxdiso=50, wol=20.3, dscat=2.0, ny=60, napera=0.5),
 }
