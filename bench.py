@@ -93,6 +93,21 @@ def accuracy(plan, t_ref, t_mix):
     return out
 
 
+def copy_calibration(mib):
+    """Plain device-to-device copies of `mib` MiB (torch's copy kernel): the reference kernel whose
+    FETCH_SIZE / WRITE_SIZE counters scale the sweep kernels' counters to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    src = torch.randn(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8,
+                      "launches": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,11 +119,16 @@ def main():
                     help="sweep reported as `value`; the FP64 sweep is always measured beside it")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run plain copies of MIB MiB (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (the sweep shards by realization; see DESIGN.md); "
-              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        print(f"bench.py measures one GPU only (realizations are independent and would shard by seed "
+              f"range across GPUs); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import torch
 

@@ -11,27 +11,46 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librecurgreen_hip.so")
-SOURCES = ["rg_plan.hip", "rg_kernels_f64.hip", "rg_blocked.hip"]
+SOURCES = ["rg_plan.hip", "rg_kernels_f64.hip", "rg_blocked.hip", "rg_gj.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops is switched off for every HIP object: the v_pk_*_f32 forms are avoided.
+ARCH = "gfx950"
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    "-I", os.path.join(HERE, "..", "include"),
 ]
 
 
-def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+def build(verbose=False):
+    """Compile every HIP source for gfx950 (one hipcc per file, in parallel) and link the
+    shared library in-tree; raises on failure."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    odir = os.path.join(HERE, "_build")
+    os.makedirs(odir, exist_ok=True)
+    objs = [os.path.join(odir, os.path.splitext(f)[0] + ".o") for f in SOURCES]
+
+    def compile_one(i):
+        cmd = [HIPCC, f"--offload-arch={ARCH}", *FLAGS, "-c", os.path.join(CSRC, SOURCES[i]), "-o", objs[i]]
+        if verbose:
+            print(" ".join(cmd))
+        return cmd, subprocess.run(cmd, capture_output=True, text=True)
+
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
+        results = list(ex.map(compile_one, range(len(SOURCES))))
+    for cmd, res in results:
+        if res.returncode != 0:
+            sys.stderr.write(" ".join(cmd) + "\n" + res.stdout + res.stderr)
+            raise RuntimeError(f"hipcc failed compiling {os.path.basename(cmd[-3])}")
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-I", os.path.join(HERE, "..", "include"), "-o", tmp] + srcs
-    if verbose:
-        print(" ".join(cmd), flush=True)
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building librecurgreen_hip.so")
+        sys.stderr.write(" ".join(cmd) + "\n" + res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed linking librecurgreen_hip.so")
     os.replace(tmp, LIB)
     return LIB
 

@@ -65,6 +65,8 @@ struct rg_timer {
 
 enum rg_kernel_id { RGK_PROJECT = 0, RGK_INVERSE, RGK_GEMM, RGK_SVD, RGK_HIST, RGK_COPY, RGK_COUNT };
 
+enum rg_inverse_kernel { RG_INV_REGISTER = 0, RG_INV_LOOKAHEAD = 1, RG_INV_BLOCKED = 2 };
+
 struct rg_plan {
     rg_params p{};
     int nmu = 0, ny = 0, nx = 0, neigv = 0, imin = 0, imax = 0, jmin = 0, jmax = 0, nprop = 0;
@@ -93,7 +95,8 @@ struct rg_plan {
     int hist_cap = 0;
 
     bool profiling = false;
-    bool fp64_blocked = false;  // FP64 inversion on the blocked MFMA kernel
+    // slice-inversion kernel per state precision (RG_INV_*): FP64 state, complex64 state
+    int inv_fp64 = 0, inv_mixed = 1;
     rg_timer timers[RGK_COUNT];
 };
 
@@ -105,6 +108,8 @@ void rg_launch_fisher_lee_svd_f64(rg_plan *P, int nb, const double2 *gs1, double
                                   double2 *tmat_out);
 void rg_launch_fisher_lee_svd_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status,
                                   double2 *tmat_out);
+// rg_gj.hip
+void rg_launch_inverse_gj(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
 // rg_blocked.hip
 void rg_launch_inverse_blocked(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
 void rg_launch_gemm_sp(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new, int precision);

@@ -199,7 +199,9 @@ void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int prec
         const bool dis = id >= 1 && id <= P->nxdiso;
         if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
         const int slot = ix == 1 ? 0 : (ix == nx ? 2 : 1);
-        if (precision == RG_FP64 && !P->fp64_blocked) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
+        const int inv = precision == RG_FP64 ? P->inv_fp64 : P->inv_mixed;
+        if (inv == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
+        else if (inv == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision);
         else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, precision);
         hipEvent_t stop = nullptr;
         if (precision == RG_FP64) {
@@ -224,6 +226,15 @@ void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int prec
     }
     if (precision == RG_FP64) rg_launch_fisher_lee_svd_f64(P, nb, P->d_gs1[cur], teig, P->d_status, tmat_out);
     else rg_launch_fisher_lee_svd_f32(P, nb, P->d_gs1_32[cur], teig, P->d_status, tmat_out);
+}
+
+int inverse_choice(const char *v, int dflt, bool fp64)
+{
+    if (!v) return dflt;
+    if (fp64 && std::strcmp(v, "register") == 0) return RG_INV_REGISTER;
+    if (std::strcmp(v, "lookahead") == 0) return RG_INV_LOOKAHEAD;
+    if (std::strcmp(v, "blocked") == 0) return RG_INV_BLOCKED;
+    return dflt;
 }
 
 rg_status check_common(rg_plan *P, int32_t n, int32_t precision)
@@ -309,9 +320,10 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
     P->device = device;
     P->batch_max = batch_max;
     for (int i = 0; i < RGK_COUNT; i++) P->timers[i].name = kTimerNames[i];
-    // FP64 slice-inversion kernel: register Gauss-Jordan (default) or blocked MFMA ("blocked").
-    const char *inv = std::getenv("RG_FP64_INVERSE");
-    P->fp64_blocked = inv && std::strcmp(inv, "blocked") == 0;
+    // Slice-inversion kernels, overridable for measurement: RG_INVERSE_FP64 in {register,
+    // lookahead, blocked}, RG_INVERSE_MIXED in {lookahead, blocked}.
+    P->inv_fp64 = inverse_choice(std::getenv("RG_INVERSE_FP64"), P->inv_fp64, true);
+    P->inv_mixed = inverse_choice(std::getenv("RG_INVERSE_MIXED"), P->inv_mixed, false);
     rg_status st = init_modes(P);  // argument checks first, like init_waveguide
     if (st) { delete P; return st; }
     int ndev = 0;

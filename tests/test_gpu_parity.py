@@ -42,7 +42,7 @@ CASES = {
     "c1_N20_ny205": dict(boundtype="dirichlet", nxdiso=200, wol=10.25, aratio=10 / 10.25, dscat=2.0, ny=205,
                          dos_model=1),
     "N37_absorb": dict(boundtype="dirichlet", nxdiso=60, wol=18.7, dscat=3.0, dabso=0.5, ny=80),
-    "N64": dict(boundtype="dirichlet", nxdiso=40, wol=32.25, aratio=20 / 32.25, dscat=1.0, ny=645, dos_model=1),
+    "N64": dict(boundtype="dirichlet", nxdiso=40, wol=32.25, aratio=2.0 / 32.25, dscat=1.0, ny=645, dos_model=1),
     "N100_filtered": dict(boundtype="dirichlet", nxdiso=30, wol=50.25, dscat=1.0, napera=0.6, naperb=0.9),
     "N128": dict(boundtype="dirichlet", nxdiso=40, wol=64.25, aratio=2.0 / 64.25, dscat=1.0, ny=1285, dos_model=1),
     # fewer slices than the Shannon limit (k dx > pi): B = mu / sin(mu k dx) is large, slices ill-conditioned
