@@ -261,7 +261,7 @@ __device__ __attribute__((always_inline)) inline void panel_factor(ACC (&ar)[WV]
 
 template <int WV, int PREC>
 __global__ __launch_bounds__(64 * WV, PREC == RG_FP64 ? 2 : 4) void k_inverse_blocked(
-    typename PT<PREC>::S *__restrict__ gss, const double *__restrict__ cq, int ntab, int boundtype,
+    typename PT<PREC>::S *__restrict__ gss, const rg_cqv cqv, int boundtype,
     const double2 *__restrict__ adiag, const double2 *__restrict__ bvec, int n, int first, int32_t *__restrict__ status)
 {
     constexpr int NP = 16 * WV, NT = 64 * WV;
@@ -276,8 +276,7 @@ __global__ __launch_bounds__(64 * WV, PREC == RG_FP64 ? 2 : 4) void k_inverse_bl
     const int b = blockIdx.x;
     S *G = gss + (size_t)b * NP * NP;
 
-    if (cq)
-        for (int q = tid; q < ntab; q += NT) s_cq[q] = cq[(size_t)b * ntab + q];
+    if (cqv.cq) rg_load_cq(s_cq, cqv, b, tid, NT);
     for (int i = tid; i < NP; i += NT) sh.piv[i] = 0;
     if (tid == 0) sh.sing = 0;
     __syncthreads();
@@ -293,7 +292,7 @@ __global__ __launch_bounds__(64 * WV, PREC == RG_FP64 ? 2 : 4) void k_inverse_bl
             double2 m;
             if (row < n && col < n) {
                 m = row == col ? adiag[row] : cmk(0.0, 0.0);
-                if (cq) m = csub(m, slice_v2(s_cq, boundtype, row, col));
+                if (cqv.cq) m = csub(m, slice_v2(s_cq, boundtype, row, col));
                 if (!first) m = csub(m, cmul(cmul(bvec[row], ld2(G + (size_t)row * NP + col)), bvec[col]));
             } else {
                 m = cmk(row == col ? 1.0 : 0.0, 0.0);
@@ -526,14 +525,14 @@ __global__ __launch_bounds__(256) void k_copy_f2(const float2 *__restrict__ src,
 // Launchers.
 // ------------------------------------------------------------------------------------------------
 template <int PREC>
-static void launch_inverse_blocked(rg_plan *P, int nb, void *gss, const double *cq, const double2 *ad, int first)
+static void launch_inverse_blocked(rg_plan *P, int nb, void *gss, const rg_cqv cq, const double2 *ad, int first)
 {
     using S = typename PT<PREC>::S;
     S *g = reinterpret_cast<S *>(gss);
     switch (P->np / 16) {
 #define RG_BINV(W)                                                                                          \
     case W:                                                                                                 \
-        hipLaunchKernelGGL((k_inverse_blocked<W, PREC>), dim3(nb), dim3(64 * W), 0, P->stream, g, cq, P->ntab, \
+        hipLaunchKernelGGL((k_inverse_blocked<W, PREC>), dim3(nb), dim3(64 * W), 0, P->stream, g, cq, \
                            P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status);                        \
         break;
         RG_BINV(1)
@@ -553,7 +552,7 @@ void rg_launch_inverse_blocked(rg_plan *P, int nb, int slot, int first, bool wit
 {
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_INVERSE, &stop);
-    const double *cq = with_disorder ? P->d_cq : nullptr;
+    const rg_cqv cq = rg_cq_view(P, nb, with_disorder);
     const double2 *ad = P->d_adiag + (size_t)slot * P->np;
     if (precision == RG_FP64) launch_inverse_blocked<RG_FP64>(P, nb, P->d_gss, cq, ad, first);
     else if (precision == RG_MIXED_FP32) launch_inverse_blocked<RG_MIXED_FP32>(P, nb, P->d_gss32, cq, ad, first);

@@ -136,7 +136,7 @@ __device__ __attribute__((always_inline)) inline void pivot_column(C (&a)[TR][4]
 
 template <int TR, class T>
 __global__ __launch_bounds__(64 * TR, sizeof(T) == 8 ? 2 : 4) void k_gj_lookahead(
-    typename CT<T>::C *__restrict__ gss, const double *__restrict__ cq, int ntab, int boundtype,
+    typename CT<T>::C *__restrict__ gss, const rg_cqv cqv, int boundtype,
     const double2 *__restrict__ adiag, const double2 *__restrict__ bvec, int n, int first, int32_t *__restrict__ status)
 {
     using C = typename CT<T>::C;
@@ -155,8 +155,7 @@ __global__ __launch_bounds__(64 * TR, sizeof(T) == 8 ? 2 : 4) void k_gj_lookahea
     const int r = tid & 15, c = tid >> 4, col0 = c * TC;
     C *G = gss + (size_t)b * NP * NP;
 
-    if (cq)
-        for (int q = tid; q < ntab; q += NT) s_cq[q] = cq[(size_t)b * ntab + q];
+    if (cqv.cq) rg_load_cq(s_cq, cqv, b, tid, NT);
     if (tid == 0) s_sing = 0;
     __syncthreads();
 
@@ -171,7 +170,7 @@ __global__ __launch_bounds__(64 * TR, sizeof(T) == 8 ? 2 : 4) void k_gj_lookahea
             double2 m;
             if (gi < n && gj < n) {
                 m = gi == gj ? adiag[gi] : cmk(0.0, 0.0);
-                if (cq) m = csub(m, slice_v3(s_cq, boundtype, gi, gj));
+                if (cqv.cq) m = csub(m, slice_v3(s_cq, boundtype, gi, gj));
                 if (!first) m = csub(m, cmul(cmul(bvec[gi], ldc2(G + (size_t)gi * NP + gj)), bvec[gj]));
             } else {
                 m = cmk(gi == gj ? 1.0 : 0.0, 0.0);
@@ -259,12 +258,12 @@ __global__ __launch_bounds__(64 * TR, sizeof(T) == 8 ? 2 : 4) void k_gj_lookahea
 }  // namespace
 
 template <class T>
-static void launch_gj(rg_plan *P, int nb, typename CT<T>::C *g, const double *cq, const double2 *ad, int first)
+static void launch_gj(rg_plan *P, int nb, typename CT<T>::C *g, const rg_cqv cq, const double2 *ad, int first)
 {
     switch (P->np / 16) {
 #define RG_GJ(W)                                                                                                \
     case W:                                                                                                     \
-        hipLaunchKernelGGL((k_gj_lookahead<W, T>), dim3(nb), dim3(64 * W), 0, P->stream, g, cq, P->ntab,          \
+        hipLaunchKernelGGL((k_gj_lookahead<W, T>), dim3(nb), dim3(64 * W), 0, P->stream, g, cq,                   \
                            P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status);                            \
         break;
         RG_GJ(1)
@@ -284,7 +283,7 @@ void rg_launch_inverse_gj(rg_plan *P, int nb, int slot, int first, bool with_dis
 {
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_INVERSE, &stop);
-    const double *cq = with_disorder ? P->d_cq : nullptr;
+    const rg_cqv cq = rg_cq_view(P, nb, with_disorder);
     const double2 *ad = P->d_adiag + (size_t)slot * P->np;
     if (precision == RG_FP64) launch_gj<double>(P, nb, P->d_gss, cq, ad, first);
     else launch_gj<float>(P, nb, P->d_gss32, cq, ad, first);

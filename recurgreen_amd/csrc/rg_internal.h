@@ -65,6 +65,23 @@ struct rg_timer {
 
 enum rg_kernel_id { RGK_PROJECT = 0, RGK_INVERSE, RGK_GEMM, RGK_SVD, RGK_HIST, RGK_COPY, RGK_COUNT };
 
+// Slice coefficients as the projection leaves them: `parts` partial sums over disjoint ranges of
+// transverse cells, cq[(part * nb + b) * ntab + q], to be added in part order and scaled by 1/ny.
+struct rg_cqv {
+    const double *cq;  // null: no disorder on this slice
+    int ntab, nb, parts;
+    double scale;
+};
+
+__device__ inline void rg_load_cq(double *s_cq, const rg_cqv &v, int b, int tid, int nthreads)
+{
+    for (int q = tid; q < v.ntab; q += nthreads) {
+        double s = 0.0;
+        for (int k = 0; k < v.parts; k++) s += v.cq[((size_t)k * v.nb + b) * v.ntab + q];
+        s_cq[q] = s * v.scale;
+    }
+}
+
 enum rg_inverse_kernel { RG_INV_REGISTER = 0, RG_INV_LOOKAHEAD = 1, RG_INV_BLOCKED = 2 };
 
 struct rg_plan {
@@ -74,6 +91,9 @@ struct rg_plan {
     double kl = 0, kdx = 0, dos = 0, stdu = 0;
     int np = 0;        // padded matrix order (multiple of 16)
     int ntab = 0;      // projection coefficients per slice (doubles)
+    int ntp = 0;       // ntab padded to a multiple of 16 (table row length)
+    int nchunk = 0;    // transverse cells in chunks of 16 (table rows = 16 * nchunk, zero padded)
+    int cq_parts = 1;  // partial sums of the projection (transverse-cell split)
     int device = 0;
     hipStream_t stream = nullptr;
     int batch_max = 0;
@@ -84,7 +104,7 @@ struct rg_plan {
 
     // device constants
     double2 *d_b = nullptr, *d_adiag = nullptr, *d_sqrtmu = nullptr;  // d_adiag: 3*np (bound, bulk, last)
-    double *d_tab = nullptr;                                           // ny x ntab
+    double *d_tab = nullptr;                                           // (16 nchunk) x ntp, zero padded
     // batch buffers
     double2 *d_gss = nullptr, *d_gs1[2] = {nullptr, nullptr}, *d_tmat = nullptr;
     float2 *d_gss32 = nullptr, *d_gs1_32[2] = {nullptr, nullptr};  // FP32 state of the mixed sweeps
@@ -97,17 +117,25 @@ struct rg_plan {
     bool profiling = false;
     // slice-inversion kernel per state precision (RG_INV_*): FP64 state, complex64 state
     int inv_fp64 = 0, inv_mixed = 1;
+    bool eig_jacobi = false;  // transmission eigenvalues by one-sided Jacobi SVD of t instead of eig(t^H t)
     rg_timer timers[RGK_COUNT];
 };
 
-// Kernel launchers (rg_kernels_f64.hip).
+// rg_project.hip
 void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double *d_upot, int upot_nreal);
+int rg_project_parts(int ny);
+bool rg_project_supported(int ntab);
+// Kernel launchers (rg_kernels_f64.hip).
 void rg_launch_inverse_f64(rg_plan *P, int nb, int slot, int first, bool with_disorder);
 void rg_launch_gemm_f64(rg_plan *P, int nb, const double2 *gss, const double2 *gs1_old, double2 *gs1_new);
 void rg_launch_fisher_lee_svd_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status,
                                   double2 *tmat_out);
 void rg_launch_fisher_lee_svd_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status,
                                   double2 *tmat_out);
+// rg_eig.hip
+void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status);
+void rg_launch_tdt_eig_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status);
+bool rg_eig_supported(const rg_plan *P);
 // rg_gj.hip
 void rg_launch_inverse_gj(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
 // rg_blocked.hip
@@ -117,6 +145,8 @@ void rg_launch_copy_f2(rg_plan *P, const float2 *src, float2 *dst, size_t n);
 void rg_launch_histogram(rg_plan *P, const double *d_samples, int64_t n, const double *d_bins, int nbins,
                          unsigned long long *d_counts);
 void rg_launch_potential(rg_plan *P, int nb, int64_t seed0, double *d_upot);
+
+rg_cqv rg_cq_view(const rg_plan *P, int nb, bool with_disorder);
 
 // Timing helpers (rg_plan.hip).
 void rg_timer_begin(rg_plan *P, int id, hipEvent_t *stop_out);

@@ -2,7 +2,6 @@
 //
 // One realization of the disorder is one row of the batch.  Per slice ix the sweep of
 // src/waveguide.f90:431-440 runs as three launches over the whole batch:
-//   k_project        disorder slice -> transform coefficients C(q)       (:366-380)
 //   k_slice_inverse  Gss <- inv(A_ix - V_ix - B Gss B)                   (:434-435)
 //   k_dyson_gemm     Gs1 <- -Gss B Gs1                                   (:438)
 // and once per realization
@@ -11,73 +10,6 @@
 #include "rg_internal.h"
 
 namespace {
-
-// ------------------------------------------------------------------------------------------------
-// Disorder slice projection.  C(q) = (1/ny) sum_i u_i tab[i][q] for the ntab table columns:
-// Dirichlet tab[i][q] = cos(q pi y_i), q = 0..2N  (V_mn = C(|m-n|) - C(m+n));
-// periodic  tab[i][2q] = cos(2 pi q y_i), tab[i][2q+1] = sin(2 pi q y_i), q = 0..N-1
-// (V_mn = D(n-m), D(q) = C(2q) + i C(2q+1)).  The slice u_i is either generated (Philox,
-// realization seed seed0 + b) or read from an explicit potential batch upot(ix, iy, b).
-// ------------------------------------------------------------------------------------------------
-// Tiling: a workgroup owns PRB realizations and all table columns; the i range advances in
-// chunks of PKI staged through LDS (the generated or loaded slice values and the table rows), and
-// each thread accumulates 4 realizations x 5 columns in registers, in a fixed order.
-constexpr int PRB = 16, PKI = 16, PNQ = 320;
-
-__global__ __launch_bounds__(256) void k_project(int ixd, int64_t seed0, const double *__restrict__ upot,
-                                                 int nxdiso, int ny, double stdu,
-                                                 const double *__restrict__ tab, int ntab,
-                                                 double *__restrict__ cq, int nb)
-{
-    __shared__ double s_u[PKI][PRB + 1];
-    __shared__ double s_t[PKI][PNQ];
-    const int tid = threadIdx.x, b0 = blockIdx.x * PRB;
-    const int tq = tid & 63, tr = tid >> 6;
-    double acc[4][5];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int j = 0; j < 5; j++) acc[a][j] = 0.0;
-    for (int i0 = 0; i0 < ny; i0 += PKI) {
-        {
-            const int rr = tid >> 4, ii = tid & 15, bb = b0 + rr, iy = i0 + ii;
-            double u = 0.0;
-            if (bb < nb && iy < ny) {
-                if (upot) u = upot[(size_t)bb * nxdiso * ny + (size_t)iy * nxdiso + ixd];
-                else u = stdu * rg_normal_cell((uint64_t)(seed0 + bb), 0u, ixd, iy);
-            }
-            s_u[ii][rr] = u;
-        }
-        for (int e = tid; e < PKI * PNQ; e += 256) {
-            const int ii = e / PNQ, q = e % PNQ, iy = i0 + ii;
-            s_t[ii][q] = (iy < ny && q < ntab) ? tab[(size_t)iy * ntab + q] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int ii = 0; ii < PKI; ii++) {
-            double uu[4], tt[5];
-#pragma unroll
-            for (int a = 0; a < 4; a++) uu[a] = s_u[ii][4 * tr + a];
-#pragma unroll
-            for (int j = 0; j < 5; j++) tt[j] = s_t[ii][tq + 64 * j];
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int j = 0; j < 5; j++) acc[a][j] = fma(uu[a], tt[j], acc[a][j]);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-        const int bb = b0 + 4 * tr + a;
-        if (bb >= nb) continue;
-#pragma unroll
-        for (int j = 0; j < 5; j++) {
-            const int q = tq + 64 * j;
-            if (q < ntab) cq[(size_t)bb * ntab + q] = acc[a][j] / ny;
-        }
-    }
-}
 
 __global__ __launch_bounds__(256) void k_potential(int64_t seed0, int nxdiso, int ny, double stdu,
                                                    double *__restrict__ upot)
@@ -200,8 +132,8 @@ __device__ inline void gj_step(double2 (&a)[TR][TC], const int k, const int jj, 
 }
 
 template <int TR>
-__global__ __launch_bounds__(64 * TR) void k_slice_inverse(double2 *__restrict__ gss, const double *__restrict__ cq,
-                                                             int ntab, int boundtype,
+__global__ __launch_bounds__(64 * TR) void k_slice_inverse(double2 *__restrict__ gss, const rg_cqv cqv,
+                                                             int boundtype,
                                                              const double2 *__restrict__ adiag,
                                                              const double2 *__restrict__ bvec, int n, int first,
                                                              int32_t *__restrict__ status)
@@ -224,8 +156,7 @@ __global__ __launch_bounds__(64 * TR) void k_slice_inverse(double2 *__restrict__
     const int col0 = c * TC;
     double2 *G = gss + (size_t)b * NP * NP;
 
-    if (cq)
-        for (int q = tid; q < ntab; q += NT) s_cq[q] = cq[(size_t)b * ntab + q];
+    if (cqv.cq) rg_load_cq(s_cq, cqv, b, tid, NT);
     if (tid == 0) s_sing = 0;
     __syncthreads();
 
@@ -240,7 +171,7 @@ __global__ __launch_bounds__(64 * TR) void k_slice_inverse(double2 *__restrict__
             double2 m;
             if (gi < n && gj < n) {
                 m = gi == gj ? adiag[gi] : cmk(0.0, 0.0);
-                if (cq) m = csub(m, slice_v(s_cq, boundtype, gi, gj));
+                if (cqv.cq) m = csub(m, slice_v(s_cq, boundtype, gi, gj));
                 if (!first) m = csub(m, cmul(cmul(bvec[gi], G[(size_t)gi * NP + gj]), bvec[gj]));
             } else {
                 m = cmk(gi == gj ? 1.0 : 0.0, 0.0);
@@ -614,16 +545,6 @@ __global__ __launch_bounds__(256) void k_histogram(const double *__restrict__ s,
 // ------------------------------------------------------------------------------------------------
 // Launchers.
 // ------------------------------------------------------------------------------------------------
-void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double *d_upot, int upot_nreal)
-{
-    (void)upot_nreal;
-    hipEvent_t stop = nullptr;
-    rg_timer_begin(P, RGK_PROJECT, &stop);
-    hipLaunchKernelGGL(k_project, dim3((nb + PRB - 1) / PRB), dim3(256), 0, P->stream, ixd, seed0, d_upot, P->nxdiso,
-                       P->ny, P->stdu, P->d_tab, P->ntab, P->d_cq, nb);
-    rg_timer_end(P, RGK_PROJECT, stop);
-}
-
 void rg_launch_potential(rg_plan *P, int nb, int64_t seed0, double *d_upot)
 {
     const int64_t ncell = (int64_t)P->nxdiso * P->ny;
@@ -636,12 +557,12 @@ void rg_launch_inverse_f64(rg_plan *P, int nb, int slot, int first, bool with_di
 {
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_INVERSE, &stop);
-    const double *cq = with_disorder ? P->d_cq : nullptr;
+    const rg_cqv cq = rg_cq_view(P, nb, with_disorder);
     const double2 *ad = P->d_adiag + (size_t)slot * P->np;
     const int TR = P->np / 16;
 #define RG_INV_CASE(T)                                                                                    \
     case T:                                                                                               \
-        hipLaunchKernelGGL(k_slice_inverse<T>, dim3(nb), dim3(64 * T), 0, P->stream, P->d_gss, cq, P->ntab, \
+        hipLaunchKernelGGL(k_slice_inverse<T>, dim3(nb), dim3(64 * T), 0, P->stream, P->d_gss, cq, \
                            P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status);                     \
         break;
     switch (TR) {

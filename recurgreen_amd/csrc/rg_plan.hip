@@ -122,6 +122,9 @@ rg_status init_modes(rg_plan *P)
     P->dos = p.dos_model == 1 ? 1.0 / (4.0 * kPi) : dsum / (p.wol * (2.0 * kPi) * (2.0 * kPi));
     P->stdu = std::sqrt(p.dscat * P->ny / (2.0 * kPi * kPi * P->dos * p.wol * p.nxdiso));
     P->ntab = p.boundtype == RG_DIRICHLET ? 2 * n + 1 : 2 * n;
+    P->ntp = (P->ntab + 15) / 16 * 16;
+    P->nchunk = (P->ny + 15) / 16;
+    P->cq_parts = rg_project_parts(P->ny);
     return RG_OK;
 }
 
@@ -136,21 +139,24 @@ rg_status upload_constants(rg_plan *P)
         // slice 2 after copying the boundary (src/waveguide.f90:359-360).
         ad[2 * np + m] = P->nx == 2 ? cadd(P->h_abound[m], P->h_abulk[m]) : P->h_abound[m];
     }
-    std::vector<double> tab((size_t)ny * nt);
+    // projection table, rows = transverse cells (zero rows up to whole 16-cell chunks), columns
+    // padded to whole 16-column tiles
+    const int ntp = P->ntp, rows = 16 * P->nchunk;
+    std::vector<double> tab((size_t)rows * ntp, 0.0);
     for (int i = 0; i < ny; i++) {
         const double y = (i + 1 - 0.5) / ny;
         if (P->p.boundtype == RG_DIRICHLET) {
-            for (int q = 0; q < nt; q++) tab[(size_t)i * nt + q] = std::cos(q * kPi * y);
+            for (int q = 0; q < nt; q++) tab[(size_t)i * ntp + q] = std::cos(q * kPi * y);
         } else {
             for (int q = 0; q < nt / 2; q++) {
-                tab[(size_t)i * nt + 2 * q] = std::cos(2.0 * kPi * q * y);
-                tab[(size_t)i * nt + 2 * q + 1] = std::sin(2.0 * kPi * q * y);
+                tab[(size_t)i * ntp + 2 * q] = std::cos(2.0 * kPi * q * y);
+                tab[(size_t)i * ntp + 2 * q + 1] = std::sin(2.0 * kPi * q * y);
             }
         }
     }
     rg_status st;
     if ((st = dalloc(&P->d_b, np)) || (st = dalloc(&P->d_adiag, 3 * (size_t)np)) || (st = dalloc(&P->d_sqrtmu, np)) ||
-        (st = dalloc(&P->d_tab, (size_t)ny * nt)))
+        (st = dalloc(&P->d_tab, tab.size())))
         return st;
     HIPCHK(hipMemcpy(P->d_b, P->h_b.data(), sizeof(double2) * np, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(P->d_adiag, ad.data(), sizeof(double2) * ad.size(), hipMemcpyHostToDevice));
@@ -164,7 +170,7 @@ rg_status alloc_batch(rg_plan *P)
     const size_t B = (size_t)P->batch_max, mat = (size_t)P->np * P->np;
     rg_status st;
     if ((st = dalloc(&P->d_gss, B * mat)) || (st = dalloc(&P->d_gs1[0], B * mat)) || (st = dalloc(&P->d_gs1[1], B * mat)) ||
-        (st = dalloc(&P->d_tmat, B * mat)) || (st = dalloc(&P->d_cq, B * P->ntab)) ||
+        (st = dalloc(&P->d_tmat, B * mat)) || (st = dalloc(&P->d_cq, B * P->ntab * P->cq_parts)) ||
         (st = dalloc(&P->d_teig, B * P->neigv)) || (st = dalloc(&P->d_status, B)) ||
         (st = dalloc(&P->d_gss32, B * mat)) || (st = dalloc(&P->d_gs1_32[0], B * mat)) ||
         (st = dalloc(&P->d_gs1_32[1], B * mat)))
@@ -224,8 +230,14 @@ void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int prec
             }
         }
     }
-    if (precision == RG_FP64) rg_launch_fisher_lee_svd_f64(P, nb, P->d_gs1[cur], teig, P->d_status, tmat_out);
-    else rg_launch_fisher_lee_svd_f32(P, nb, P->d_gs1_32[cur], teig, P->d_status, tmat_out);
+    if (tmat_out || P->eig_jacobi || !rg_eig_supported(P)) {
+        if (precision == RG_FP64) rg_launch_fisher_lee_svd_f64(P, nb, P->d_gs1[cur], teig, P->d_status, tmat_out);
+        else rg_launch_fisher_lee_svd_f32(P, nb, P->d_gs1_32[cur], teig, P->d_status, tmat_out);
+    } else if (precision == RG_FP64) {
+        rg_launch_tdt_eig_f64(P, nb, P->d_gs1[cur], teig, P->d_status);
+    } else {
+        rg_launch_tdt_eig_f32(P, nb, P->d_gs1_32[cur], teig, P->d_status);
+    }
 }
 
 int inverse_choice(const char *v, int dflt, bool fp64)
@@ -324,6 +336,9 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
     // lookahead, blocked}, RG_INVERSE_MIXED in {lookahead, blocked}.
     P->inv_fp64 = inverse_choice(std::getenv("RG_INVERSE_FP64"), P->inv_fp64, true);
     P->inv_mixed = inverse_choice(std::getenv("RG_INVERSE_MIXED"), P->inv_mixed, false);
+    // Transmission eigenvalues: Hermitian eigensolver of t^H t (default) or RG_EIGEN=jacobi.
+    const char *eg = std::getenv("RG_EIGEN");
+    P->eig_jacobi = eg && std::strcmp(eg, "jacobi") == 0;
     rg_status st = init_modes(P);  // argument checks first, like init_waveguide
     if (st) { delete P; return st; }
     int ndev = 0;

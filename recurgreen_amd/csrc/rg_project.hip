@@ -1,0 +1,135 @@
+// rg_project.hip -- disorder slice projection on the FP64 matrix cores (gfx950).
+//
+// src/waveguide.f90:366-380 adds modal^H diag(u_x) modal to every disordered slice.  With sine
+// modes (Dirichlet) chi_m chi_n = (1/ny)[cos((m-n) pi y) - cos((m+n) pi y)], so the projected slice
+// is V_mn = C(|m-n|) - C(m+n+2) (0-based m, n) with C(q) = (1/ny) sum_i u_i cos(q pi y_i); with
+// plane waves (periodic) V_mn = D(n-m), D(q) = C(2q) + i C(2q+1) over a cos/sin table.  For the
+// whole batch and one slice this is the product  C[b][q] = sum_i U[b][i] tab[i][q]:
+// M = realizations, N = table columns (2N+1 or 2N), K = ny transverse cells.
+//
+// Each workgroup (4 wavefronts) owns 32 realizations (two 16-row MFMA tiles) and one contiguous
+// range of 16-cell chunks of K; wavefront w owns the table column tiles {w, w+4, ...}.  The slice
+// values U (Philox/Box-Muller generated here, or read from an explicit potential batch) are staged
+// through LDS one chunk ahead of the v_mfma_f64_16x16x4 products; table tiles stream from L2.
+// Partial sums of the K ranges go to cq[(part * nb + b) * ntab + q]; the slice-inversion kernels add
+// them in part order and scale by 1/ny, so the result does not depend on scheduling.
+//
+// MFMA lane layout (v_mfma_f64_16x16x4_f64, measured): A[i][k] in lane i + 16k, B[k][j] in lane
+// j + 16k, result register v of lane l holds D[4v + l/16][l%16].
+#include "rg_internal.h"
+
+namespace {
+
+constexpr int PR_RB = 32;     // realizations per workgroup
+constexpr int PR_KC = 16;     // transverse cells per staged chunk
+constexpr int PR_MAXNT = 20;  // table column tiles supported (ntab <= 320)
+constexpr int PR_TPW = (PR_MAXNT + 3) / 4;
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// One chunk of slice values into s_u[KC][RB]: two cells per thread.
+__device__ inline void stage_chunk(double (*s_u)[PR_RB + 1], int chunk, int b0, int nb, int ixd, int64_t seed0,
+                                   const double *__restrict__ upot, int nxdiso, int ny, double stdu, int tid)
+{
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int e = tid + 256 * h;
+        const int rr = e & (PR_RB - 1), ii = e >> 5;
+        const int b = b0 + rr, iy = chunk * PR_KC + ii;
+        double u = 0.0;
+        if (b < nb && iy < ny) {
+            if (upot) u = upot[(size_t)b * nxdiso * ny + (size_t)iy * nxdiso + ixd];
+            else u = stdu * rg_normal_cell((uint64_t)(seed0 + b), 0u, ixd, iy);
+        }
+        s_u[ii][rr] = u;
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void k_project_mfma(int ixd, int64_t seed0, const double *__restrict__ upot,
+                                                         int nxdiso, int ny, double stdu,
+                                                         const double *__restrict__ tab, int ntp, int ntab,
+                                                         int nchunk, int parts, double *__restrict__ cq, int nb)
+{
+    __shared__ double s_u[2][PR_KC][PR_RB + 1];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int b0 = blockIdx.x * PR_RB, part = blockIdx.y;
+    const int c0 = (int)((int64_t)part * nchunk / parts), c1 = (int)((int64_t)(part + 1) * nchunk / parts);
+    const int ntiles = ntp / 16;
+    const int li = lane & 15, lk = lane >> 4;
+
+    dbl4 acc[2][PR_TPW];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int j = 0; j < PR_TPW; j++) acc[m][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    if (c0 < c1) stage_chunk(s_u[0], c0, b0, nb, ixd, seed0, upot, nxdiso, ny, stdu, tid);
+    __syncthreads();
+    for (int c = c0; c < c1; c++) {
+        const int buf = (c - c0) & 1;
+        if (c + 1 < c1) stage_chunk(s_u[buf ^ 1], c + 1, b0, nb, ixd, seed0, upot, nxdiso, ny, stdu, tid);
+        const double *trow = tab + (size_t)(c * PR_KC + lk) * ntp + li;
+#pragma unroll
+        for (int kk = 0; kk < PR_KC / 4; kk++) {
+            const double a0 = s_u[buf][4 * kk + lk][li];
+            const double a1 = s_u[buf][4 * kk + lk][16 + li];
+#pragma unroll
+            for (int j = 0; j < PR_TPW; j++) {
+                const int nt = w + 4 * j;
+                if (nt < ntiles) {
+                    const double bv = trow[(size_t)(4 * kk) * ntp + 16 * nt];
+                    acc[0][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[0][j], 0, 0, 0);
+                    acc[1][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[1][j], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double *out = cq + (size_t)part * nb * ntab;
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int j = 0; j < PR_TPW; j++) {
+            const int nt = w + 4 * j;
+            if (nt >= ntiles) continue;
+            const int q = 16 * nt + li;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int b = b0 + 16 * m + 4 * v + lk;
+                if (b < nb && q < ntab) out[(size_t)b * ntab + q] = acc[m][j][v];
+            }
+        }
+}
+
+}  // namespace
+
+int rg_project_parts(int ny)
+{
+    const int nchunk = (ny + PR_KC - 1) / PR_KC;
+    int parts = (nchunk + 9) / 10;  // about ten chunks per workgroup
+    return parts < 1 ? 1 : (parts > 8 ? 8 : parts);
+}
+
+bool rg_project_supported(int ntab) { return (ntab + 15) / 16 <= PR_MAXNT; }
+
+void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double *d_upot, int upot_nreal)
+{
+    (void)upot_nreal;
+    hipEvent_t stop = nullptr;
+    rg_timer_begin(P, RGK_PROJECT, &stop);
+    hipLaunchKernelGGL(k_project_mfma, dim3((nb + PR_RB - 1) / PR_RB, P->cq_parts), dim3(256), 0, P->stream, ixd,
+                       seed0, d_upot, P->nxdiso, P->ny, P->stdu, P->d_tab, P->ntp, P->ntab, P->nchunk, P->cq_parts,
+                       P->d_cq, nb);
+    rg_timer_end(P, RGK_PROJECT, stop);
+}
+
+rg_cqv rg_cq_view(const rg_plan *P, int nb, bool with_disorder)
+{
+    rg_cqv v;
+    v.cq = with_disorder ? P->d_cq : nullptr;
+    v.ntab = P->ntab;
+    v.nb = nb;
+    v.parts = P->cq_parts;
+    v.scale = 1.0 / P->ny;
+    return v;
+}
