@@ -92,6 +92,15 @@ rg_status rg_tdistrib_potential(rg_plan *plan, const double *upot, int32_t nreal
  * complex each) for explicit host potentials; upot == NULL means no disorder. */
 rg_status rg_transmission_potential(rg_plan *plan, const double *upot, int32_t nreal, int32_t precision,
                                     double *tmat, int32_t *status);
+/* Mirrors compute_refl_trans_v1 (src/waveguide.f90:469-519) for seeds seed0 .. seed0+nseed-1:
+ * the reflection and transmission matrices r = -I + 2i s G11 s, t = 2i s G(L;0) s with
+ * s = sqrt(Re mu) (:508-512), full nmu x nmu, column-major complex, realization-major in rmat
+ * and tmat (host, nseed*nmu*nmu complex each).  Complex128 only (the sweep of :484-506 with G11
+ * and G1s carried beside Gss and Gs1). */
+rg_status rg_refl_trans(rg_plan *plan, int64_t seed0, int32_t nseed, double *rmat, double *tmat, int32_t *status);
+/* Same for explicit host potentials upot (nxdiso*ny*nreal doubles). */
+rg_status rg_refl_trans_potential(rg_plan *plan, const double *upot, int32_t nreal, double *rmat, double *tmat,
+                                  int32_t *status);
 /* Disorder potentials the device generator produces for seeds seed0.. (host upot out,
  * nxdiso*ny*nreal doubles): lets a caller feed the identical disorder to another code. */
 rg_status rg_generate_potential(rg_plan *plan, int64_t seed0, int32_t nreal, double *upot);

@@ -21,7 +21,8 @@ BOUNDTYPES = {"dirichlet": 0, "periodic": 1}
 # Every symbol include/recurgreen_hip.h declares.
 EXPORTS = (
     "rg_plan_create", "rg_plan_destroy", "rg_plan_dims", "rg_tdistrib", "rg_tdistrib_device",
-    "rg_tdistrib_potential", "rg_transmission_potential", "rg_generate_potential", "rg_histogram",
+    "rg_tdistrib_potential", "rg_transmission_potential", "rg_refl_trans", "rg_refl_trans_potential",
+    "rg_generate_potential", "rg_histogram",
     "rg_plan_stream", "rg_plan_synchronize", "rg_plan_profile", "rg_plan_kernel_times",
     "rg_status_string", "rg_version",
 )
@@ -65,6 +66,8 @@ def lib():
         "rg_tdistrib_device": [vp, i64, i32, i32, vp, vp],
         "rg_tdistrib_potential": [vp, vp, i32, i32, vp, vp],
         "rg_transmission_potential": [vp, vp, i32, i32, vp, vp],
+        "rg_refl_trans": [vp, i64, i32, vp, vp, vp],
+        "rg_refl_trans_potential": [vp, vp, i32, vp, vp, vp],
         "rg_generate_potential": [vp, i64, i32, vp],
         "rg_histogram": [vp, vp, i64, i32, dp, i32, i64p, i64p, i64p],
         "rg_plan_stream": [vp, C.POINTER(vp)],
@@ -182,6 +185,26 @@ class Plan:
         if code not in (RG_OK, RG_ESINGULAR, RG_ENONFINITE):
             check(code)
         return np.ascontiguousarray(t.transpose(0, 2, 1)), st, code
+
+    def refl_trans(self, upot: np.ndarray | None = None, seed0: int = 1, nseed: int = 1):
+        """(r, t, status, code) of compute_refl_trans_v1 (src/waveguide.f90:469-519), complex128:
+        r and t are (nreal, nmu, nmu); explicit potentials upot(ix, iy[, r]) or seeds seed0...."""
+        n = self.nmu
+        if upot is not None:
+            u = np.asfortranarray(upot, dtype=np.float64)
+            nreal = 1 if u.ndim == 2 else u.shape[2]
+        else:
+            u, nreal = None, int(nseed)
+        r = np.zeros((nreal, n, n), complex)  # column-major per realization
+        t = np.zeros((nreal, n, n), complex)
+        st = np.zeros(nreal, np.int32)
+        if u is None:
+            code = lib().rg_refl_trans(self._h, int(seed0), nreal, ptr(r), ptr(t), ptr(st))
+        else:
+            code = lib().rg_refl_trans_potential(self._h, ptr(u), nreal, ptr(r), ptr(t), ptr(st))
+        if code not in (RG_OK, RG_ESINGULAR, RG_ENONFINITE):
+            check(code)
+        return (np.ascontiguousarray(r.transpose(0, 2, 1)), np.ascontiguousarray(t.transpose(0, 2, 1)), st, code)
 
     def generate_potential(self, seed0: int, nreal: int) -> np.ndarray:
         """upot(ix, iy, r) as a Fortran-ordered (nxdiso, ny, nreal) array."""

@@ -103,11 +103,15 @@ struct rg_plan {
     std::vector<double2> h_mu, h_b, h_abound, h_abulk, h_sqrtmu;
 
     // device constants
-    double2 *d_b = nullptr, *d_adiag = nullptr, *d_sqrtmu = nullptr;  // d_adiag: 3*np (bound, bulk, last)
+    double2 *d_b = nullptr, *d_adiag = nullptr, *d_sqrtmu = nullptr;
+    double *d_sqrtremu = nullptr;  // sqrt(Re mu), reflection/transmission relations (np)  // d_adiag: 3*np (bound, bulk, last)
     double *d_tab = nullptr;                                           // (16 nchunk) x ntp, zero padded
     // batch buffers
     double2 *d_gss = nullptr, *d_gs1[2] = {nullptr, nullptr}, *d_tmat = nullptr;
     float2 *d_gss32 = nullptr, *d_gs1_32[2] = {nullptr, nullptr};  // FP32 state of the mixed sweeps
+    // reflection sweep (G11, G1s ping-pong) for up to refl_cap realizations per pass
+    int refl_cap = 0;
+    double2 *d_g11 = nullptr, *d_g1s[2] = {nullptr, nullptr};
     double *d_cq = nullptr, *d_teig = nullptr, *d_upot = nullptr;
     int32_t *d_status = nullptr;
     unsigned long long *d_hist = nullptr;  // counts + missing + dropped
@@ -132,6 +136,12 @@ void rg_launch_fisher_lee_svd_f64(rg_plan *P, int nb, const double2 *gs1, double
                                   double2 *tmat_out);
 void rg_launch_fisher_lee_svd_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status,
                                   double2 *tmat_out);
+// rg_zgemm.hip: C = alpha diag(dl) op(A) diag(dk) op(B) diag(dr) + beta C (op: 0 = N, 1 = H), n = nmu;
+// C must not alias A or B.
+void rg_zgemm(rg_plan *P, int nb, double2 alpha, const double2 *A, int opA, const double2 *B, int opB, double2 beta,
+              double2 *C, const double2 *dl, const double2 *dk, const double2 *dr);
+// rg_refl.hip
+void rg_launch_refl_trans_out(rg_plan *P, int nb, const double2 *g11, const double2 *gs1, double2 *rmat, double2 *tmat);
 // rg_eig.hip
 void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status);
 void rg_launch_tdt_eig_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status);

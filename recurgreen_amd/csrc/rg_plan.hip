@@ -46,7 +46,8 @@ void free_plan(rg_plan *P)
     (void)hipSetDevice(P->device);
     void *bufs[] = {P->d_b, P->d_adiag, P->d_sqrtmu, P->d_tab, P->d_gss, P->d_gs1[0], P->d_gs1[1],
                     P->d_tmat, P->d_cq, P->d_teig, P->d_upot, P->d_status, P->d_hist, P->d_bins,
-                    P->d_gss32, P->d_gs1_32[0], P->d_gs1_32[1]};
+                    P->d_gss32, P->d_gs1_32[0], P->d_gs1_32[1], P->d_g11, P->d_g1s[0], P->d_g1s[1],
+                    P->d_sqrtremu};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &t : P->timers) {
@@ -161,6 +162,10 @@ rg_status upload_constants(rg_plan *P)
     HIPCHK(hipMemcpy(P->d_b, P->h_b.data(), sizeof(double2) * np, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(P->d_adiag, ad.data(), sizeof(double2) * ad.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(P->d_sqrtmu, P->h_sqrtmu.data(), sizeof(double2) * np, hipMemcpyHostToDevice));
+    std::vector<double> sre(np, 0.0);
+    for (int m = 0; m < P->nmu; m++) sre[m] = std::sqrt(P->h_mu[m].x);  // src/waveguide.f90:510
+    if ((st = dalloc(&P->d_sqrtremu, np))) return st;
+    HIPCHK(hipMemcpy(P->d_sqrtremu, sre.data(), sizeof(double) * np, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(P->d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
     return RG_OK;
 }
@@ -179,6 +184,10 @@ rg_status alloc_batch(rg_plan *P)
     const size_t cap_bytes = (size_t)256 << 20;
     P->upot_cap = (int)std::max<size_t>(1, std::min<size_t>(B, cap_bytes / (per * sizeof(double))));
     if ((st = dalloc(&P->d_upot, (size_t)P->upot_cap * per))) return st;
+    P->refl_cap = (int)std::min<size_t>(B, 256);
+    if ((st = dalloc(&P->d_g11, (size_t)P->refl_cap * mat)) || (st = dalloc(&P->d_g1s[0], (size_t)P->refl_cap * mat)) ||
+        (st = dalloc(&P->d_g1s[1], (size_t)P->refl_cap * mat)))
+        return st;
     P->hist_cap = 4096;
     if ((st = dalloc(&P->d_hist, (size_t)P->hist_cap + 2)) || (st = dalloc(&P->d_bins, (size_t)P->hist_cap + 1))) return st;
     return RG_OK;
@@ -238,6 +247,41 @@ void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int prec
     } else {
         rg_launch_tdt_eig_f32(P, nb, P->d_gs1_32[cur], teig, P->d_status);
     }
+}
+
+// The reflection/transmission sweep of compute_refl_trans_v1 (src/waveguide.f90:469-519), FP64:
+// besides Gss and Gs1 it carries G11 and G1s, updated after Gs1 in the reference's order
+// (:498-505): Gs1 <- -Gss B Gs1, G11 <- G11 - G1s B Gs1 (new Gs1), G1s <- -G1s B Gss.
+// Writes r (column-major nmu x nmu per realization) to r_out and t to the Gs1 buffer the sweep
+// finished without, which it returns.
+double2 *run_sweep_refl(rg_plan *P, int nb, int64_t seed0, const double *d_upot, double2 *r_out)
+{
+    int cur = 0, c1s = 0;
+    const int nx = P->nx;
+    const size_t mat = (size_t)nb * P->np * P->np;
+    const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
+    for (int ix = 1; ix <= nx; ix++) {
+        const int id = ix - P->nxfree;
+        const bool dis = id >= 1 && id <= P->nxdiso;
+        if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
+        const int slot = ix == 1 ? 0 : (ix == nx ? 2 : 1);
+        if (P->inv_fp64 == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
+        else if (P->inv_fp64 == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, RG_FP64);
+        else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, RG_FP64);
+        if (ix == 1) {
+            (void)hipMemcpyAsync(P->d_gs1[0], P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+            (void)hipMemcpyAsync(P->d_g11, P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+            (void)hipMemcpyAsync(P->d_g1s[0], P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+        } else {
+            rg_launch_gemm_f64(P, nb, P->d_gss, P->d_gs1[cur], P->d_gs1[cur ^ 1]);
+            cur ^= 1;
+            rg_zgemm(P, nb, mone, P->d_g1s[c1s], 0, P->d_gs1[cur], 0, one, P->d_g11, nullptr, P->d_b, nullptr);
+            rg_zgemm(P, nb, mone, P->d_g1s[c1s], 0, P->d_gss, 0, zero, P->d_g1s[c1s ^ 1], nullptr, P->d_b, nullptr);
+            c1s ^= 1;
+        }
+    }
+    rg_launch_refl_trans_out(P, nb, P->d_g11, P->d_gs1[cur], r_out, P->d_gs1[cur ^ 1]);
+    return P->d_gs1[cur ^ 1];
 }
 
 int inverse_choice(const char *v, int dflt, bool fp64)
@@ -457,6 +501,51 @@ rg_status rg_transmission_potential(rg_plan *P, const double *upot, int32_t nrea
                                     int32_t *status)
 {
     return potential_pass(P, upot, nreal, precision, tmat, status, true);
+}
+
+// Reflection and transmission matrices, FP64 (compute_refl_trans_v1, src/waveguide.f90:469-519),
+// for seeds (upot == nullptr) or explicit host potentials.
+static rg_status refl_trans_pass(rg_plan *P, int64_t seed0, const double *upot, int32_t nreal, double *rmat,
+                                 double *tmat, int32_t *status)
+{
+    rg_status st = check_common(P, nreal, RG_FP64);
+    if (st) return st;
+    if ((!rmat || !tmat) && nreal > 0) return RG_EINVAL;
+    DeviceGuard g(P->device);
+    const size_t per = (size_t)P->nxdiso * P->ny, nn2 = (size_t)P->nmu * P->nmu;
+    const int cap = upot ? std::min(P->upot_cap, P->refl_cap) : P->refl_cap;
+    double2 *d_r = P->d_tmat;  // batch_max x np^2 complex >= nb x nmu^2
+    int32_t worst = 0;
+    for (int32_t off = 0; off < nreal; off += cap) {
+        const int nb = std::min(cap, nreal - off);
+        HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
+        const double *dup = nullptr;
+        if (upot) {
+            for (size_t e = 0; e < per * nb; e++)
+                if (!std::isfinite(upot[per * off + e])) return RG_ENONFINITE;
+            HIPCHK(hipMemcpyAsync(P->d_upot, upot + per * off, sizeof(double) * per * nb, hipMemcpyHostToDevice, P->stream));
+            dup = P->d_upot;
+        }
+        const double2 *d_t = run_sweep_refl(P, nb, seed0 + off, dup, d_r);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(rmat + 2 * nn2 * off, d_r, sizeof(double2) * nn2 * nb, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(tmat + 2 * nn2 * off, d_t, sizeof(double2) * nn2 * nb, hipMemcpyDeviceToHost, P->stream));
+        if ((st = harvest_status(P, nb, status ? status + off : nullptr, &worst))) return st;
+    }
+    if (P->profiling) harvest_timers(P);
+    return worst ? (rg_status)worst : RG_OK;
+}
+
+rg_status rg_refl_trans(rg_plan *P, int64_t seed0, int32_t nseed, double *rmat, double *tmat, int32_t *status)
+{
+    return refl_trans_pass(P, seed0, nullptr, nseed, rmat, tmat, status);
+}
+
+rg_status rg_refl_trans_potential(rg_plan *P, const double *upot, int32_t nreal, double *rmat, double *tmat,
+                                  int32_t *status)
+{
+    if (!upot && nreal > 0) return RG_EINVAL;
+    return refl_trans_pass(P, 0, upot, nreal, rmat, tmat, status);
 }
 
 rg_status rg_generate_potential(rg_plan *P, int64_t seed0, int32_t nreal, double *upot)

@@ -88,6 +88,14 @@ class Waveguide:
             raise RecurGreenError(code)
         return t[0] if t.shape[0] == 1 else t
 
+    def compute_refl_trans(self, iseed: int):
+        """(rmat, tmat) of realization `iseed`, as compute_refl_trans_v1 (src/waveguide.f90:469-519)
+        returns them: nmu x nmu complex128, with the sqrt(Re mu) Fisher-Lee factors of :510-512."""
+        r, t, st, code = self.plan.refl_trans(None, seed0=iseed, nseed=1)
+        if code:
+            raise RecurGreenError(code)
+        return r[0], t[0]
+
     def generate_potential(self, seed0: int, nreal: int = 1) -> np.ndarray:
         return self.plan.generate_potential(seed0, nreal)
 

@@ -171,3 +171,37 @@ def test_mixed_precision_against_oracle(name, precision):
     print(f"[gpu-case] {precision} {name}: max |dT| = {err:.3e}, mean dT = {np.mean(Tg - To):.3e}", flush=True)
     assert err <= MIXED_TOL[precision]
     plan.close()
+
+
+# ---------------------------------------------------------------------------------------------
+# Reflection and transmission matrices (compute_refl_trans_v1, complex128).
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["ref_N10_free", "c1_N20_ny205", "N37_absorb", "periodic_N41_ny60", "N64"])
+def test_refl_trans_parity(name):
+    p = CASES[name]
+    say(f"refl/trans parity {name}")
+    ow = O.Waveguide(**p)
+    plan = binding.Plan(p, batch_max=3)
+    U = upot_batch(ow, [5, 6, 7])
+    r, t, st, code = plan.refl_trans(U)
+    assert code == 0 and not st.any()
+    for k in range(3):
+        ro, to = ow.refl_trans(U[:, :, k].T, version=1)
+        assert np.max(np.abs(t[k] - to)) <= RTOL * max(1.0, np.max(np.abs(to)))
+        assert np.max(np.abs(r[k] - ro)) <= RTOL * max(1.0, np.max(np.abs(ro)))
+    plan.close()
+
+
+def test_refl_trans_unitarity_seeded_n128():
+    """Flux conservation r^H r + t^H t = I at zero absorption (src/waveguide.test.f90:54-86) on the
+    N=128 workload shape, over more realizations than one device pass holds."""
+    p = dict(CASES["N128"], nxdiso=200, aratio=10.0 / 64.25)
+    say("refl/trans unitarity N128")
+    plan = binding.Plan(p, batch_max=300)
+    r, t, st, code = plan.refl_trans(None, seed0=3, nseed=260)
+    assert code == 0 and not st.any()
+    eye = np.eye(plan.nmu)
+    for k in (0, 127, 255, 259):
+        err = np.max(np.abs(r[k].conj().T @ r[k] + t[k].conj().T @ t[k] - eye))
+        assert err < 1e-9, err
+    plan.close()

@@ -338,3 +338,61 @@ def test_replot_fixture():
     np.testing.assert_array_equal(e, O.bins("chebyshev", 256))
     h, _ = O.histogram(samples, e)
     np.testing.assert_allclose(distrib.rho_from_counts(h, e), ref, rtol=1e-15, atol=0)
+
+
+# ---------------------------------------------------------------------------------------------
+# FP64 adjoint (no reference counterpart): pinned by finite differences of the forward sweep.
+# ---------------------------------------------------------------------------------------------
+ADJ_CASES = {
+    "dirichlet_absorb": dict(boundtype="dirichlet", nxdiso=30, nxfree=3, wol=3.3, dscat=1.0, ny=17, dos_model=1,
+                             dabso=0.2),
+    "periodic": dict(boundtype="periodic", nxdiso=25, nxfree=2, wol=3.3, dscat=0.8, ny=15),
+    "dirichlet_filtered": dict(boundtype="dirichlet", nxdiso=20, wol=4.3, exfac=1.2, dscat=0.5, napera=0.7,
+                               naperb=0.9),
+}
+
+
+def _loss(ow, u, kind):
+    if kind == "g":
+        return float(np.sum(np.abs(ow.transmission_matrix(u)) ** 2))
+    return float(ow.tdistrib_potential(u)[0])
+
+
+@pytest.mark.parametrize("kind", ["g", "tmax"])
+@pytest.mark.parametrize("name", list(ADJ_CASES))
+def test_adjoint_matches_finite_differences(name, kind):
+    """d loss / d u(x, y) from the reverse sweep against (a) central differences with step 1e-6,
+    whose own round-off is ~eps*L/h = 1e-9 absolute, and (b) the fourth-order stencil with step
+    1e-3 (truncation ~h^4), to 1e-6 relative."""
+    ow = O.Waveguide(**ADJ_CASES[name])
+    U = ow.potential(4)
+    val, G = ow.adjoint_grad(U, kind)
+    assert val == pytest.approx(_loss(ow, U, kind), rel=1e-13)
+    rng = np.random.default_rng(1)
+    for _ in range(8):
+        iy, ix = int(rng.integers(U.shape[0])), int(rng.integers(U.shape[1]))
+
+        def f(dh):
+            V = U.copy()
+            V[iy, ix] += dh
+            return _loss(ow, V, kind)
+
+        h = 1e-6
+        fd = (f(h) - f(-h)) / (2 * h)
+        assert abs(G[iy, ix] - fd) <= 1e-6 * abs(fd) + 2e-9
+        h = 1e-3
+        fd4 = (-f(2 * h) + 8 * f(h) - 8 * f(-h) + f(-2 * h)) / (12 * h)
+        assert abs(G[iy, ix] - fd4) <= 1e-6 * abs(fd4) + 1e-11
+
+
+def test_adjoint_zero_outside_disorder_and_linear_in_scale():
+    """The gradient lives on the disordered slices only (layout upot(iy, ix)), and the loss is
+    invariant under u -> u on a mirrored guide, so the gradient of the mirrored potential is the
+    mirrored gradient (src/waveguide.test.f90 symmetry used as an invariant)."""
+    p = dict(boundtype="dirichlet", nxdiso=18, nxfree=4, wol=2.6, dscat=1.0)
+    ow = O.Waveguide(**p)
+    U = ow.potential(9)
+    _, G = ow.adjoint_grad(U, "g")
+    assert G.shape == U.shape and np.all(np.isfinite(G))
+    _, Gm = ow.adjoint_grad(U[:, ::-1].copy(), "g")
+    np.testing.assert_allclose(Gm, G[:, ::-1], rtol=1e-8, atol=1e-12 * np.max(np.abs(G)))
