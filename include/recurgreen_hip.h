@@ -101,6 +101,20 @@ rg_status rg_refl_trans(rg_plan *plan, int64_t seed0, int32_t nseed, double *rma
 /* Same for explicit host potentials upot (nxdiso*ny*nreal doubles). */
 rg_status rg_refl_trans_potential(rg_plan *plan, const double *upot, int32_t nreal, double *rmat, double *tmat,
                                   int32_t *status);
+/* Adjoint sweep (no counterpart in the reference: the reverse-mode derivative of the sweep of
+ * src/waveguide.f90:427-445).  loss_kind 0: L = Tr(t^H t) = sum of the transmission eigenvalues
+ * (t over the filtered ranges, complex sqrt(mu) as in :443).  For each realization: loss[r] = L and
+ * grad = dL/d upot with the potential layout (nxdiso, ny, nreal), units of upot (U dx / k; dL/dU =
+ * grad * dx / k).  Complex128; slice Green functions are recomputed from checkpoints kept every
+ * ceil(sqrt(nx)) slices.  rg_adjoint_reserve allocates the workspace for nreal_max realizations
+ * per device pass (the only allocating adjoint call); the adjoint calls return RG_EINVAL before. */
+rg_status rg_adjoint_reserve(rg_plan *plan, int32_t nreal_max);
+/* Host buffers: upot (nxdiso*ny*nreal), loss (nreal), grad (nxdiso*ny*nreal), status (may be NULL). */
+rg_status rg_adjoint_potential(rg_plan *plan, const double *upot, int32_t nreal, int32_t loss_kind, double *loss,
+                               double *grad, int32_t *status);
+/* Device buffers, asynchronous on the plan's stream (status_dev may be NULL). */
+rg_status rg_adjoint_potential_device(rg_plan *plan, const double *upot_dev, int32_t nreal, int32_t loss_kind,
+                                      double *loss_dev, double *grad_dev, int32_t *status_dev);
 /* Disorder potentials the device generator produces for seeds seed0.. (host upot out,
  * nxdiso*ny*nreal doubles): lets a caller feed the identical disorder to another code. */
 rg_status rg_generate_potential(rg_plan *plan, int64_t seed0, int32_t nreal, double *upot);

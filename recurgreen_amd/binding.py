@@ -22,6 +22,7 @@ BOUNDTYPES = {"dirichlet": 0, "periodic": 1}
 EXPORTS = (
     "rg_plan_create", "rg_plan_destroy", "rg_plan_dims", "rg_tdistrib", "rg_tdistrib_device",
     "rg_tdistrib_potential", "rg_transmission_potential", "rg_refl_trans", "rg_refl_trans_potential",
+    "rg_adjoint_reserve", "rg_adjoint_potential", "rg_adjoint_potential_device",
     "rg_generate_potential", "rg_histogram",
     "rg_plan_stream", "rg_plan_synchronize", "rg_plan_profile", "rg_plan_kernel_times",
     "rg_status_string", "rg_version",
@@ -55,6 +56,16 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RecurGreenError(RG_EDEVICE, f"{LIB_PATH} is not built; run recurgreen_amd.build.build()")
+    # One HIP runtime per process: when PyTorch-ROCm is installed, let it load and initialise its
+    # runtime first, so that this library binds to the same libamdhip64 and device pointers pass
+    # freely between torch tensors and the plan (a second runtime loaded later hides the GPUs
+    # from torch).
+    try:
+        import torch
+
+        torch.cuda.is_available()
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i32, i64, dp = C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double)
     i32p, i64p = C.POINTER(C.c_int32), C.POINTER(C.c_int64)
@@ -67,6 +78,9 @@ def lib():
         "rg_tdistrib_potential": [vp, vp, i32, i32, vp, vp],
         "rg_transmission_potential": [vp, vp, i32, i32, vp, vp],
         "rg_refl_trans": [vp, i64, i32, vp, vp, vp],
+        "rg_adjoint_reserve": [vp, i32],
+        "rg_adjoint_potential": [vp, vp, i32, i32, vp, vp, vp],
+        "rg_adjoint_potential_device": [vp, vp, i32, i32, vp, vp, vp],
         "rg_refl_trans_potential": [vp, vp, i32, vp, vp, vp],
         "rg_generate_potential": [vp, i64, i32, vp],
         "rg_histogram": [vp, vp, i64, i32, dp, i32, i64p, i64p, i64p],
@@ -205,6 +219,40 @@ class Plan:
         if code not in (RG_OK, RG_ESINGULAR, RG_ENONFINITE):
             check(code)
         return (np.ascontiguousarray(r.transpose(0, 2, 1)), np.ascontiguousarray(t.transpose(0, 2, 1)), st, code)
+
+    LOSSES = {"g": 0, "tmax": 1}
+
+    def adjoint_reserve(self, nreal_max: int) -> None:
+        check(lib().rg_adjoint_reserve(self._h, int(nreal_max)))
+
+    def adjoint(self, upot: np.ndarray, loss: str = "g"):
+        """(loss[r], dL/du as (nxdiso, ny, nreal) Fortran-ordered, status, code) for explicit host
+        potentials; complex128 adjoint sweep."""
+        u = np.asfortranarray(upot, dtype=np.float64)
+        if u.ndim == 2:
+            u = u[:, :, None]
+        nreal = u.shape[2]
+        self.adjoint_reserve(min(nreal, self.batch_max))
+        val = np.zeros(nreal)
+        grad = np.zeros((nreal, self.ny, self.nxdiso))
+        st = np.zeros(nreal, np.int32)
+        code = lib().rg_adjoint_potential(self._h, ptr(u), nreal, self.LOSSES[loss], ptr(val), ptr(grad), ptr(st))
+        if code not in (RG_OK, RG_ESINGULAR, RG_ENONFINITE):
+            check(code)
+        return val, grad.transpose(2, 1, 0), st, code
+
+    def adjoint_device(self, upot, loss: str = "g"):
+        """Device variant on torch CUDA tensors: upot is (nreal, ny, nxdiso) float64 contiguous (the
+        (nxdiso, ny, nreal) Fortran layout).  Returns (loss, grad) tensors; asynchronous on stream()."""
+        import torch
+
+        nreal = upot.shape[0]
+        self.adjoint_reserve(min(nreal, self.batch_max))
+        val = torch.empty(nreal, dtype=torch.float64, device=upot.device)
+        grad = torch.empty_like(upot)
+        check(lib().rg_adjoint_potential_device(self._h, ptr(upot), nreal, self.LOSSES[loss], ptr(val), ptr(grad),
+                                                None))
+        return val, grad
 
     def generate_potential(self, seed0: int, nreal: int) -> np.ndarray:
         """upot(ix, iy, r) as a Fortran-ordered (nxdiso, ny, nreal) array."""

@@ -112,6 +112,12 @@ struct rg_plan {
     // reflection sweep (G11, G1s ping-pong) for up to refl_cap realizations per pass
     int refl_cap = 0;
     double2 *d_g11 = nullptr, *d_g1s[2] = {nullptr, nullptr};
+    // adjoint sweep workspace (rg_adjoint_reserve): checkpoints every adj_seg slices, one segment
+    // of recomputed states, five adjoint matrices, the gradient staging, for adj_cap realizations
+    int adj_cap = 0, adj_seg = 0, adj_nck = 0;
+    double2 *d_ck = nullptr, *d_seg = nullptr, *d_adj = nullptr;
+    double *d_loss = nullptr, *d_grad = nullptr;
+    double2 *d_bconj = nullptr;  // conj(b) (np)
     double *d_cq = nullptr, *d_teig = nullptr, *d_upot = nullptr;
     int32_t *d_status = nullptr;
     unsigned long long *d_hist = nullptr;  // counts + missing + dropped
@@ -140,6 +146,11 @@ void rg_launch_fisher_lee_svd_f32(rg_plan *P, int nb, const float2 *gs1, double 
 // C must not alias A or B.
 void rg_zgemm(rg_plan *P, int nb, double2 alpha, const double2 *A, int opA, const double2 *B, int opB, double2 beta,
               double2 *C, const double2 *dl, const double2 *dk, const double2 *dr);
+// rg_adjoint.hip
+void rg_launch_adj_terminal(rg_plan *P, int nb, const double2 *gs1, const double2 *tw, double *loss, double2 *sbar);
+void rg_launch_adj_add(rg_plan *P, int nb, double2 *gbar, const double2 *sbar);
+void rg_launch_adj_scale(rg_plan *P, int nb, double2 *gbar, const double2 *cbar);
+void rg_launch_adj_grad(rg_plan *P, int nb, const double2 *cbar, int ixd, double *grad);
 // rg_refl.hip
 void rg_launch_refl_trans_out(rg_plan *P, int nb, const double2 *g11, const double2 *gs1, double2 *rmat, double2 *tmat);
 // rg_eig.hip

@@ -47,7 +47,7 @@ void free_plan(rg_plan *P)
     void *bufs[] = {P->d_b, P->d_adiag, P->d_sqrtmu, P->d_tab, P->d_gss, P->d_gs1[0], P->d_gs1[1],
                     P->d_tmat, P->d_cq, P->d_teig, P->d_upot, P->d_status, P->d_hist, P->d_bins,
                     P->d_gss32, P->d_gs1_32[0], P->d_gs1_32[1], P->d_g11, P->d_g1s[0], P->d_g1s[1],
-                    P->d_sqrtremu};
+                    P->d_sqrtremu, P->d_ck, P->d_seg, P->d_adj, P->d_loss, P->d_grad, P->d_bconj};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &t : P->timers) {
@@ -162,6 +162,10 @@ rg_status upload_constants(rg_plan *P)
     HIPCHK(hipMemcpy(P->d_b, P->h_b.data(), sizeof(double2) * np, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(P->d_adiag, ad.data(), sizeof(double2) * ad.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(P->d_sqrtmu, P->h_sqrtmu.data(), sizeof(double2) * np, hipMemcpyHostToDevice));
+    std::vector<double2> bc(np, make_double2(0, 0));
+    for (int m = 0; m < P->nmu; m++) bc[m] = cconj(P->h_b[m]);
+    if ((st = dalloc(&P->d_bconj, np))) return st;
+    HIPCHK(hipMemcpy(P->d_bconj, bc.data(), sizeof(double2) * np, hipMemcpyHostToDevice));
     std::vector<double> sre(np, 0.0);
     for (int m = 0; m < P->nmu; m++) sre[m] = std::sqrt(P->h_mu[m].x);  // src/waveguide.f90:510
     if ((st = dalloc(&P->d_sqrtremu, np))) return st;
@@ -282,6 +286,88 @@ double2 *run_sweep_refl(rg_plan *P, int nb, int64_t seed0, const double *d_upot,
     }
     rg_launch_refl_trans_out(P, nb, P->d_g11, P->d_gs1[cur], r_out, P->d_gs1[cur ^ 1]);
     return P->d_gs1[cur ^ 1];
+}
+
+// One complex128 slice step: Gss_x in place in P->d_gss, and s1_out = -Gss_x B s1_in (a copy of
+// Gss_1 at ix == 1).  d_upot: explicit potentials of the pass (nullptr: no disorder).
+void fwd_step_f64(rg_plan *P, int nb, int ix, const double *d_upot, const double2 *s1_in, double2 *s1_out)
+{
+    const int id = ix - P->nxfree;
+    const bool dis = d_upot && id >= 1 && id <= P->nxdiso;
+    if (dis) rg_launch_project(P, nb, id - 1, 0, d_upot, nb);
+    const int slot = ix == 1 ? 0 : (ix == P->nx ? 2 : 1);
+    if (P->inv_fp64 == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
+    else if (P->inv_fp64 == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, RG_FP64);
+    else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, RG_FP64);
+    if (ix == 1)
+        (void)hipMemcpyAsync(s1_out, P->d_gss, sizeof(double2) * (size_t)nb * P->np * P->np, hipMemcpyDeviceToDevice,
+                             P->stream);
+    else
+        rg_launch_gemm_f64(P, nb, P->d_gss, s1_in, s1_out);
+}
+
+// Adjoint of L = Tr(t^H t) over nb realizations with explicit device potentials d_upot (layout
+// (nxdiso, ny, nb)): the loss to d_loss[nb], dL/du to d_grad (same layout as d_upot).  Forward
+// sweep with a checkpoint (Gss, Gs1) after every adj_seg slices; then, segment by segment from
+// the end, the states are recomputed from the checkpoint and the reverse recursion runs through
+// them (every product on rg_zgemm, complex128).
+void run_adjoint_f64(rg_plan *P, int nb, const double *d_upot, double *d_loss, double *d_grad)
+{
+    const size_t mat = (size_t)nb * P->np * P->np, cmat = (size_t)P->adj_cap * P->np * P->np;
+    const int nx = P->nx, S = P->adj_seg;
+    const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
+    auto ck_g = [&](int c) { return P->d_ck + (size_t)(2 * c) * cmat; };
+    auto ck_s = [&](int c) { return P->d_ck + (size_t)(2 * c + 1) * cmat; };
+    auto seg_g = [&](int j) { return P->d_seg + (size_t)(2 * j) * cmat; };
+    auto seg_s = [&](int j) { return P->d_seg + (size_t)(2 * j + 1) * cmat; };
+    double2 *gbar = P->d_adj, *sbar = P->d_adj + cmat, *sbar2 = P->d_adj + 2 * cmat, *T1 = P->d_adj + 3 * cmat,
+            *cbar = P->d_adj + 4 * cmat;
+    // forward sweep with checkpoints after slices S, 2S, ...
+    double2 *sa = P->d_gs1[0], *sb = P->d_gs1[1];
+    for (int ix = 1; ix <= nx; ix++) {
+        fwd_step_f64(P, nb, ix, d_upot, sa, sb);
+        std::swap(sa, sb);
+        if (ix % S == 0 && ix < nx) {
+            const int c = ix / S - 1;
+            (void)hipMemcpyAsync(ck_g(c), P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+            (void)hipMemcpyAsync(ck_s(c), sa, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+        }
+    }
+    rg_launch_adj_terminal(P, nb, sa, nullptr, d_loss, sbar);
+    (void)hipMemsetAsync(gbar, 0, sizeof(double2) * mat, P->stream);
+    (void)hipMemsetAsync(d_grad, 0, sizeof(double) * (size_t)nb * P->nxdiso * P->ny, P->stream);
+    const int nseg = (nx + S - 1) / S;
+    for (int c = nseg - 1; c >= 0; c--) {
+        const int x0 = c * S + 1, x1 = std::min((c + 1) * S, nx);
+        const double2 *s_in = nullptr;
+        if (c >= 1) {
+            (void)hipMemcpyAsync(P->d_gss, ck_g(c - 1), sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+            s_in = ck_s(c - 1);
+        }
+        for (int x = x0; x <= x1; x++) {  // recompute the segment's states
+            const int j = x - x0;
+            fwd_step_f64(P, nb, x, d_upot, s_in, seg_s(j));
+            (void)hipMemcpyAsync(seg_g(j), P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+            s_in = seg_s(j);
+        }
+        for (int x = x1; x >= x0; x--) {  // reverse recursion
+            const int j = x - x0;
+            const double2 *gx = seg_g(j);
+            if (x >= 2) {
+                const double2 *sprev = j > 0 ? seg_s(j - 1) : ck_s(c - 1);
+                rg_zgemm(P, nb, mone, sbar, 0, sprev, 1, one, gbar, nullptr, nullptr, P->d_bconj);
+                rg_zgemm(P, nb, mone, gx, 1, sbar, 0, zero, sbar2, P->d_bconj, nullptr, nullptr);
+                std::swap(sbar, sbar2);
+            } else {
+                rg_launch_adj_add(P, nb, gbar, sbar);
+            }
+            rg_zgemm(P, nb, one, gbar, 0, gx, 1, zero, T1, nullptr, nullptr, nullptr);
+            rg_zgemm(P, nb, mone, gx, 1, T1, 0, zero, cbar, nullptr, nullptr, nullptr);
+            const int id = x - P->nxfree;
+            if (id >= 1 && id <= P->nxdiso) rg_launch_adj_grad(P, nb, cbar, id - 1, d_grad);
+            if (x >= 2) rg_launch_adj_scale(P, nb, gbar, cbar);
+        }
+    }
 }
 
 int inverse_choice(const char *v, int dflt, bool fp64)
@@ -546,6 +632,89 @@ rg_status rg_refl_trans_potential(rg_plan *P, const double *upot, int32_t nreal,
 {
     if (!upot && nreal > 0) return RG_EINVAL;
     return refl_trans_pass(P, 0, upot, nreal, rmat, tmat, status);
+}
+
+rg_status rg_adjoint_reserve(rg_plan *P, int32_t nreal_max)
+{
+    if (!P || nreal_max < 1) return RG_EINVAL;
+    if (P->np > 128) return RG_ENOTSUP;
+    DeviceGuard g(P->device);
+    if (P->adj_cap >= nreal_max) return RG_OK;
+    for (void *b : {(void *)P->d_ck, (void *)P->d_seg, (void *)P->d_adj, (void *)P->d_loss, (void *)P->d_grad})
+        if (b) (void)hipFree(b);
+    P->d_ck = P->d_seg = P->d_adj = nullptr;
+    P->d_loss = P->d_grad = nullptr;
+    P->adj_cap = 0;
+    const int nx = P->nx;
+    int seg = (int)std::ceil(std::sqrt((double)nx));
+    if (seg < 1) seg = 1;
+    const int nck = std::max(0, (nx - 1) / seg);  // checkpoints after slices seg, 2 seg, ... < nx
+    const size_t cmat = (size_t)nreal_max * P->np * P->np;
+    rg_status st;
+    if ((st = dalloc(&P->d_ck, std::max<size_t>(1, 2 * (size_t)nck) * cmat)) ||
+        (st = dalloc(&P->d_seg, 2 * (size_t)seg * cmat)) || (st = dalloc(&P->d_adj, 5 * cmat)) ||
+        (st = dalloc(&P->d_loss, (size_t)nreal_max)) ||
+        (st = dalloc(&P->d_grad, (size_t)nreal_max * P->nxdiso * P->ny)))
+        return st;
+    P->adj_cap = nreal_max;
+    P->adj_seg = seg;
+    P->adj_nck = nck;
+    return RG_OK;
+}
+
+static rg_status adjoint_check(rg_plan *P, int32_t nreal, int32_t loss_kind)
+{
+    rg_status st = check_common(P, nreal, RG_FP64);
+    if (st) return st;
+    if (loss_kind != 0) return RG_ENOTSUP;
+    if (P->adj_cap < 1) return RG_EINVAL;  // rg_adjoint_reserve first
+    return RG_OK;
+}
+
+rg_status rg_adjoint_potential_device(rg_plan *P, const double *upot_dev, int32_t nreal, int32_t loss_kind,
+                                      double *loss_dev, double *grad_dev, int32_t *status_dev)
+{
+    rg_status st = adjoint_check(P, nreal, loss_kind);
+    if (st) return st;
+    if (nreal > 0 && (!upot_dev || !loss_dev || !grad_dev)) return RG_EINVAL;
+    DeviceGuard g(P->device);
+    const size_t per = (size_t)P->nxdiso * P->ny;
+    const int cap = std::min(P->adj_cap, P->batch_max);
+    for (int32_t off = 0; off < nreal; off += cap) {
+        const int nb = std::min(cap, nreal - off);
+        HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
+        run_adjoint_f64(P, nb, upot_dev + per * off, loss_dev + off, grad_dev + per * off);
+        HIPCHK(hipGetLastError());
+        if (status_dev)
+            HIPCHK(hipMemcpyAsync(status_dev + off, P->d_status, sizeof(int32_t) * nb, hipMemcpyDeviceToDevice, P->stream));
+    }
+    return RG_OK;
+}
+
+rg_status rg_adjoint_potential(rg_plan *P, const double *upot, int32_t nreal, int32_t loss_kind, double *loss,
+                               double *grad, int32_t *status)
+{
+    rg_status st = adjoint_check(P, nreal, loss_kind);
+    if (st) return st;
+    if (nreal > 0 && (!upot || !loss || !grad)) return RG_EINVAL;
+    DeviceGuard g(P->device);
+    const size_t per = (size_t)P->nxdiso * P->ny;
+    const int cap = std::min(std::min(P->adj_cap, P->upot_cap), P->batch_max);
+    int32_t worst = 0;
+    for (int32_t off = 0; off < nreal; off += cap) {
+        const int nb = std::min(cap, nreal - off);
+        for (size_t e = 0; e < per * nb; e++)
+            if (!std::isfinite(upot[per * off + e])) return RG_ENONFINITE;
+        HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
+        HIPCHK(hipMemcpyAsync(P->d_upot, upot + per * off, sizeof(double) * per * nb, hipMemcpyHostToDevice, P->stream));
+        run_adjoint_f64(P, nb, P->d_upot, P->d_loss, P->d_grad);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(loss + off, P->d_loss, sizeof(double) * nb, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(grad + per * off, P->d_grad, sizeof(double) * per * nb, hipMemcpyDeviceToHost, P->stream));
+        if ((st = harvest_status(P, nb, status ? status + off : nullptr, &worst))) return st;
+    }
+    if (P->profiling) harvest_timers(P);
+    return worst ? (rg_status)worst : RG_OK;
 }
 
 rg_status rg_generate_potential(rg_plan *P, int64_t seed0, int32_t nreal, double *upot)
