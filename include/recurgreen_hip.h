@@ -102,8 +102,9 @@ rg_status rg_refl_trans(rg_plan *plan, int64_t seed0, int32_t nseed, double *rma
 rg_status rg_refl_trans_potential(rg_plan *plan, const double *upot, int32_t nreal, double *rmat, double *tmat,
                                   int32_t *status);
 /* Adjoint sweep (no counterpart in the reference: the reverse-mode derivative of the sweep of
- * src/waveguide.f90:427-445).  loss_kind 0: L = Tr(t^H t) = sum of the transmission eigenvalues
- * (t over the filtered ranges, complex sqrt(mu) as in :443).  For each realization: loss[r] = L and
+ * src/waveguide.f90:427-445).  loss_kind 0: L = Tr(t^H t) = sum of the transmission eigenvalues;
+ * loss_kind 1: L = T_max, the largest transmission eigenvalue (t over the filtered ranges, complex
+ * sqrt(mu) as in :443; the derivative is that of a simple eigenvalue).  For each realization: loss[r] = L and
  * grad = dL/d upot with the potential layout (nxdiso, ny, nreal), units of upot (U dx / k; dL/dU =
  * grad * dx / k).  Complex128; slice Green functions are recomputed from checkpoints kept every
  * ceil(sqrt(nx)) slices.  rg_adjoint_reserve allocates the workspace for nreal_max realizations

@@ -14,12 +14,10 @@
 
 namespace {
 
-// Loss and terminal adjoint for L = Tr(t^H t) = sum |t_ab|^2 (kind 0), or for kind 1 with the
-// weights tw = t v v^H supplied in `tw` (column-major per realization, m x nn, may be null for 0).
+// Loss L = Tr(t^H t) = sum |t_ab|^2 and its terminal adjoint Sbar_ij = -4i t_ab conj(s_i s_j).
 __global__ __launch_bounds__(256) void k_adj_terminal(const double2 *__restrict__ gs1, int NP, int n, int imin, int imax,
                                                       int jmin, int jmax, const double2 *__restrict__ sqrtmu,
-                                                      const double2 *__restrict__ tw, double *__restrict__ loss,
-                                                      double2 *__restrict__ sbar)
+                                                      double *__restrict__ loss, double2 *__restrict__ sbar)
 {
     __shared__ double s_red[256];
     const int b = blockIdx.x, tid = threadIdx.x;
@@ -36,8 +34,7 @@ __global__ __launch_bounds__(256) void k_adj_terminal(const double2 *__restrict_
             const double2 v = cmul(sij, G[e]);
             const double2 t = cmk(-2.0 * v.y, 2.0 * v.x);  // 2i s_i G_ij s_j
             acc += t.x * t.x + t.y * t.y;
-            const double2 w = tw ? tw[(size_t)b * m * nn + (size_t)c * m + a] : t;
-            const double2 q = cmul(w, cconj(sij));
+            const double2 q = cmul(t, cconj(sij));
             out = cmk(4.0 * q.y, -4.0 * q.x);  // -4i q
         }
         SB[e] = out;
@@ -48,7 +45,62 @@ __global__ __launch_bounds__(256) void k_adj_terminal(const double2 *__restrict_
         if (tid < off) s_red[tid] += s_red[tid + off];
         __syncthreads();
     }
-    if (tid == 0 && !tw) loss[b] = s_red[0];
+    if (tid == 0) loss[b] = s_red[0];
+}
+
+// Loss T_max and its terminal adjoint from the top eigenvector x (unit norm) of the Gram matrix:
+// gram_cols (A = t^H t, x = v): y = t v, T_max = |y|^2, t v v^H = y v^H;
+// otherwise (A = t t^H, x = u): z = t^H u, T_max = |z|^2, u u^H t = u z^H.
+__global__ __launch_bounds__(256) void k_adj_terminal_tmax(const double2 *__restrict__ gs1, int NP, int n, int imin,
+                                                           int imax, int jmin, int jmax,
+                                                           const double2 *__restrict__ sqrtmu,
+                                                           const double2 *__restrict__ xvec, int xstride,
+                                                           double *__restrict__ loss, double2 *__restrict__ sbar)
+{
+    __shared__ double2 s_x[256], s_y[256];
+    __shared__ double s_red[256];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int m = imax - imin + 1, nn = jmax - jmin + 1;
+    const bool gram_cols = nn <= m;
+    const int nx_ = gram_cols ? nn : m, ny_ = gram_cols ? m : nn;
+    const double2 *G = gs1 + (size_t)b * NP * NP;
+    double2 *SB = sbar + (size_t)b * NP * NP;
+    auto tval = [&](int a, int c) {  // t_ac = 2i s_i G_ij s_j
+        const int i = imin - 1 + a, j = jmin - 1 + c;
+        const double2 v = cmul(cmul(sqrtmu[i], G[(size_t)i * NP + j]), sqrtmu[j]);
+        return cmk(-2.0 * v.y, 2.0 * v.x);
+    };
+    for (int i = tid; i < nx_; i += 256) s_x[i] = xvec[(size_t)b * xstride + i];
+    __syncthreads();
+    double acc = 0.0;
+    for (int r = tid; r < ny_; r += 256) {
+        double2 y = cmk(0.0, 0.0);
+        if (gram_cols)
+            for (int c = 0; c < nn; c++) y = cfma(y, tval(r, c), s_x[c]);
+        else
+            for (int a = 0; a < m; a++) y = cfma(y, cconj(tval(a, r)), s_x[a]);
+        s_y[r] = y;
+        acc += y.x * y.x + y.y * y.y;
+    }
+    s_red[tid] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) s_red[tid] += s_red[tid + off];
+        __syncthreads();
+    }
+    if (tid == 0) loss[b] = s_red[0];
+    for (int e = tid; e < NP * NP; e += 256) {
+        const int i = e / NP, j = e % NP;
+        double2 out = cmk(0.0, 0.0);
+        const int a = i - (imin - 1), c = j - (jmin - 1);
+        if (i < n && j < n && a >= 0 && a < m && c >= 0 && c < nn) {
+            // tw_ac = y_a conj(v_c)  or  u_a conj(z_c)
+            const double2 w = gram_cols ? cmul(s_y[a], cconj(s_x[c])) : cmul(s_x[a], cconj(s_y[c]));
+            const double2 q = cmul(w, cconj(cmul(sqrtmu[i], sqrtmu[j])));
+            out = cmk(4.0 * q.y, -4.0 * q.x);
+        }
+        SB[e] = out;
+    }
 }
 
 // gbar += sbar on the active n x n block.
@@ -122,10 +174,17 @@ __global__ __launch_bounds__(256) void k_adj_grad(const double2 *__restrict__ cb
 
 }  // namespace
 
-void rg_launch_adj_terminal(rg_plan *P, int nb, const double2 *gs1, const double2 *tw, double *loss, double2 *sbar)
+void rg_launch_adj_terminal(rg_plan *P, int nb, const double2 *gs1, double *loss, double2 *sbar)
 {
     hipLaunchKernelGGL(k_adj_terminal, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->nmu, P->imin, P->imax, P->jmin,
-                       P->jmax, P->d_sqrtmu, tw, loss, sbar);
+                       P->jmax, P->d_sqrtmu, loss, sbar);
+}
+
+void rg_launch_adj_terminal_tmax(rg_plan *P, int nb, const double2 *gs1, const double2 *xvec, int xstride, double *loss,
+                                 double2 *sbar)
+{
+    hipLaunchKernelGGL(k_adj_terminal_tmax, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->nmu, P->imin, P->imax,
+                       P->jmin, P->jmax, P->d_sqrtmu, xvec, xstride, loss, sbar);
 }
 
 void rg_launch_adj_add(rg_plan *P, int nb, double2 *gbar, const double2 *sbar)

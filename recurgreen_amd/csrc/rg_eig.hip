@@ -53,13 +53,20 @@ __device__ inline int sturm_count(const double *d, const double *e2, int n, doub
     return cnt;
 }
 
-template <class E>
+// VEC = false: the neigv largest eigenvalues to teig.  VEC = true: only the top eigenpair; its
+// eigenvector (unit norm, n complex) to xvec[b * EG_MAXN ..]: inverse iteration on the tridiagonal
+// (partial-pivoting LU, as LAPACK's dlagtf/dlagts) and back-transformation by the Householder
+// reflectors, which are kept below the subdiagonal of the packed triangle.
+template <class E, bool VEC>
 __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1, int NP, int imin, int imax, int jmin,
                                                       int jmax, const double2 *__restrict__ sqrtmu,
                                                       double *__restrict__ teig, int neigv,
-                                                      int32_t *__restrict__ status)
+                                                      int32_t *__restrict__ status, double2 *__restrict__ xvec)
 {
     __shared__ double2 sA[EG_MAXN * (EG_MAXN + 1) / 2];
+    __shared__ double2 s_taus[VEC ? EG_MAXN : 1];
+    __shared__ double2 s_x[VEC ? EG_MAXN : 1];
+    __shared__ double s_lam;
     __shared__ double2 s_v[EG_MAXN], s_p[EG_MAXN];
     __shared__ double s_d[EG_MAXN], s_e[EG_MAXN], s_e2[EG_MAXN];
     __shared__ double s_w[EG_MAXN];     // |sqrt(mu_l)|^2 weights
@@ -138,7 +145,7 @@ __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1,
     // 2. Householder tridiagonalisation of the packed lower triangle.
     for (int k = 0; k + 1 < n; k++) {
         const int Lk = n - k - 1;  // order of the trailing block
-        const double2 *xk = sA + pk(k + 1, k, n);
+        double2 *xk = sA + pk(k + 1, k, n);
         if (tid < 64) {
             double ss = 0.0;
             for (int t = 1 + lane; t < Lk; t += 64) ss += xk[t].x * xk[t].x + xk[t].y * xk[t].y;
@@ -153,8 +160,13 @@ __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1,
                 tau = cmk((beta - alpha.x) / beta, -alpha.y / beta);
                 scale = crecip(cmk(alpha.x - beta, alpha.y));
             }
-            for (int t = lane; t < Lk; t += 64) s_v[t] = t == 0 ? cmk(1.0, 0.0) : cmul(xk[t], scale);
+            for (int t = lane; t < Lk; t += 64) {
+                const double2 vt = t == 0 ? cmk(1.0, 0.0) : cmul(xk[t], scale);
+                s_v[t] = vt;
+                if (VEC && t > 0) xk[t] = vt;  // column k below the subdiagonal is not read again
+            }
             if (lane == 0) {
+                if (VEC) s_taus[k] = tau;
                 s_tau = tau;
                 s_d[k] = sA[pk(k, k, n)].x;
                 s_e[k] = beta;
@@ -247,7 +259,7 @@ __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1,
     hi += 2.220446049250313e-16 * span * n;
     const double pivmin = 1e-290 * fmax(1.0, e2max);
     const int j = tid >> 1, h = tid & 1;
-    if (j < n && n - 1 - j < neigv) {
+    if (j < n && (VEC ? j == n - 1 : n - 1 - j < neigv)) {
         double a = lo, z = hi;
         for (int it = 0; it < 40; it++) {
             const double step = (z - a) * (1.0 / 3.0);
@@ -261,7 +273,88 @@ __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1,
             else a = x2;
             if (z - a <= 2.220446049250313e-16 * fmax(fabs(a), fabs(z)) + pivmin) break;
         }
-        if (h == 0) teig[(size_t)b * neigv + (n - 1 - j)] = fmax(0.5 * (a + z), 0.0);
+        if (h == 0) {
+            if (VEC) s_lam = 0.5 * (a + z);
+            else teig[(size_t)b * neigv + (n - 1 - j)] = fmax(0.5 * (a + z), 0.0);
+        }
+    }
+    if constexpr (VEC) {
+        __syncthreads();
+        if (tid == 0) {
+            // (T - sigma I) z = r by Gaussian elimination with row interchanges; rows kept as
+            // (u0, u1, u2) = entries at columns i, i+1, i+2.  Three solves from r = 1.
+            double *u0 = reinterpret_cast<double *>(s_p), *u1 = u0 + EG_MAXN, *u2 = s_e2;
+            double *zz = reinterpret_cast<double *>(s_v), *rh = zz + EG_MAXN;
+            const double sigma = s_lam + 1e-14 * fmax(1.0, fabs(s_lam));
+            for (int i = 0; i < n; i++) zz[i] = 1.0;
+            for (int it = 0; it < 3; it++) {
+                for (int i = 0; i < n; i++) {
+                    u0[i] = s_d[i] - sigma;
+                    u1[i] = i + 1 < n ? s_e[i] : 0.0;
+                    u2[i] = 0.0;
+                    rh[i] = zz[i];
+                }
+                for (int i = 0; i + 1 < n; i++) {
+                    const double sub = s_e[i];  // row i+1, column i
+                    if (fabs(sub) > fabs(u0[i])) {
+                        const double r0 = u0[i], r1 = u1[i], r2 = u2[i];
+                        u0[i] = sub;
+                        u1[i] = u0[i + 1];
+                        u2[i] = u1[i + 1];
+                        const double mlt = r0 / sub;
+                        u0[i + 1] = r1 - mlt * u1[i];
+                        u1[i + 1] = r2 - mlt * u2[i];
+                        const double t = rh[i];
+                        rh[i] = rh[i + 1];
+                        rh[i + 1] = t - mlt * rh[i];
+                    } else {
+                        if (u0[i] == 0.0) u0[i] = pivmin;
+                        const double mlt = sub / u0[i];
+                        u0[i + 1] -= mlt * u1[i];
+                        u1[i + 1] -= mlt * u2[i];
+                        rh[i + 1] -= mlt * rh[i];
+                    }
+                }
+                if (u0[n - 1] == 0.0) u0[n - 1] = pivmin;
+                double nrm = 0.0;
+                for (int i = n - 1; i >= 0; i--) {
+                    double v = rh[i];
+                    if (i + 1 < n) v -= u1[i] * zz[i + 1];
+                    if (i + 2 < n) v -= u2[i] * zz[i + 2];
+                    zz[i] = v / u0[i];
+                    nrm += zz[i] * zz[i];
+                }
+                const double inv = 1.0 / sqrt(nrm);
+                for (int i = 0; i < n; i++) zz[i] *= inv;
+            }
+            for (int i = 0; i < n; i++) s_x[i] = cmk(zz[i], 0.0);
+        }
+        __syncthreads();
+        // x <- H_0 H_1 ... H_{n-2} z, H_k = I - tau_k v_k v_k^H on rows k+1.. (v_k[0] = 1)
+        if (tid < 64) {
+            for (int k = n - 2; k >= 0; k--) {
+                const double2 tau = s_taus[k];
+                if (tau.x == 0.0 && tau.y == 0.0) continue;
+                const int Lk = n - k - 1;
+                const double2 *vk = sA + pk(k + 1, k, n);
+                double2 dot = cmk(0.0, 0.0);
+                for (int t = lane; t < Lk; t += 64) {
+                    const double2 vt = t == 0 ? cmk(1.0, 0.0) : vk[t];
+                    dot = cfma(dot, cconj(vt), s_x[k + 1 + t]);
+                }
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                    dot.x += __shfl_xor(dot.x, off, 64);
+                    dot.y += __shfl_xor(dot.y, off, 64);
+                }
+                const double2 f = cmul(tau, dot);
+                for (int t = lane; t < Lk; t += 64) {
+                    const double2 vt = t == 0 ? cmk(1.0, 0.0) : vk[t];
+                    s_x[k + 1 + t] = csub(s_x[k + 1 + t], cmul(vt, f));
+                }
+            }
+            for (int i = lane; i < n; i += 64) xvec[(size_t)b * EG_MAXN + i] = s_x[i];
+        }
     }
 }
 
@@ -276,8 +369,14 @@ bool rg_eig_supported(const rg_plan *P)
 template <class E>
 static void launch_eig(rg_plan *P, int nb, const E *gs1, double *teig, int32_t *status)
 {
-    hipLaunchKernelGGL(k_tdt_eig<E>, dim3(nb), dim3(EG_NT), 0, P->stream, gs1, P->np, P->imin, P->imax, P->jmin,
-                       P->jmax, P->d_sqrtmu, teig, P->neigv, status);
+    hipLaunchKernelGGL((k_tdt_eig<E, false>), dim3(nb), dim3(EG_NT), 0, P->stream, gs1, P->np, P->imin, P->imax,
+                       P->jmin, P->jmax, P->d_sqrtmu, teig, P->neigv, status, nullptr);
+}
+
+void rg_launch_tdt_topvec_f64(rg_plan *P, int nb, const double2 *gs1, int32_t *status, double2 *xvec)
+{
+    hipLaunchKernelGGL((k_tdt_eig<double2, true>), dim3(nb), dim3(EG_NT), 0, P->stream, gs1, P->np, P->imin, P->imax,
+                       P->jmin, P->jmax, P->d_sqrtmu, nullptr, P->neigv, status, xvec);
 }
 
 void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status)

@@ -147,7 +147,9 @@ void rg_launch_fisher_lee_svd_f32(rg_plan *P, int nb, const float2 *gs1, double 
 void rg_zgemm(rg_plan *P, int nb, double2 alpha, const double2 *A, int opA, const double2 *B, int opB, double2 beta,
               double2 *C, const double2 *dl, const double2 *dk, const double2 *dr);
 // rg_adjoint.hip
-void rg_launch_adj_terminal(rg_plan *P, int nb, const double2 *gs1, const double2 *tw, double *loss, double2 *sbar);
+void rg_launch_adj_terminal(rg_plan *P, int nb, const double2 *gs1, double *loss, double2 *sbar);
+void rg_launch_adj_terminal_tmax(rg_plan *P, int nb, const double2 *gs1, const double2 *xvec, int xstride, double *loss,
+                                 double2 *sbar);
 void rg_launch_adj_add(rg_plan *P, int nb, double2 *gbar, const double2 *sbar);
 void rg_launch_adj_scale(rg_plan *P, int nb, double2 *gbar, const double2 *cbar);
 void rg_launch_adj_grad(rg_plan *P, int nb, const double2 *cbar, int ixd, double *grad);
@@ -157,6 +159,8 @@ void rg_launch_refl_trans_out(rg_plan *P, int nb, const double2 *g11, const doub
 void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status);
 void rg_launch_tdt_eig_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status);
 bool rg_eig_supported(const rg_plan *P);
+// top eigenvector of t^H t (or t t^H when t has fewer rows), 128 complex slots per realization
+void rg_launch_tdt_topvec_f64(rg_plan *P, int nb, const double2 *gs1, int32_t *status, double2 *xvec);
 // rg_gj.hip
 void rg_launch_inverse_gj(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
 // rg_blocked.hip

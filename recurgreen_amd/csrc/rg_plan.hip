@@ -311,7 +311,7 @@ void fwd_step_f64(rg_plan *P, int nb, int ix, const double *d_upot, const double
 // sweep with a checkpoint (Gss, Gs1) after every adj_seg slices; then, segment by segment from
 // the end, the states are recomputed from the checkpoint and the reverse recursion runs through
 // them (every product on rg_zgemm, complex128).
-void run_adjoint_f64(rg_plan *P, int nb, const double *d_upot, double *d_loss, double *d_grad)
+void run_adjoint_f64(rg_plan *P, int nb, int loss_kind, const double *d_upot, double *d_loss, double *d_grad)
 {
     const size_t mat = (size_t)nb * P->np * P->np, cmat = (size_t)P->adj_cap * P->np * P->np;
     const int nx = P->nx, S = P->adj_seg;
@@ -333,7 +333,12 @@ void run_adjoint_f64(rg_plan *P, int nb, const double *d_upot, double *d_loss, d
             (void)hipMemcpyAsync(ck_s(c), sa, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
         }
     }
-    rg_launch_adj_terminal(P, nb, sa, nullptr, d_loss, sbar);
+    if (loss_kind == 0) {
+        rg_launch_adj_terminal(P, nb, sa, d_loss, sbar);
+    } else {  // T_max: top eigenvector of the Gram matrix into T1 (128 slots per realization)
+        rg_launch_tdt_topvec_f64(P, nb, sa, P->d_status, T1);
+        rg_launch_adj_terminal_tmax(P, nb, sa, T1, 128, d_loss, sbar);
+    }
     (void)hipMemsetAsync(gbar, 0, sizeof(double2) * mat, P->stream);
     (void)hipMemsetAsync(d_grad, 0, sizeof(double) * (size_t)nb * P->nxdiso * P->ny, P->stream);
     const int nseg = (nx + S - 1) / S;
@@ -666,7 +671,8 @@ static rg_status adjoint_check(rg_plan *P, int32_t nreal, int32_t loss_kind)
 {
     rg_status st = check_common(P, nreal, RG_FP64);
     if (st) return st;
-    if (loss_kind != 0) return RG_ENOTSUP;
+    if (loss_kind != 0 && loss_kind != 1) return RG_EINVAL;
+    if (loss_kind == 1 && !rg_eig_supported(P)) return RG_ENOTSUP;
     if (P->adj_cap < 1) return RG_EINVAL;  // rg_adjoint_reserve first
     return RG_OK;
 }
@@ -683,7 +689,7 @@ rg_status rg_adjoint_potential_device(rg_plan *P, const double *upot_dev, int32_
     for (int32_t off = 0; off < nreal; off += cap) {
         const int nb = std::min(cap, nreal - off);
         HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
-        run_adjoint_f64(P, nb, upot_dev + per * off, loss_dev + off, grad_dev + per * off);
+        run_adjoint_f64(P, nb, loss_kind, upot_dev + per * off, loss_dev + off, grad_dev + per * off);
         HIPCHK(hipGetLastError());
         if (status_dev)
             HIPCHK(hipMemcpyAsync(status_dev + off, P->d_status, sizeof(int32_t) * nb, hipMemcpyDeviceToDevice, P->stream));
@@ -707,7 +713,7 @@ rg_status rg_adjoint_potential(rg_plan *P, const double *upot, int32_t nreal, in
             if (!std::isfinite(upot[per * off + e])) return RG_ENONFINITE;
         HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
         HIPCHK(hipMemcpyAsync(P->d_upot, upot + per * off, sizeof(double) * per * nb, hipMemcpyHostToDevice, P->stream));
-        run_adjoint_f64(P, nb, P->d_upot, P->d_loss, P->d_grad);
+        run_adjoint_f64(P, nb, loss_kind, P->d_upot, P->d_loss, P->d_grad);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(loss + off, P->d_loss, sizeof(double) * nb, hipMemcpyDeviceToHost, P->stream));
         HIPCHK(hipMemcpyAsync(grad + per * off, P->d_grad, sizeof(double) * per * nb, hipMemcpyDeviceToHost, P->stream));

@@ -113,3 +113,21 @@ def test_autograd_function_backward():
         np.testing.assert_allclose(u.grad[k].cpu().numpy(), float(w[k]) * go, rtol=0,
                                    atol=1e-9 * abs(float(w[k])) * np.max(np.abs(go)))
     plan.close()
+
+
+@pytest.mark.parametrize("name", ["dirichlet_absorb", "periodic", "dirichlet_filtered", "N20_ny205"])
+def test_adjoint_tmax_matches_cpu_adjoint(name):
+    """Gradient of the largest transmission eigenvalue (top eigenvector of the Gram matrix by
+    inverse iteration on its tridiagonal form, back-transformed) against the CPU adjoint."""
+    p = ADJ[name]
+    say(f"adjoint T_max vs oracle {name}")
+    ow = O.Waveguide(**p)
+    plan = binding.Plan(p, batch_max=3)
+    U = np.stack([ow.potential(s).T for s in (4, 5, 6)], axis=2)
+    val, grad, st, code = plan.adjoint(U, "tmax")
+    assert code == 0 and not st.any()
+    for k in range(3):
+        vo, go = ow.adjoint_grad(U[:, :, k].T, "tmax")
+        assert val[k] == pytest.approx(vo, rel=1e-10)
+        np.testing.assert_allclose(grad[:, :, k].T, go, rtol=0, atol=1e-8 * np.max(np.abs(go)))
+    plan.close()
