@@ -137,9 +137,11 @@ __device__ __attribute__((always_inline)) inline void pivot_column(C (&a)[TR][4]
 template <int TR, class T>
 __global__ __launch_bounds__(64 * TR, sizeof(T) == 8 ? 2 : 4) void k_gj_lookahead(
     typename CT<T>::C *__restrict__ gss, const rg_cqv cqv, int boundtype,
-    const double2 *__restrict__ adiag, const double2 *__restrict__ bvec, int n, int first, int32_t *__restrict__ status)
+    const double2 *__restrict__ adiag, const double2 *__restrict__ bvec, int n, int first, int32_t *__restrict__ status,
+    const int32_t *__restrict__ only)
 {
     using C = typename CT<T>::C;
+    if (only && only[blockIdx.x] == 0) return;
     constexpr int NP = 16 * TR, TC = 4, NT = 64 * TR;
     constexpr int STAGE_ROWS = NP <= 16 ? 16 : (NP <= 32 ? 64 : (NP <= 64 ? 32 : 16));
     __shared__ double s_cq[2 * 256 + 8];
@@ -258,13 +260,14 @@ __global__ __launch_bounds__(64 * TR, sizeof(T) == 8 ? 2 : 4) void k_gj_lookahea
 }  // namespace
 
 template <class T>
-static void launch_gj(rg_plan *P, int nb, typename CT<T>::C *g, const rg_cqv cq, const double2 *ad, int first)
+static void launch_gj(rg_plan *P, int nb, typename CT<T>::C *g, const rg_cqv cq, const double2 *ad, int first,
+                      const int32_t *only)
 {
     switch (P->np / 16) {
 #define RG_GJ(W)                                                                                                \
     case W:                                                                                                     \
         hipLaunchKernelGGL((k_gj_lookahead<W, T>), dim3(nb), dim3(64 * W), 0, P->stream, g, cq,                   \
-                           P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status);                            \
+                           P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status, only);                      \
         break;
         RG_GJ(1)
         RG_GJ(2)
@@ -279,13 +282,14 @@ static void launch_gj(rg_plan *P, int nb, typename CT<T>::C *g, const rg_cqv cq,
     }
 }
 
-void rg_launch_inverse_gj(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision)
+void rg_launch_inverse_gj(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision,
+                          const int32_t *only)
 {
     hipEvent_t stop = nullptr;
-    rg_timer_begin(P, RGK_INVERSE, &stop);
+    rg_timer_begin(P, only ? RGK_REDO : RGK_INVERSE, &stop);
     const rg_cqv cq = rg_cq_view(P, nb, with_disorder);
     const double2 *ad = P->d_adiag + (size_t)slot * P->np;
-    if (precision == RG_FP64) launch_gj<double>(P, nb, P->d_gss, cq, ad, first);
-    else launch_gj<float>(P, nb, P->d_gss32, cq, ad, first);
-    rg_timer_end(P, RGK_INVERSE, stop);
+    if (precision == RG_FP64) launch_gj<double>(P, nb, P->d_gss, cq, ad, first, only);
+    else launch_gj<float>(P, nb, P->d_gss32, cq, ad, first, only);
+    rg_timer_end(P, only ? RGK_REDO : RGK_INVERSE, stop);
 }

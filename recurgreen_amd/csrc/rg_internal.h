@@ -63,7 +63,7 @@ struct rg_timer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
 };
 
-enum rg_kernel_id { RGK_PROJECT = 0, RGK_INVERSE, RGK_GEMM, RGK_SVD, RGK_HIST, RGK_COPY, RGK_COUNT };
+enum rg_kernel_id { RGK_PROJECT = 0, RGK_INVERSE, RGK_GEMM, RGK_SVD, RGK_HIST, RGK_COPY, RGK_REDO, RGK_COUNT };
 
 // Slice coefficients as the projection leaves them: `parts` partial sums over disjoint ranges of
 // transverse cells, cq[(part * nb + b) * ntab + q], to be added in part order and scaled by 1/ny.
@@ -82,7 +82,7 @@ __device__ inline void rg_load_cq(double *s_cq, const rg_cqv &v, int b, int tid,
     }
 }
 
-enum rg_inverse_kernel { RG_INV_REGISTER = 0, RG_INV_LOOKAHEAD = 1, RG_INV_BLOCKED = 2 };
+enum rg_inverse_kernel { RG_INV_REGISTER = 0, RG_INV_LOOKAHEAD = 1, RG_INV_BLOCKED = 2, RG_INV_NOPIV = 3 };
 
 struct rg_plan {
     rg_params p{};
@@ -120,13 +120,14 @@ struct rg_plan {
     double2 *d_bconj = nullptr;  // conj(b) (np)
     double *d_cq = nullptr, *d_teig = nullptr, *d_upot = nullptr;
     int32_t *d_status = nullptr;
+    int32_t *d_redo = nullptr;  // per realization: 1 = the verified-pivot inversion handed it back
     unsigned long long *d_hist = nullptr;  // counts + missing + dropped
     double *d_bins = nullptr;
     int hist_cap = 0;
 
     bool profiling = false;
     // slice-inversion kernel per state precision (RG_INV_*): FP64 state, complex64 state
-    int inv_fp64 = 0, inv_mixed = 1;
+    int inv_fp64 = 0, inv_mixed = 3;
     bool eig_jacobi = false;  // transmission eigenvalues by one-sided Jacobi SVD of t instead of eig(t^H t)
     rg_timer timers[RGK_COUNT];
 };
@@ -161,8 +162,11 @@ void rg_launch_tdt_eig_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, 
 bool rg_eig_supported(const rg_plan *P);
 // top eigenvector of t^H t (or t t^H when t has fewer rows), 128 complex slots per realization
 void rg_launch_tdt_topvec_f64(rg_plan *P, int nb, const double2 *gs1, int32_t *status, double2 *xvec);
-// rg_gj.hip
-void rg_launch_inverse_gj(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
+// rg_gj.hip (only != nullptr: invert just the realizations with only[b] != 0)
+void rg_launch_inverse_gj(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision,
+                          const int32_t *only = nullptr);
+// rg_nopiv.hip
+void rg_launch_inverse_nopiv(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo);
 // rg_blocked.hip
 void rg_launch_inverse_blocked(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
 void rg_launch_gemm_sp(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new, int precision);

@@ -1,0 +1,348 @@
+// rg_nopiv.hip -- blocked Gauss-Jordan slice inversion with verified diagonal pivots, FP32 state on
+// the matrix cores (gfx950).
+//
+// Slice inversion of the mixed sweeps, src/waveguide.f90:434-435 (zgetrf + zgetri of
+// src/base_utils.f90:235-274).  zgetrf pivots on max |Re|+|Im| of the current column over the
+// rows not pivoted yet.  On the slice matrices of the sweep this choice is the diagonal element
+// (the RGF Schur complements A_x - B Gss B), so the elimination here takes the diagonal pivots in
+// 16-column blocks on the matrix cores and VERIFIES, with FP64 comparisons, that every pivot is the
+// one zgetrf's rule picks (within a relative margin of 1e-4): inside a block from the elimination
+// itself, below it from V = P U_D^-1 diag(u) (P: the rows below the block, D = L_D U_D the block,
+// u its pivots; V_ij is the candidate entry of row i at step j).  A realization whose check fails
+// (or meets a zero pivot) is not written: it is flagged in `redo` and inverted again by the
+// pivoted kernel (k_gj_lookahead), which starts from the same, unmodified Gss.
+//
+// Layout: wavefront w owns the 16-column block w as np/16 MFMA accumulator tiles of 16x16 (real
+// and imaginary planes), register v of lane l holding row 4(l/16) + v, column l%16
+// (v_mfma_f32_16x16x4_f32).  Block step K (D = tile (K,K), P = column block K):
+//   wavefront K: Gauss-Jordan on D in registers (16 steps, lane shuffles) -> D^-1, pivots, and
+//   W = D^-1 L_D diag(u) = U_D^-1 diag(u); writes P (rows outside K), D^-1 and W to LDS;
+//   every wavefront: Rt = D^-1 X[K, w] (its own tile K), X[K, w] := Rt, X[i, w] -= P_i Rt (i != K)
+//   (wavefront K: X[i, K] := -P_i D^-1, X[K, K] := D^-1), and checks the rows of its own tile
+//   index against V.  The product operands use the accumulator tiles directly: a k-step v of the
+//   MFMA takes k = 4g + v in lane group g, so a tile in accumulator layout is a valid B operand.
+// Look-ahead: wavefront K+1 updates its diagonal tile first and inverts the next D before its
+// other tiles, so one workgroup barrier per block step.
+#include "rg_internal.h"
+
+namespace {
+
+typedef float np_f4 __attribute__((ext_vector_type(4)));
+constexpr double NP_MARGIN = 1.0 - 1e-4;
+
+template <int WV>
+struct NopivShared {
+    static constexpr int NP = 16 * WV;
+    float pr[2][NP][17], pi[2][NP][17];  // P (column block, rows outside the block; zero inside)
+    float dr[2][16][17], di[2][16][17];  // D^-1
+    float wr[2][16][17], wi[2][16][17];  // W = U_D^-1 diag(u)
+    double ua[2][16];                    // |u_j|_1 (FP64)
+    int bad;
+    alignas(16) double cq[2 * 256 + 8];
+};
+
+__device__ inline float shfl_f(float v, int src) { return __shfl(v, src, 64); }
+
+// Gauss-Jordan on the 16x16 block (xr, xi) held in accumulator layout by one wavefront, diagonal
+// pivots, FP64 reciprocals.  On return (xr, xi) = D^-1, (lr, li) = L_D diag(u) (the column of the
+// reduced block at each step, rows >= step), ua[j] = |u_j|_1; returns false if a pivot fails the
+// zgetrf check or is zero.
+__device__ __attribute__((always_inline)) inline bool block_gj(np_f4 &xr, np_f4 &xi, np_f4 &lr, np_f4 &li, double *ua,
+                                                               int lane)
+{
+    const int c = lane & 15, g = lane >> 4;
+    bool ok = true;
+    lr = np_f4{0.f, 0.f, 0.f, 0.f};
+    li = np_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int j = 0; j < 16; j++) {
+        const int vj = j & 3, src_row = c + 16 * (j >> 2);
+        // pivot row D[j][c] and pivot D[j][j]
+        const float prr = shfl_f(xr[vj], src_row), pri = shfl_f(xi[vj], src_row);
+        const float pvr = shfl_f(prr, j), pvi = shfl_f(pri, j);
+        // column j of my rows: D[4g+v][j]
+        float mr[4], mi[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            mr[v] = shfl_f(xr[v], j + 16 * g);
+            mi[v] = shfl_f(xi[v], j + 16 * g);
+        }
+        const double apv = fabs((double)pvr) + fabs((double)pvi);
+        if (!(apv > 0.0)) ok = false;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = 4 * g + v;
+            if (row > j && fabs((double)mr[v]) + fabs((double)mi[v]) > NP_MARGIN * apv) ok = false;
+            if (c == j && row >= j) { lr[v] = mr[v]; li[v] = mi[v]; }
+        }
+        if (lane == 0) ua[j] = apv;
+        const double2 d = apv > 0.0 ? crecip(cmk((double)pvr, (double)pvi)) : cmk(1.0, 0.0);
+        const float dr = (float)d.x, di = (float)d.y;
+        // scaled pivot row (column j of X reset to e_j first, so its entry is d)
+        float rvr, rvi;
+        if (c == j) { rvr = dr; rvi = di; }
+        else { rvr = prr * dr - pri * di; rvi = prr * di + pri * dr; }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = 4 * g + v;
+            float mur = mr[v], mui = mi[v];
+            if (row == j) { mur = pvr - 1.0f; mui = pvi; }
+            float xrv = xr[v], xiv = xi[v];
+            if (c == j) { xrv = row == j ? 1.0f : 0.0f; xiv = 0.0f; }
+            xr[v] = xrv - (mur * rvr - mui * rvi);
+            xi[v] = xiv - (mur * rvi + mui * rvr);
+        }
+    }
+    // every lane must agree: combine the flags over the wavefront
+    return __all(ok);
+}
+
+// C (tile) -= A B for one 16x16 complex tile: A rows from LDS (ar_s/ai_s [row][k], k = block
+// column), B in accumulator layout (k-step v = rows 4g + v).
+__device__ __attribute__((always_inline)) inline void tile_fnms(np_f4 &cr, np_f4 &ci, const float (*asr)[17],
+                                                                const float (*asi)[17], int r0, np_f4 br, np_f4 bi,
+                                                                int lane)
+{
+    const int i = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const float a_r = asr[r0 + i][4 * g + v], a_i = asi[r0 + i][4 * g + v];
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_r, br[v], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, bi[v], cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_r, bi[v], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_i, br[v], ci, 0, 0, 0);
+    }
+}
+
+// (pr, pi) = A B for A from LDS [16][17] (rows i, k = 4g+v) and B in accumulator layout.
+__device__ __attribute__((always_inline)) inline void tile_mul(np_f4 &cr, np_f4 &ci, const float (*asr)[17],
+                                                               const float (*asi)[17], np_f4 br, np_f4 bi, int lane)
+{
+    const int i = lane & 15, g = lane >> 4;
+    cr = np_f4{0.f, 0.f, 0.f, 0.f};
+    ci = np_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const float a_r = asr[i][4 * g + v], a_i = asi[i][4 * g + v];
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, br[v], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_i, bi[v], cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, bi[v], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, br[v], ci, 0, 0, 0);
+    }
+}
+
+// B operand (accumulator layout) from an LDS 16x16 matrix: reg v of lane l = B[4g+v][l%16].
+__device__ inline void load_b(np_f4 &br, np_f4 &bi, const float (*sr)[17], const float (*si)[17], int lane)
+{
+    const int c = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        br[v] = sr[4 * g + v][c];
+        bi[v] = si[4 * g + v][c];
+    }
+}
+
+// The owner of block K (wavefront K) inverts its diagonal tile, held in accumulator slot SLOT, and
+// publishes D^-1, W and the pivot magnitudes to buffer K & 1.
+template <int WV, int SLOT>
+__device__ __attribute__((always_inline)) inline void publish_inverse(const np_f4 (&ar)[WV], const np_f4 (&ai)[WV],
+                                                                      int K, int lane, NopivShared<WV> &sh)
+{
+    const int c = lane & 15, g = lane >> 4, buf = K & 1;
+    np_f4 dr = ar[SLOT], di = ai[SLOT], lr, li;
+    const bool ok = block_gj(dr, di, lr, li, sh.ua[buf], lane);
+    if (!ok && lane == 0) sh.bad = 1;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {  // D^-1 to LDS (row-major [row][col])
+        sh.dr[buf][4 * g + v][c] = dr[v];
+        sh.di[buf][4 * g + v][c] = di[v];
+    }
+    __builtin_amdgcn_wave_barrier();
+    np_f4 wr, wi;
+    tile_mul(wr, wi, sh.dr[buf], sh.di[buf], lr, li, lane);  // W = D^-1 (L_D diag(u))
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        sh.wr[buf][4 * g + v][c] = wr[v];
+        sh.wi[buf][4 * g + v][c] = wi[v];
+    }
+}
+
+// ... and the column block P of block K (rows outside the block; zero inside) to buffer K & 1,
+// once every slot holds its updated tile.  Slot s holds row block (s + rot) mod WV.
+template <int WV, int SLOT>
+__device__ __attribute__((always_inline)) inline void publish_column(const np_f4 (&ar)[WV], const np_f4 (&ai)[WV],
+                                                                     int K, int rot, int lane, NopivShared<WV> &sh)
+{
+    const int c = lane & 15, g = lane >> 4, buf = K & 1;
+#pragma unroll
+    for (int s = 0; s < WV; s++) {
+        const int rb = (s + rot) % WV;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            sh.pr[buf][16 * rb + 4 * g + v][c] = s == SLOT ? 0.f : ar[s][v];
+            sh.pi[buf][16 * rb + 4 * g + v][c] = s == SLOT ? 0.f : ai[s][v];
+        }
+    }
+}
+
+template <int WV>
+__global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict__ gss, const rg_cqv cqv, int boundtype,
+                                                              const double2 *__restrict__ adiag,
+                                                              const double2 *__restrict__ bvec, int n, int first,
+                                                              int32_t *__restrict__ redo)
+{
+    constexpr int NP = 16 * WV, NT = 64 * WV;
+    __shared__ NopivShared<WV> sh;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, c = lane & 15, g = lane >> 4;
+    const int b = blockIdx.x;
+    float2 *G = gss + (size_t)b * NP * NP;
+
+    if (cqv.cq) rg_load_cq(sh.cq, cqv, b, tid, NT);
+    if (tid == 0) sh.bad = 0;
+    __syncthreads();
+
+    // M = (diag(a) - V) - B Gss B in FP64, rounded into the accumulators; identity on the padding.
+    np_f4 ar[WV], ai[WV];
+    const int col = 16 * w + c;
+#pragma unroll
+    for (int ti = 0; ti < WV; ti++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = 16 * ti + 4 * g + v;
+            double2 m;
+            if (row < n && col < n) {
+                m = row == col ? adiag[row] : cmk(0.0, 0.0);
+                if (cqv.cq) {
+                    double vv;
+                    if (boundtype == RG_DIRICHLET) {
+                        vv = sh.cq[abs(row - col)] - sh.cq[row + col + 2];
+                        m.x -= vv;
+                    } else {
+                        const int dq = col - row;
+                        const double2 dd = dq >= 0 ? cmk(sh.cq[2 * dq], sh.cq[2 * dq + 1])
+                                                   : cmk(sh.cq[-2 * dq], -sh.cq[-2 * dq + 1]);
+                        m = csub(m, dd);
+                    }
+                }
+                if (!first) {
+                    const float2 gv = G[(size_t)row * NP + col];
+                    m = csub(m, cmul(cmul(bvec[row], cmk((double)gv.x, (double)gv.y)), bvec[col]));
+                }
+            } else {
+                m = cmk(row == col ? 1.0 : 0.0, 0.0);
+            }
+            ar[ti][v] = (float)m.x;
+            ai[ti][v] = (float)m.y;
+            __builtin_amdgcn_sched_barrier(0);
+        }
+
+    // Accumulator slot s holds row block (s + K) mod WV during block step K: the block being
+    // eliminated is always slot 0, the next one slot 1 (all register indices static).
+    if (w == 0) {
+        publish_inverse<WV, 0>(ar, ai, 0, lane, sh);
+        publish_column<WV, 0>(ar, ai, 0, 0, lane, sh);
+    }
+    __syncthreads();
+    for (int K = 0; K < WV; K++) {
+        const int buf = K & 1;
+        // Rt = D^-1 X[K, w]; wavefront K: Rt = D^-1 and its other tiles restart from zero
+        np_f4 rr, ri;
+        if (w == K) {
+            load_b(rr, ri, sh.dr[buf], sh.di[buf], lane);
+#pragma unroll
+            for (int s = 1; s < WV; s++) {
+                ar[s] = np_f4{0.f, 0.f, 0.f, 0.f};
+                ai[s] = np_f4{0.f, 0.f, 0.f, 0.f};
+            }
+        } else {
+            tile_mul(rr, ri, sh.dr[buf], sh.di[buf], ar[0], ai[0], lane);
+        }
+        ar[0] = rr;
+        ai[0] = ri;
+        // verification of the pivots of block K against the rows of block w (w > K)
+        if (w > K) {
+            np_f4 br, bi, vr = np_f4{0.f, 0.f, 0.f, 0.f}, vi = np_f4{0.f, 0.f, 0.f, 0.f};
+            load_b(br, bi, sh.wr[buf], sh.wi[buf], lane);
+            tile_fnms(vr, vi, sh.pr[buf], sh.pi[buf], 16 * w, br, bi, lane);  // V = -P_w W
+            bool ok = true;
+            const double ua = sh.ua[buf][c];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if (fabs((double)vr[v]) + fabs((double)vi[v]) > NP_MARGIN * ua) ok = false;
+            if (!__all(ok) && lane == 0) sh.bad = 1;
+        }
+        // X[i, w] -= P_i Rt for the other row blocks; the next block's owner updates its diagonal
+        // tile (slot 1) first and publishes that block before the rest (look-ahead)
+        if (WV > 1) tile_fnms(ar[1 % WV], ai[1 % WV], sh.pr[buf], sh.pi[buf], 16 * ((1 + K) % WV), rr, ri, lane);
+        __builtin_amdgcn_sched_barrier(0);
+        if (w == K + 1) publish_inverse<WV, 1 % WV>(ar, ai, K + 1, lane, sh);
+#pragma unroll
+        for (int s = 2; s < WV; s++) {
+            tile_fnms(ar[s], ai[s], sh.pr[buf], sh.pi[buf], 16 * ((s + K) % WV), rr, ri, lane);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (w == K + 1) publish_column<WV, 1 % WV>(ar, ai, K + 1, K, lane, sh);
+        // rotate: slot s <- slot s + 1
+        const np_f4 t0r = ar[0], t0i = ai[0];
+#pragma unroll
+        for (int s = 0; s + 1 < WV; s++) {
+            ar[s] = ar[s + 1];
+            ai[s] = ai[s + 1];
+        }
+        ar[WV - 1] = t0r;
+        ai[WV - 1] = t0i;
+        __syncthreads();
+    }
+    if (sh.bad) {
+        if (tid == 0) redo[b] = 1;
+        return;
+    }
+    if (tid == 0) redo[b] = 0;
+    // write X = inv(M): stage 16 rows at a time through the (now free) P buffers
+    float(*str)[17] = reinterpret_cast<float(*)[17]>(&sh.pr[0][0][0]);
+    float(*sti)[17] = reinterpret_cast<float(*)[17]>(&sh.pi[0][0][0]);
+    for (int ch = 0; ch < WV; ch++) {
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < WV; s++)
+            if (s == ch)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    str[16 * w + 4 * g + v][c] = ar[s][v];  // rows of chunk ch, columns of block w
+                    sti[16 * w + 4 * g + v][c] = ai[s][v];
+                }
+        __syncthreads();
+        for (int e = tid; e < 16 * NP; e += NT) {
+            const int r = e / NP, j = e % NP;  // row 16 ch + r, column j = 16 (j/16) + j%16
+            G[(size_t)(16 * ch + r) * NP + j] = make_float2(str[16 * (j >> 4) + r][j & 15], sti[16 * (j >> 4) + r][j & 15]);
+        }
+    }
+}
+
+}  // namespace
+
+// np >= 32 (with a single 16-column block the pivoted kernel is used directly).
+void rg_launch_inverse_nopiv(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo)
+{
+    hipEvent_t stop = nullptr;
+    rg_timer_begin(P, RGK_INVERSE, &stop);
+    const rg_cqv cq = rg_cq_view(P, nb, with_disorder);
+    const double2 *ad = P->d_adiag + (size_t)slot * P->np;
+    switch (P->np / 16) {
+#define RG_NP(W)                                                                                              \
+    case W:                                                                                                   \
+        hipLaunchKernelGGL(k_inverse_nopiv<W>, dim3(nb), dim3(64 * W), 0, P->stream, P->d_gss32, cq,           \
+                           P->p.boundtype, ad, P->d_b, P->nmu, first, redo);                                 \
+        break;
+        RG_NP(2)
+        RG_NP(3)
+        RG_NP(4)
+        RG_NP(5)
+        RG_NP(6)
+        RG_NP(7)
+        RG_NP(8)
+#undef RG_NP
+        default: break;
+    }
+    rg_timer_end(P, RGK_INVERSE, stop);
+}

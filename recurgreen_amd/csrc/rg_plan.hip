@@ -47,7 +47,7 @@ void free_plan(rg_plan *P)
     void *bufs[] = {P->d_b, P->d_adiag, P->d_sqrtmu, P->d_tab, P->d_gss, P->d_gs1[0], P->d_gs1[1],
                     P->d_tmat, P->d_cq, P->d_teig, P->d_upot, P->d_status, P->d_hist, P->d_bins,
                     P->d_gss32, P->d_gs1_32[0], P->d_gs1_32[1], P->d_g11, P->d_g1s[0], P->d_g1s[1],
-                    P->d_sqrtremu, P->d_ck, P->d_seg, P->d_adj, P->d_loss, P->d_grad, P->d_bconj};
+                    P->d_sqrtremu, P->d_redo, P->d_ck, P->d_seg, P->d_adj, P->d_loss, P->d_grad, P->d_bconj};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &t : P->timers) {
@@ -180,7 +180,7 @@ rg_status alloc_batch(rg_plan *P)
     rg_status st;
     if ((st = dalloc(&P->d_gss, B * mat)) || (st = dalloc(&P->d_gs1[0], B * mat)) || (st = dalloc(&P->d_gs1[1], B * mat)) ||
         (st = dalloc(&P->d_tmat, B * mat)) || (st = dalloc(&P->d_cq, B * P->ntab * P->cq_parts)) ||
-        (st = dalloc(&P->d_teig, B * P->neigv)) || (st = dalloc(&P->d_status, B)) ||
+        (st = dalloc(&P->d_teig, B * P->neigv)) || (st = dalloc(&P->d_status, B)) || (st = dalloc(&P->d_redo, B)) ||
         (st = dalloc(&P->d_gss32, B * mat)) || (st = dalloc(&P->d_gs1_32[0], B * mat)) ||
         (st = dalloc(&P->d_gs1_32[1], B * mat)))
         return st;
@@ -218,10 +218,14 @@ void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int prec
         const bool dis = id >= 1 && id <= P->nxdiso;
         if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
         const int slot = ix == 1 ? 0 : (ix == nx ? 2 : 1);
-        const int inv = precision == RG_FP64 ? P->inv_fp64 : P->inv_mixed;
+        int inv = precision == RG_FP64 ? P->inv_fp64 : P->inv_mixed;
+        if (inv == RG_INV_NOPIV && P->np < 32) inv = RG_INV_LOOKAHEAD;
         if (inv == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
         else if (inv == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision);
-        else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, precision);
+        else if (inv == RG_INV_NOPIV) {
+            rg_launch_inverse_nopiv(P, nb, slot, ix == 1, dis, P->d_redo);
+            rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision, P->d_redo);
+        } else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, precision);
         hipEvent_t stop = nullptr;
         if (precision == RG_FP64) {
             if (ix == 1) {
@@ -381,6 +385,7 @@ int inverse_choice(const char *v, int dflt, bool fp64)
     if (fp64 && std::strcmp(v, "register") == 0) return RG_INV_REGISTER;
     if (std::strcmp(v, "lookahead") == 0) return RG_INV_LOOKAHEAD;
     if (std::strcmp(v, "blocked") == 0) return RG_INV_BLOCKED;
+    if (!fp64 && std::strcmp(v, "verified") == 0) return RG_INV_NOPIV;
     return dflt;
 }
 
@@ -410,7 +415,7 @@ rg_status harvest_status(rg_plan *P, int nb, int32_t *status_host, int32_t *wors
 // Timing.
 // ------------------------------------------------------------------------------------------------
 static const char *kTimerNames[RGK_COUNT] = {"project", "slice_inverse", "dyson_gemm", "fisher_lee_svd",
-                                             "histogram", "copy"};
+                                             "histogram", "copy", "inverse_redo"};
 
 void rg_timer_begin(rg_plan *P, int id, hipEvent_t *stop_out)
 {
@@ -468,7 +473,7 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
     P->batch_max = batch_max;
     for (int i = 0; i < RGK_COUNT; i++) P->timers[i].name = kTimerNames[i];
     // Slice-inversion kernels, overridable for measurement: RG_INVERSE_FP64 in {register,
-    // lookahead, blocked}, RG_INVERSE_MIXED in {lookahead, blocked}.
+    // lookahead, blocked}, RG_INVERSE_MIXED in {lookahead, blocked, verified}.
     P->inv_fp64 = inverse_choice(std::getenv("RG_INVERSE_FP64"), P->inv_fp64, true);
     P->inv_mixed = inverse_choice(std::getenv("RG_INVERSE_MIXED"), P->inv_mixed, false);
     // Transmission eigenvalues: Hermitian eigensolver of t^H t (default) or RG_EIGEN=jacobi.
