@@ -76,8 +76,12 @@ struct rg_cqv {
 __device__ inline void rg_load_cq(double *s_cq, const rg_cqv &v, int b, int tid, int nthreads)
 {
     for (int q = tid; q < v.ntab; q += nthreads) {
+        double part[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) part[k] = k < v.parts ? v.cq[((size_t)k * v.nb + b) * v.ntab + q] : 0.0;
         double s = 0.0;
-        for (int k = 0; k < v.parts; k++) s += v.cq[((size_t)k * v.nb + b) * v.ntab + q];
+#pragma unroll
+        for (int k = 0; k < 8; k++) s += part[k];  // part order: the same sum on every path
         s_cq[q] = s * v.scale;
     }
 }

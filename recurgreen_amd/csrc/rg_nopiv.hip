@@ -29,13 +29,15 @@ namespace {
 
 typedef float np_f4 __attribute__((ext_vector_type(4)));
 constexpr double NP_MARGIN = 1.0 - 1e-4;
+constexpr int LDW = 20;  // LDS row stride (floats): 16-byte aligned rows, four k values per read
 
 template <int WV>
 struct NopivShared {
     static constexpr int NP = 16 * WV;
-    float pr[2][NP][17], pi[2][NP][17];  // P (column block, rows outside the block; zero inside)
-    float dr[2][16][17], di[2][16][17];  // D^-1
-    float wr[2][16][17], wi[2][16][17];  // W = U_D^-1 diag(u)
+    alignas(16) float pr[2][NP][LDW], pi[2][NP][LDW];  // P (column block, rows outside the block; zero inside)
+    alignas(16) float dr[2][16][LDW], di[2][16][LDW];  // D^-1, row-major
+    alignas(16) float dtr[2][16][LDW], dti[2][16][LDW];  // D^-1, transposed (B operand reads)
+    alignas(16) float wtr[2][16][LDW], wti[2][16][LDW];  // W = U_D^-1 diag(u), transposed
     double ua[2][16];                    // |u_j|_1 (FP64)
     int bad;
     alignas(16) double cq[2 * 256 + 8];
@@ -43,10 +45,13 @@ struct NopivShared {
 
 __device__ inline float shfl_f(float v, int src) { return __shfl(v, src, 64); }
 
+__device__ inline float readlane_f(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
+
 // Gauss-Jordan on the 16x16 block (xr, xi) held in accumulator layout by one wavefront, diagonal
-// pivots, FP64 reciprocals.  On return (xr, xi) = D^-1, (lr, li) = L_D diag(u) (the column of the
-// reduced block at each step, rows >= step), ua[j] = |u_j|_1; returns false if a pivot fails the
-// zgetrf check or is zero.
+// pivots.  On return (xr, xi) = D^-1, (lr, li) = L_D diag(u) (the column of the reduced block at
+// each step, rows >= step), ua[j] = |u_j|_1; returns false if a pivot fails the zgetrf check (FP64
+// comparisons) or is zero.  The pivot comes from one lane (readlane) and its reciprocal is formed
+// while the pivot row and column travel through the lane shuffles, which keeps one step short.
 __device__ __attribute__((always_inline)) inline bool block_gj(np_f4 &xr, np_f4 &xi, np_f4 &lr, np_f4 &li, double *ua,
                                                                int lane)
 {
@@ -57,27 +62,30 @@ __device__ __attribute__((always_inline)) inline bool block_gj(np_f4 &xr, np_f4 
 #pragma unroll 1
     for (int j = 0; j < 16; j++) {
         const int vj = j & 3, src_row = c + 16 * (j >> 2);
-        // pivot row D[j][c] and pivot D[j][j]
+        // pivot D[j][j] from its lane; pivot row D[j][c]; column j of my rows D[4g+v][j]
+        const float pvr = readlane_f(xr[vj], j + 16 * (j >> 2)), pvi = readlane_f(xi[vj], j + 16 * (j >> 2));
         const float prr = shfl_f(xr[vj], src_row), pri = shfl_f(xi[vj], src_row);
-        const float pvr = shfl_f(prr, j), pvi = shfl_f(pri, j);
-        // column j of my rows: D[4g+v][j]
         float mr[4], mi[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             mr[v] = shfl_f(xr[v], j + 16 * g);
             mi[v] = shfl_f(xi[v], j + 16 * g);
         }
+        const float den = pvr * pvr + pvi * pvi;
+        const float inv = den > 0.f ? 1.0f / den : 1.0f;
+        const float dr = pvr * inv, di = -pvi * inv;
         const double apv = fabs((double)pvr) + fabs((double)pvi);
-        if (!(apv > 0.0)) ok = false;
+        ok = ok && apv > 0.0 && den > 0.f;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int row = 4 * g + v;
-            if (row > j && fabs((double)mr[v]) + fabs((double)mi[v]) > NP_MARGIN * apv) ok = false;
-            if (c == j && row >= j) { lr[v] = mr[v]; li[v] = mi[v]; }
+            const bool big = fabs((double)mr[v]) + fabs((double)mi[v]) > NP_MARGIN * apv;
+            ok = ok && !(row > j && big);
+            const bool cap = c == j && row >= j;
+            lr[v] = cap ? mr[v] : lr[v];
+            li[v] = cap ? mi[v] : li[v];
         }
         if (lane == 0) ua[j] = apv;
-        const double2 d = apv > 0.0 ? crecip(cmk((double)pvr, (double)pvi)) : cmk(1.0, 0.0);
-        const float dr = (float)d.x, di = (float)d.y;
         // scaled pivot row (column j of X reset to e_j first, so its entry is d)
         float rvr, rvi;
         if (c == j) { rvr = dr; rvi = di; }
@@ -97,49 +105,49 @@ __device__ __attribute__((always_inline)) inline bool block_gj(np_f4 &xr, np_f4 
     return __all(ok);
 }
 
-// C (tile) -= A B for one 16x16 complex tile: A rows from LDS (ar_s/ai_s [row][k], k = block
-// column), B in accumulator layout (k-step v = rows 4g + v).
-__device__ __attribute__((always_inline)) inline void tile_fnms(np_f4 &cr, np_f4 &ci, const float (*asr)[17],
-                                                                const float (*asi)[17], int r0, np_f4 br, np_f4 bi,
+// C (tile) -= A B for one 16x16 complex tile: A rows from LDS ([row][k], k = block column, four
+// consecutive k per lane in one read), B in accumulator layout (k-step v = rows 4g + v).
+__device__ __attribute__((always_inline)) inline void tile_fnms(np_f4 &cr, np_f4 &ci, const float (*asr)[LDW],
+                                                                const float (*asi)[LDW], int r0, np_f4 br, np_f4 bi,
                                                                 int lane)
 {
     const int i = lane & 15, g = lane >> 4;
+    const np_f4 a_r = *reinterpret_cast<const np_f4 *>(&asr[r0 + i][4 * g]);
+    const np_f4 a_i = *reinterpret_cast<const np_f4 *>(&asi[r0 + i][4 * g]);
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const float a_r = asr[r0 + i][4 * g + v], a_i = asi[r0 + i][4 * g + v];
-        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_r, br[v], cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, bi[v], cr, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_r, bi[v], ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_i, br[v], ci, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_r[v], br[v], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i[v], bi[v], cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_r[v], bi[v], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_i[v], br[v], ci, 0, 0, 0);
     }
 }
 
-// (pr, pi) = A B for A from LDS [16][17] (rows i, k = 4g+v) and B in accumulator layout.
-__device__ __attribute__((always_inline)) inline void tile_mul(np_f4 &cr, np_f4 &ci, const float (*asr)[17],
-                                                               const float (*asi)[17], np_f4 br, np_f4 bi, int lane)
+// C = A B for A from LDS [16][LDW] (rows i, k = 4g+v) and B in accumulator layout.
+__device__ __attribute__((always_inline)) inline void tile_mul(np_f4 &cr, np_f4 &ci, const float (*asr)[LDW],
+                                                               const float (*asi)[LDW], np_f4 br, np_f4 bi, int lane)
 {
     const int i = lane & 15, g = lane >> 4;
+    const np_f4 a_r = *reinterpret_cast<const np_f4 *>(&asr[i][4 * g]);
+    const np_f4 a_i = *reinterpret_cast<const np_f4 *>(&asi[i][4 * g]);
     cr = np_f4{0.f, 0.f, 0.f, 0.f};
     ci = np_f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const float a_r = asr[i][4 * g + v], a_i = asi[i][4 * g + v];
-        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, br[v], cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_i, bi[v], cr, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, bi[v], ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, br[v], ci, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r[v], br[v], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_i[v], bi[v], cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r[v], bi[v], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i[v], br[v], ci, 0, 0, 0);
     }
 }
 
-// B operand (accumulator layout) from an LDS 16x16 matrix: reg v of lane l = B[4g+v][l%16].
-__device__ inline void load_b(np_f4 &br, np_f4 &bi, const float (*sr)[17], const float (*si)[17], int lane)
+// B operand (accumulator layout) from a transposed LDS 16x16 matrix St[col][row]: reg v of lane l
+// = B[4g+v][l%16] = St[l%16][4g+v], one read.
+__device__ inline void load_b(np_f4 &br, np_f4 &bi, const float (*tr)[LDW], const float (*ti)[LDW], int lane)
 {
     const int c = lane & 15, g = lane >> 4;
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-        br[v] = sr[4 * g + v][c];
-        bi[v] = si[4 * g + v][c];
-    }
+    br = *reinterpret_cast<const np_f4 *>(&tr[c][4 * g]);
+    bi = *reinterpret_cast<const np_f4 *>(&ti[c][4 * g]);
 }
 
 // The owner of block K (wavefront K) inverts its diagonal tile, held in accumulator slot SLOT, and
@@ -153,18 +161,17 @@ __device__ __attribute__((always_inline)) inline void publish_inverse(const np_f
     const bool ok = block_gj(dr, di, lr, li, sh.ua[buf], lane);
     if (!ok && lane == 0) sh.bad = 1;
 #pragma unroll
-    for (int v = 0; v < 4; v++) {  // D^-1 to LDS (row-major [row][col])
+    for (int v = 0; v < 4; v++) {  // D^-1 to LDS, row-major and transposed
         sh.dr[buf][4 * g + v][c] = dr[v];
         sh.di[buf][4 * g + v][c] = di[v];
     }
+    *reinterpret_cast<np_f4 *>(&sh.dtr[buf][c][4 * g]) = dr;
+    *reinterpret_cast<np_f4 *>(&sh.dti[buf][c][4 * g]) = di;
     __builtin_amdgcn_wave_barrier();
     np_f4 wr, wi;
     tile_mul(wr, wi, sh.dr[buf], sh.di[buf], lr, li, lane);  // W = D^-1 (L_D diag(u))
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-        sh.wr[buf][4 * g + v][c] = wr[v];
-        sh.wi[buf][4 * g + v][c] = wi[v];
-    }
+    *reinterpret_cast<np_f4 *>(&sh.wtr[buf][c][4 * g]) = wr;
+    *reinterpret_cast<np_f4 *>(&sh.wti[buf][c][4 * g]) = wi;
 }
 
 // ... and the column block P of block K (rows outside the block; zero inside) to buffer K & 1,
@@ -213,10 +220,8 @@ __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict
             if (row < n && col < n) {
                 m = row == col ? adiag[row] : cmk(0.0, 0.0);
                 if (cqv.cq) {
-                    double vv;
                     if (boundtype == RG_DIRICHLET) {
-                        vv = sh.cq[abs(row - col)] - sh.cq[row + col + 2];
-                        m.x -= vv;
+                        m.x -= sh.cq[abs(row - col)] - sh.cq[row + col + 2];
                     } else {
                         const int dq = col - row;
                         const double2 dd = dq >= 0 ? cmk(sh.cq[2 * dq], sh.cq[2 * dq + 1])
@@ -233,7 +238,6 @@ __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict
             }
             ar[ti][v] = (float)m.x;
             ai[ti][v] = (float)m.y;
-            __builtin_amdgcn_sched_barrier(0);
         }
 
     // Accumulator slot s holds row block (s + K) mod WV during block step K: the block being
@@ -248,7 +252,7 @@ __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict
         // Rt = D^-1 X[K, w]; wavefront K: Rt = D^-1 and its other tiles restart from zero
         np_f4 rr, ri;
         if (w == K) {
-            load_b(rr, ri, sh.dr[buf], sh.di[buf], lane);
+            load_b(rr, ri, sh.dtr[buf], sh.dti[buf], lane);
 #pragma unroll
             for (int s = 1; s < WV; s++) {
                 ar[s] = np_f4{0.f, 0.f, 0.f, 0.f};
@@ -262,7 +266,7 @@ __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict
         // verification of the pivots of block K against the rows of block w (w > K)
         if (w > K) {
             np_f4 br, bi, vr = np_f4{0.f, 0.f, 0.f, 0.f}, vi = np_f4{0.f, 0.f, 0.f, 0.f};
-            load_b(br, bi, sh.wr[buf], sh.wi[buf], lane);
+            load_b(br, bi, sh.wtr[buf], sh.wti[buf], lane);
             tile_fnms(vr, vi, sh.pr[buf], sh.pi[buf], 16 * w, br, bi, lane);  // V = -P_w W
             bool ok = true;
             const double ua = sh.ua[buf][c];
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict
             if (!__all(ok) && lane == 0) sh.bad = 1;
         }
         // X[i, w] -= P_i Rt for the other row blocks; the next block's owner updates its diagonal
-        // tile (slot 1) first and publishes that block before the rest (look-ahead)
+        // tile (slot 1) first and inverts it before the rest (look-ahead)
         if (WV > 1) tile_fnms(ar[1 % WV], ai[1 % WV], sh.pr[buf], sh.pi[buf], 16 * ((1 + K) % WV), rr, ri, lane);
         __builtin_amdgcn_sched_barrier(0);
         if (w == K + 1) publish_inverse<WV, 1 % WV>(ar, ai, K + 1, lane, sh);
@@ -299,21 +303,21 @@ __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict
     }
     if (tid == 0) redo[b] = 0;
     // write X = inv(M): stage 16 rows at a time through the (now free) P buffers
-    float(*str)[17] = reinterpret_cast<float(*)[17]>(&sh.pr[0][0][0]);
-    float(*sti)[17] = reinterpret_cast<float(*)[17]>(&sh.pi[0][0][0]);
+    float(*str)[LDW] = sh.pr[0];
+    float(*sti)[LDW] = sh.pi[0];
     for (int ch = 0; ch < WV; ch++) {
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < WV; s++)
-            if (s == ch)
+        for (int s2 = 0; s2 < WV; s2++)
+            if (s2 == ch)
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    str[16 * w + 4 * g + v][c] = ar[s][v];  // rows of chunk ch, columns of block w
-                    sti[16 * w + 4 * g + v][c] = ai[s][v];
+                    str[16 * w + 4 * g + v][c] = ar[s2][v];  // rows of chunk ch, columns of block w
+                    sti[16 * w + 4 * g + v][c] = ai[s2][v];
                 }
         __syncthreads();
         for (int e = tid; e < 16 * NP; e += NT) {
-            const int r = e / NP, j = e % NP;  // row 16 ch + r, column j = 16 (j/16) + j%16
+            const int r = e / NP, j = e % NP;  // row 16 ch + r, column j
             G[(size_t)(16 * ch + r) * NP + j] = make_float2(str[16 * (j >> 4) + r][j & 15], sti[16 * (j >> 4) + r][j & 15]);
         }
     }
