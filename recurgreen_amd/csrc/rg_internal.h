@@ -123,6 +123,7 @@ struct rg_plan {
     double *d_loss = nullptr, *d_grad = nullptr;
     double2 *d_bconj = nullptr;  // conj(b) (np)
     double *d_cq = nullptr, *d_teig = nullptr, *d_upot = nullptr;
+    double *d_us = nullptr;  // disorder of the current slice for the batch (batch_max x ny)
     int32_t *d_status = nullptr;
     int32_t *d_redo = nullptr;  // per realization: 1 = the verified-pivot inversion handed it back
     unsigned long long *d_hist = nullptr;  // counts + missing + dropped

@@ -47,7 +47,7 @@ void free_plan(rg_plan *P)
     void *bufs[] = {P->d_b, P->d_adiag, P->d_sqrtmu, P->d_tab, P->d_gss, P->d_gs1[0], P->d_gs1[1],
                     P->d_tmat, P->d_cq, P->d_teig, P->d_upot, P->d_status, P->d_hist, P->d_bins,
                     P->d_gss32, P->d_gs1_32[0], P->d_gs1_32[1], P->d_g11, P->d_g1s[0], P->d_g1s[1],
-                    P->d_sqrtremu, P->d_redo, P->d_ck, P->d_seg, P->d_adj, P->d_loss, P->d_grad, P->d_bconj};
+                    P->d_sqrtremu, P->d_redo, P->d_us, P->d_ck, P->d_seg, P->d_adj, P->d_loss, P->d_grad, P->d_bconj};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &t : P->timers) {
@@ -180,7 +180,7 @@ rg_status alloc_batch(rg_plan *P)
     rg_status st;
     if ((st = dalloc(&P->d_gss, B * mat)) || (st = dalloc(&P->d_gs1[0], B * mat)) || (st = dalloc(&P->d_gs1[1], B * mat)) ||
         (st = dalloc(&P->d_tmat, B * mat)) || (st = dalloc(&P->d_cq, B * P->ntab * P->cq_parts)) ||
-        (st = dalloc(&P->d_teig, B * P->neigv)) || (st = dalloc(&P->d_status, B)) || (st = dalloc(&P->d_redo, B)) ||
+        (st = dalloc(&P->d_teig, B * P->neigv)) || (st = dalloc(&P->d_status, B)) || (st = dalloc(&P->d_redo, B)) || (st = dalloc(&P->d_us, B * P->ny)) ||
         (st = dalloc(&P->d_gss32, B * mat)) || (st = dalloc(&P->d_gs1_32[0], B * mat)) ||
         (st = dalloc(&P->d_gs1_32[1], B * mat)))
         return st;

@@ -9,8 +9,9 @@
 //
 // Each workgroup (4 wavefronts) owns 32 realizations (two 16-row MFMA tiles) and one contiguous
 // range of 16-cell chunks of K; wavefront w owns the table column tiles {w, w+4, ...}.  The slice
-// values U (Philox/Box-Muller generated here, or read from an explicit potential batch) are staged
-// through LDS one chunk ahead of the v_mfma_f64_16x16x4 products; table tiles stream from L2.
+// values U (generated for the slice by k_noise_slice, or read from an explicit potential batch) are
+// staged through LDS one chunk ahead of the v_mfma_f64_16x16x4 products; table tiles stream from
+// L2.
 // Partial sums of the K ranges go to cq[(part * nb + b) * ntab + q]; the slice-inversion kernels add
 // them in part order and scale by 1/ny, so the result does not depend on scheduling.
 //
@@ -27,27 +28,30 @@ constexpr int PR_TPW = (PR_MAXNT + 3) / 4;
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-// One chunk of slice values into s_u[KC][RB]: two cells per thread.
-__device__ inline void stage_chunk(double (*s_u)[PR_RB + 1], int chunk, int b0, int nb, int ixd, int64_t seed0,
-                                   const double *__restrict__ upot, int nxdiso, int ny, double stdu, int tid)
+// One chunk of slice values into s_u[KC][RB]: two cells per thread, u(b, iy) = us[b sb + iy si].
+__device__ inline void stage_chunk(double (*s_u)[PR_RB + 1], int chunk, int b0, int nb, const double *__restrict__ us,
+                                   int64_t sb, int si, int ny, int tid)
 {
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         const int e = tid + 256 * h;
         const int rr = e & (PR_RB - 1), ii = e >> 5;
         const int b = b0 + rr, iy = chunk * PR_KC + ii;
-        double u = 0.0;
-        if (b < nb && iy < ny) {
-            if (upot) u = upot[(size_t)b * nxdiso * ny + (size_t)iy * nxdiso + ixd];
-            else u = stdu * rg_normal_cell((uint64_t)(seed0 + b), 0u, ixd, iy);
-        }
-        s_u[ii][rr] = u;
+        s_u[ii][rr] = (b < nb && iy < ny) ? us[(size_t)b * sb + (size_t)iy * si] : 0.0;
     }
 }
 
-__global__ __launch_bounds__(256, 2) void k_project_mfma(int ixd, int64_t seed0, const double *__restrict__ upot,
-                                                         int nxdiso, int ny, double stdu,
-                                                         const double *__restrict__ tab, int ntp, int ntab,
+// Disorder of one slice for a batch of seeds: us[b ny + iy] = stdu * N(seed0 + b, ixd, iy), the
+// Philox4x32-10 / Box-Muller normal of the cell (generated here, at full occupancy, so that the
+// projection kernel's matrix cores are not held up by the transcendental functions).
+__global__ __launch_bounds__(256) void k_noise_slice(int ixd, int64_t seed0, int ny, double stdu, double *__restrict__ us)
+{
+    const int b = blockIdx.y, iy = blockIdx.x * 256 + threadIdx.x;
+    if (iy < ny) us[(size_t)b * ny + iy] = stdu * rg_normal_cell((uint64_t)(seed0 + b), 0u, ixd, iy);
+}
+
+__global__ __launch_bounds__(256) void k_project_mfma(const double *__restrict__ us, int64_t sb, int si, int ny,
+                                                      const double *__restrict__ tab, int ntp, int ntab,
                                                          int nchunk, int parts, double *__restrict__ cq, int nb)
 {
     __shared__ double s_u[2][PR_KC][PR_RB + 1];
@@ -63,11 +67,11 @@ __global__ __launch_bounds__(256, 2) void k_project_mfma(int ixd, int64_t seed0,
 #pragma unroll
         for (int j = 0; j < PR_TPW; j++) acc[m][j] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-    if (c0 < c1) stage_chunk(s_u[0], c0, b0, nb, ixd, seed0, upot, nxdiso, ny, stdu, tid);
+    if (c0 < c1) stage_chunk(s_u[0], c0, b0, nb, us, sb, si, ny, tid);
     __syncthreads();
     for (int c = c0; c < c1; c++) {
         const int buf = (c - c0) & 1;
-        if (c + 1 < c1) stage_chunk(s_u[buf ^ 1], c + 1, b0, nb, ixd, seed0, upot, nxdiso, ny, stdu, tid);
+        if (c + 1 < c1) stage_chunk(s_u[buf ^ 1], c + 1, b0, nb, us, sb, si, ny, tid);
         const double *trow = tab + (size_t)(c * PR_KC + lk) * ntp + li;
 #pragma unroll
         for (int kk = 0; kk < PR_KC / 4; kk++) {
@@ -117,9 +121,18 @@ void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double 
     (void)upot_nreal;
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_PROJECT, &stop);
-    hipLaunchKernelGGL(k_project_mfma, dim3((nb + PR_RB - 1) / PR_RB, P->cq_parts), dim3(256), 0, P->stream, ixd,
-                       seed0, d_upot, P->nxdiso, P->ny, P->stdu, P->d_tab, P->ntp, P->ntab, P->nchunk, P->cq_parts,
-                       P->d_cq, nb);
+    const double *us = d_upot ? d_upot + ixd : nullptr;  // explicit potentials upot(ix, iy, b)
+    int64_t sb = (int64_t)P->nxdiso * P->ny;
+    int si = P->nxdiso;
+    if (!d_upot) {  // seeded: this slice's disorder first
+        hipLaunchKernelGGL(k_noise_slice, dim3((P->ny + 255) / 256, nb), dim3(256), 0, P->stream, ixd, seed0, P->ny,
+                           P->stdu, P->d_us);
+        us = P->d_us;
+        sb = P->ny;
+        si = 1;
+    }
+    hipLaunchKernelGGL(k_project_mfma, dim3((nb + PR_RB - 1) / PR_RB, P->cq_parts), dim3(256), 0, P->stream, us, sb, si,
+                       P->ny, P->d_tab, P->ntp, P->ntab, P->nchunk, P->cq_parts, P->d_cq, nb);
     rg_timer_end(P, RGK_PROJECT, stop);
 }
 
