@@ -85,11 +85,15 @@ def accuracy(plan, t_ref, t_mix):
 
     out = {"mean_T_fp64": float(t_ref.mean().item()), "mean_T_err": float((t_mix.mean() - t_ref.mean()).abs().item()),
            "max_abs_dT": float((t_mix - t_ref).abs().max().item())}
+    out["fp64_T_above_1"] = int((t_ref > 1.0).sum().item())
+    out["mixed_T_at_1"] = int((t_mix == 1.0).sum().item())
     for bt, nb in (("linear", 50), ("chebyshev", 256)):
         e = distrib.bins(bt, nb)
         rr = distrib.rho_from_counts(plan.histogram(t_ref, e)[0], e)[:, 1]
         rm = distrib.rho_from_counts(plan.histogram(t_mix, e)[0], e)[:, 1]
-        out[f"max_rho_bin_err_{bt}{nb}"] = float(abs(rm - rr).max())
+        d = abs(rm - rr)
+        out[f"max_rho_bin_err_{bt}{nb}"] = float(d.max())
+        out[f"max_rho_bin_{bt}{nb}"] = int(d.argmax())
     return out
 
 
@@ -148,10 +152,13 @@ def main():
         dom = max(("slice_inverse", "dyson_gemm"), key=lambda k: times.get(k, {}).get("ms", 0.0))
         launches = max(1, times[dom]["launches"])
         avg_ms = times[dom]["ms"] / launches
-        per_launch = 8.0 * n ** 3 * B
+        # realizations per launch: the batch, or half of it when the plan sweeps two lanes
+        per_step = nx if dom == "slice_inverse" else nx - 1
+        lanes = max(1, round(launches / (args.steps * per_step)))
+        per_launch = 8.0 * n ** 3 * B / lanes
         runs[prec] = {"value": args.steps * B / el, "ms_per_step": el / args.steps * 1e3, "failed_realizations": bad,
                       "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in times.items() if v["launches"]},
-                      "dominant": dom, "avg_launch_ms": avg_ms,
+                      "dominant": dom, "avg_launch_ms": avg_ms, "realizations_per_launch": B // lanes,
                       "achieved_tflops": per_launch / (avg_ms * 1e-3) / 1e12}
         print(f"[bench] {prec}: {runs[prec]['value']:.1f} realizations/s, {runs[prec]['ms_per_step']:.0f} ms/step",
               file=sys.stderr, flush=True)
@@ -176,7 +183,8 @@ def main():
         "speedup_vs_fp64": r["value"] / runs["fp64"]["value"],
         "roofline": {"bound": "mfma", "kernel": r["dominant"], "achieved": r["achieved_tflops"], "peak": peak,
                      "unit": "TFLOP/s", "frac": r["achieved_tflops"] / peak, "traffic": traffic,
-                     "per_launch_flops": 8.0 * n ** 3 * B, "avg_launch_ms": r["avg_launch_ms"]},
+                     "per_launch_flops": 8.0 * n ** 3 * r["realizations_per_launch"],
+                     "avg_launch_ms": r["avg_launch_ms"]},
         "kernel_ms_per_step": r["kernel_ms_per_step"],
         "failed_realizations": r["failed_realizations"],
     }

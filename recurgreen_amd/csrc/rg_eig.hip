@@ -275,7 +275,12 @@ __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1,
         }
         if (h == 0) {
             if (VEC) s_lam = 0.5 * (a + z);
-            else teig[(size_t)b * neigv + (n - 1 - j)] = fmax(0.5 * (a + z), 0.0);
+            else {
+                // complex64 state: T projected onto [0, 1], where every exact eigenvalue lies
+                // (unitarity, no gain), so FP32 round-off does not push open channels out of range
+                const double tv = fmax(0.5 * (a + z), 0.0);
+                teig[(size_t)b * neigv + (n - 1 - j)] = std::is_same<E, float2>::value ? fmin(tv, 1.0) : tv;
+            }
         }
     }
     if constexpr (VEC) {

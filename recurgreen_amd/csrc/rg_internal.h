@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -134,8 +135,17 @@ struct rg_plan {
     // slice-inversion kernel per state precision (RG_INV_*): FP64 state, complex64 state
     int inv_fp64 = 0, inv_mixed = 3;
     bool eig_jacobi = false;  // transmission eigenvalues by one-sided Jacobi SVD of t instead of eig(t^H t)
+    bool gemm_wg = true;      // FP32 Dyson product: one realization per workgroup (NP >= 80) or 64 x 64 tiles
     rg_timer timers[RGK_COUNT];
+    // Concurrent lanes: the batch split in halves, each swept on a stream of its own by a view of
+    // this plan (same buffers, re-based pointers).  lanes = 1: one stream.
+    int lanes = 1;
+    rg_plan *lane[2] = {nullptr, nullptr};
+    hipEvent_t lane_ev[3] = {nullptr, nullptr, nullptr};
 };
+
+// Smallest half-batch worth a lane of its own.
+constexpr int kLaneMin = 128;
 
 // rg_project.hip
 void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double *d_upot, int upot_nreal);
@@ -175,6 +185,9 @@ void rg_launch_inverse_nopiv(rg_plan *P, int nb, int slot, int first, bool with_
 // rg_blocked.hip
 void rg_launch_inverse_blocked(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
 void rg_launch_gemm_sp(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new, int precision);
+// rg_dyson32.hip: the FP32 Dyson product with one realization per workgroup; false if NP is not
+// one of 80, 96, 112, 128
+bool rg_launch_gemm_wg(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new);
 void rg_launch_copy_f2(rg_plan *P, const float2 *src, float2 *dst, size_t n);
 void rg_launch_histogram(rg_plan *P, const double *d_samples, int64_t n, const double *d_bins, int nbins,
                          unsigned long long *d_counts);

@@ -478,7 +478,9 @@ __global__ __launch_bounds__(256) void k_fisher_lee_svd(const E *__restrict__ gs
             const double w = s_t[k2];
             rank += (w > v) || (w == v && k2 < j);
         }
-        if (rank < neigv) teig[(size_t)b * neigv + rank] = v;
+        // complex64 state: T is projected onto [0, 1], where every exact eigenvalue lies
+        // (unitarity, no gain), so FP32 round-off does not push open channels out of range
+        if (rank < neigv) teig[(size_t)b * neigv + rank] = std::is_same<E, float2>::value ? fmin(v, 1.0) : v;
         if (!(v == v) || v > 1e300) bad = true;
     }
     if (bad) status[b] = RG_ENONFINITE;

@@ -59,7 +59,8 @@ __device__ __attribute__((always_inline)) inline bool block_gj(np_f4 &xr, np_f4 
     bool ok = true;
     lr = np_f4{0.f, 0.f, 0.f, 0.f};
     li = np_f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
+    double my_ua = 0.0;  // lane c keeps |u_c|_1
+#pragma unroll 4
     for (int j = 0; j < 16; j++) {
         const int vj = j & 3, src_row = c + 16 * (j >> 2);
         // pivot D[j][j] from its lane; pivot row D[j][c]; column j of my rows D[4g+v][j]
@@ -72,7 +73,13 @@ __device__ __attribute__((always_inline)) inline bool block_gj(np_f4 &xr, np_f4 
             mi[v] = shfl_f(xi[v], j + 16 * g);
         }
         const float den = pvr * pvr + pvi * pvi;
-        const float inv = den > 0.f ? 1.0f / den : 1.0f;
+        // 1/den: v_rcp_f32 (1 ulp) and one Newton step, which removes its one-sided error (a
+        // per-slice bias that would otherwise build up over the sweep) at 2 FMAs of latency
+        float inv = 1.0f;
+        if (den > 0.f) {
+            const float r0 = __builtin_amdgcn_rcpf(den);
+            inv = fmaf(r0, fmaf(-den, r0, 1.0f), r0);
+        }
         const float dr = pvr * inv, di = -pvi * inv;
         const double apv = fabs((double)pvr) + fabs((double)pvi);
         ok = ok && apv > 0.0 && den > 0.f;
@@ -85,7 +92,7 @@ __device__ __attribute__((always_inline)) inline bool block_gj(np_f4 &xr, np_f4 
             lr[v] = cap ? mr[v] : lr[v];
             li[v] = cap ? mi[v] : li[v];
         }
-        if (lane == 0) ua[j] = apv;
+        my_ua = c == j ? apv : my_ua;
         // scaled pivot row (column j of X reset to e_j first, so its entry is d)
         float rvr, rvi;
         if (c == j) { rvr = dr; rvi = di; }
@@ -101,6 +108,7 @@ __device__ __attribute__((always_inline)) inline bool block_gj(np_f4 &xr, np_f4 
             xi[v] = xiv - (mur * rvi + mui * rvr);
         }
     }
+    if (lane < 16) ua[c] = my_ua;
     // every lane must agree: combine the flags over the wavefront
     return __all(ok);
 }

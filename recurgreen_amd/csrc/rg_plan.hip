@@ -55,6 +55,17 @@ void free_plan(rg_plan *P)
         for (auto &e : t.pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     }
     if (P->stream) (void)hipStreamDestroy(P->stream);
+    for (hipEvent_t e : P->lane_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (rg_plan *L : P->lane) {
+        if (!L) continue;  // a lane shares the plan's buffers: only its stream and events are its own
+        for (auto &t : L->timers) {
+            for (auto &e : t.pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+            for (auto &e : t.pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        }
+        if (L->stream) (void)hipStreamDestroy(L->stream);
+        delete L;
+    }
     (void)hipSetDevice(cur);
     delete P;
 }
@@ -203,50 +214,52 @@ struct DeviceGuard {
     ~DeviceGuard() { (void)hipSetDevice(prev); }
 };
 
-// One pass of the sweep over nb realizations (src/waveguide.f90:424-440) followed by the
-// Fisher-Lee relation and the singular values (:443-449) into teig / tmat_out.  Exactly one of
-// seed0-based generation or d_upot is used.  FP64 keeps complex128 state; the mixed modes keep
-// complex64 state with FP64 projection, pivoting and singular values.
-void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int precision, double *teig,
-               double2 *tmat_out)
+// One slice of the sweep over nb realizations (src/waveguide.f90:424-440): projection of the
+// slice's disorder, inversion of the slice matrix into Gss, and Gs1 <- -Gss B Gs1 (a copy of Gss
+// on the first slice).  *cur tracks which Gs1 buffer holds the state.  FP64 keeps complex128
+// state; the mixed modes keep complex64 state with FP64 projection, pivot checks and eigenvalues.
+void sweep_slice(rg_plan *P, int nb, int ix, int64_t seed0, const double *d_upot, int precision, int *cur)
 {
-    int cur = 0;
     const int nx = P->nx;
     const size_t mat = (size_t)nb * P->np * P->np;
-    for (int ix = 1; ix <= nx; ix++) {
-        const int id = ix - P->nxfree;  // 1-based index in the disordered region
-        const bool dis = id >= 1 && id <= P->nxdiso;
-        if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
-        const int slot = ix == 1 ? 0 : (ix == nx ? 2 : 1);
-        int inv = precision == RG_FP64 ? P->inv_fp64 : P->inv_mixed;
-        if (inv == RG_INV_NOPIV && P->np < 32) inv = RG_INV_LOOKAHEAD;
-        if (inv == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
-        else if (inv == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision);
-        else if (inv == RG_INV_NOPIV) {
-            rg_launch_inverse_nopiv(P, nb, slot, ix == 1, dis, P->d_redo);
-            rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision, P->d_redo);
-        } else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, precision);
-        hipEvent_t stop = nullptr;
-        if (precision == RG_FP64) {
-            if (ix == 1) {
-                rg_timer_begin(P, RGK_COPY, &stop);
-                (void)hipMemcpyAsync(P->d_gs1[0], P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
-                rg_timer_end(P, RGK_COPY, stop);
-            } else {
-                rg_launch_gemm_f64(P, nb, P->d_gss, P->d_gs1[cur], P->d_gs1[cur ^ 1]);
-                cur ^= 1;
-            }
+    const int id = ix - P->nxfree;  // 1-based index in the disordered region
+    const bool dis = id >= 1 && id <= P->nxdiso;
+    if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
+    const int slot = ix == 1 ? 0 : (ix == nx ? 2 : 1);
+    int inv = precision == RG_FP64 ? P->inv_fp64 : P->inv_mixed;
+    if (inv == RG_INV_NOPIV && P->np < 32) inv = RG_INV_LOOKAHEAD;
+    if (inv == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
+    else if (inv == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision);
+    else if (inv == RG_INV_NOPIV) {
+        rg_launch_inverse_nopiv(P, nb, slot, ix == 1, dis, P->d_redo);
+        rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision, P->d_redo);
+    } else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, precision);
+    hipEvent_t stop = nullptr;
+    if (precision == RG_FP64) {
+        if (ix == 1) {
+            rg_timer_begin(P, RGK_COPY, &stop);
+            (void)hipMemcpyAsync(P->d_gs1[0], P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+            rg_timer_end(P, RGK_COPY, stop);
         } else {
-            if (ix == 1) {
-                rg_timer_begin(P, RGK_COPY, &stop);
-                rg_launch_copy_f2(P, P->d_gss32, P->d_gs1_32[0], mat);
-                rg_timer_end(P, RGK_COPY, stop);
-            } else {
-                rg_launch_gemm_sp(P, nb, P->d_gss32, P->d_gs1_32[cur], P->d_gs1_32[cur ^ 1], precision);
-                cur ^= 1;
-            }
+            rg_launch_gemm_f64(P, nb, P->d_gss, P->d_gs1[*cur], P->d_gs1[*cur ^ 1]);
+            *cur ^= 1;
+        }
+    } else {
+        if (ix == 1) {
+            rg_timer_begin(P, RGK_COPY, &stop);
+            rg_launch_copy_f2(P, P->d_gss32, P->d_gs1_32[0], mat);
+            rg_timer_end(P, RGK_COPY, stop);
+        } else {
+            rg_launch_gemm_sp(P, nb, P->d_gss32, P->d_gs1_32[*cur], P->d_gs1_32[*cur ^ 1], precision);
+            *cur ^= 1;
         }
     }
+}
+
+// After the last slice: the Fisher-Lee relation and the transmission eigenvalues (:443-449) into
+// teig, or the t matrices into tmat_out.
+void sweep_finish(rg_plan *P, int nb, int precision, int cur, double *teig, double2 *tmat_out)
+{
     if (tmat_out || P->eig_jacobi || !rg_eig_supported(P)) {
         if (precision == RG_FP64) rg_launch_fisher_lee_svd_f64(P, nb, P->d_gs1[cur], teig, P->d_status, tmat_out);
         else rg_launch_fisher_lee_svd_f32(P, nb, P->d_gs1_32[cur], teig, P->d_status, tmat_out);
@@ -254,6 +267,67 @@ void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int prec
         rg_launch_tdt_eig_f64(P, nb, P->d_gs1[cur], teig, P->d_status);
     } else {
         rg_launch_tdt_eig_f32(P, nb, P->d_gs1_32[cur], teig, P->d_status);
+    }
+}
+
+// Lane k of a split batch: the realizations [off, off + nb) of the plan's buffers, launched on the
+// lane's own stream.  Every pointer the sweep touches is re-based on each call.
+static void lane_view(const rg_plan *P, rg_plan *L, int off, int nb)
+{
+    const size_t mat = (size_t)P->np * P->np;
+    L->d_gss = P->d_gss + off * mat;
+    L->d_gss32 = P->d_gss32 + off * mat;
+    for (int i = 0; i < 2; i++) {
+        L->d_gs1[i] = P->d_gs1[i] + off * mat;
+        L->d_gs1_32[i] = P->d_gs1_32[i] + off * mat;
+    }
+    // partial sums [part][nb][ntab]: lane 1's block follows lane 0's (cq_parts * off * ntab)
+    L->d_cq = P->d_cq + (size_t)P->cq_parts * off * P->ntab;
+    L->d_us = P->d_us + (size_t)off * P->ny;
+    L->d_status = P->d_status + off;
+    L->d_redo = P->d_redo + off;
+    L->inv_fp64 = P->inv_fp64;
+    L->inv_mixed = P->inv_mixed;
+    L->eig_jacobi = P->eig_jacobi;
+    L->profiling = P->profiling;
+    (void)nb;
+}
+
+// The sweep over nb realizations into teig (and tmat_out when given).  With two lanes the batch is
+// split in halves on two streams, slice launches interleaved and the second lane started one slice
+// inversion behind the first, so that one half's latency-bound inversion shares the compute units
+// with the other half's Dyson products.
+void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int precision, double *teig,
+               double2 *tmat_out)
+{
+    if (P->lanes < 2 || tmat_out || nb < 2 * kLaneMin) {
+        int cur = 0;
+        for (int ix = 1; ix <= P->nx; ix++) sweep_slice(P, nb, ix, seed0, d_upot, precision, &cur);
+        sweep_finish(P, nb, precision, cur, teig, tmat_out);
+        return;
+    }
+    const int h = (nb + 1) / 2, n[2] = {h, nb - h}, off[2] = {0, h};
+    const size_t per = (size_t)P->nxdiso * P->ny;
+    int cur[2] = {0, 0};
+    (void)hipEventRecord(P->lane_ev[0], P->stream);
+    for (int k = 0; k < 2; k++) {
+        lane_view(P, P->lane[k], off[k], n[k]);
+        (void)hipStreamWaitEvent(P->lane[k]->stream, P->lane_ev[0], 0);
+    }
+    for (int ix = 1; ix <= P->nx; ix++)
+        for (int k = 0; k < 2; k++) {
+            rg_plan *L = P->lane[k];
+            sweep_slice(L, n[k], ix, seed0 + off[k], d_upot ? d_upot + off[k] * per : nullptr, precision, &cur[k]);
+            if (ix == 1 && k == 0) {
+                (void)hipEventRecord(P->lane_ev[1], L->stream);
+                (void)hipStreamWaitEvent(P->lane[1]->stream, P->lane_ev[1], 0);
+            }
+        }
+    for (int k = 0; k < 2; k++) {
+        rg_plan *L = P->lane[k];
+        sweep_finish(L, n[k], precision, cur[k], teig + (size_t)off[k] * P->neigv, nullptr);
+        (void)hipEventRecord(P->lane_ev[1 + k], L->stream);
+        (void)hipStreamWaitEvent(P->stream, P->lane_ev[1 + k], 0);
     }
 }
 
@@ -442,18 +516,32 @@ void rg_timer_end(rg_plan *P, int id, hipEvent_t stop)
     (void)id;
 }
 
+static void harvest_one(rg_timer &t)
+{
+    for (auto &e : t.pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(e.second) == hipSuccess && hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+            t.ms += ms;
+            t.launches++;
+        }
+        t.pool.push_back(e);
+    }
+    t.pending.clear();
+}
+
+// Launch times of the plan's stream and of its lanes' streams, added into the plan's timers.
 static void harvest_timers(rg_plan *P)
 {
-    for (auto &t : P->timers) {
-        for (auto &e : t.pending) {
-            float ms = 0.f;
-            if (hipEventSynchronize(e.second) == hipSuccess && hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
-                t.ms += ms;
-                t.launches++;
-            }
-            t.pool.push_back(e);
+    for (auto &t : P->timers) harvest_one(t);
+    for (rg_plan *L : P->lane) {
+        if (!L) continue;
+        for (int i = 0; i < RGK_COUNT; i++) {
+            harvest_one(L->timers[i]);
+            P->timers[i].ms += L->timers[i].ms;
+            P->timers[i].launches += L->timers[i].launches;
+            L->timers[i].ms = 0.0;
+            L->timers[i].launches = 0;
         }
-        t.pending.clear();
     }
 }
 
@@ -479,6 +567,10 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
     // Transmission eigenvalues: Hermitian eigensolver of t^H t (default) or RG_EIGEN=jacobi.
     const char *eg = std::getenv("RG_EIGEN");
     P->eig_jacobi = eg && std::strcmp(eg, "jacobi") == 0;
+    // FP32 Dyson product: RG_DYSON=tile64 for the 64 x 64 tiled kernel instead of one workgroup
+    // per realization.
+    const char *dy = std::getenv("RG_DYSON");
+    P->gemm_wg = !(dy && std::strcmp(dy, "tile64") == 0);
     rg_status st = init_modes(P);  // argument checks first, like init_waveguide
     if (st) { delete P; return st; }
     int ndev = 0;
@@ -486,6 +578,22 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
     DeviceGuard g(device);
     if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess) { free_plan(P); return RG_EDEVICE; }
     if ((st = upload_constants(P)) || (st = alloc_batch(P))) { free_plan(P); return st; }
+    // Concurrent lanes (RG_LANES=1 or 2): views of this plan on streams of their own.
+    const char *ln = std::getenv("RG_LANES");
+    P->lanes = ln ? std::max(1, std::min(2, std::atoi(ln))) : P->lanes;
+    if (P->lanes > 1) {
+        for (hipEvent_t &e : P->lane_ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { free_plan(P); return RG_EDEVICE; }
+        for (rg_plan *&L : P->lane) {
+            L = new (std::nothrow) rg_plan(*P);
+            if (!L) { free_plan(P); return RG_ENOMEM; }
+            L->lanes = 1;
+            L->lane[0] = L->lane[1] = nullptr;
+            for (hipEvent_t &e : L->lane_ev) e = nullptr;
+            L->stream = nullptr;
+            if (hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess) { free_plan(P); return RG_EDEVICE; }
+        }
+    }
     *plan = P;
     return RG_OK;
 }
