@@ -123,6 +123,8 @@ def main():
                     help="sweep reported as `value`; the FP64 sweep is always measured beside it")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch options instead of the autotuner's (recurgreen_amd.autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run plain copies of MIB MiB (traffic-counter calibration)")
     args = ap.parse_args()
@@ -146,7 +148,15 @@ def main():
     runs = {}
     order = ["fp64"] + ([args.precision] if args.precision != "fp64" else [])
     outs = {}
+    options = {}
     for prec in order:
+        if not args.no_tune:
+            from recurgreen_amd import autotune
+
+            entry = autotune.apply(plan, params, B, prec) or autotune.tune(
+                params, B, prec, log=lambda m: print(m, file=sys.stderr, flush=True))
+            autotune.apply(plan, params, B, prec)
+            options[prec] = {k: v for k, v in entry.items() if k in dict(autotune.CANDIDATES[prec])}
         outs[prec] = torch.empty(B * plan.neigv, dtype=torch.float64, device="cuda:0")
         el, times, bad = timed_sweeps(plan, prec, B, args.steps, args.warmup, outs[prec], status)
         dom = max(("slice_inverse", "dyson_gemm"), key=lambda k: times.get(k, {}).get("ms", 0.0))
@@ -186,6 +196,7 @@ def main():
                      "per_launch_flops": 8.0 * n ** 3 * r["realizations_per_launch"],
                      "avg_launch_ms": r["avg_launch_ms"]},
         "kernel_ms_per_step": r["kernel_ms_per_step"],
+        "launch_options": options or "defaults",
         "failed_realizations": r["failed_realizations"],
     }
     if main_prec != "fp64":

@@ -138,6 +138,17 @@ rg_status rg_plan_profile(rg_plan *plan, int32_t enable);
 rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *ms, int64_t *launches,
                                int32_t *count);
 
+/* Launch options (no counterpart in the reference: the choices the autotuner makes, module
+ * recurgreen_amd.autotune).  Keys and values:
+ *   "inverse_fp64"  0 register-resident Gauss-Jordan, 1 look-ahead Gauss-Jordan, 2 blocked LU;
+ *   "inverse_mixed" 1 look-ahead, 2 blocked, 3 verified diagonal pivots (default);
+ *   "dyson"         0 64x64 output tiles, 1 one workgroup per realization (default, NP >= 80);
+ *   "project_parts" 1..8 partial sums of the projection over transverse cells;
+ *   "lanes"         1 one stream, 2 the batch split in halves on two streams;
+ *   "eigen"         0 Hermitian eigensolver of t^H t (default), 1 one-sided Jacobi SVD of t.
+ * RG_EINVAL for an unknown key or value.  Waits for the plan's stream first. */
+rg_status rg_plan_set_option(rg_plan *plan, const char *key, int32_t value);
+rg_status rg_plan_get_option(const rg_plan *plan, const char *key, int32_t *value);
 /* Human-readable text of a status code (static storage). */
 const char *rg_status_string(int32_t code);
 /* Library version string. */

@@ -25,7 +25,7 @@ EXPORTS = (
     "rg_adjoint_reserve", "rg_adjoint_potential", "rg_adjoint_potential_device",
     "rg_generate_potential", "rg_histogram",
     "rg_plan_stream", "rg_plan_synchronize", "rg_plan_profile", "rg_plan_kernel_times",
-    "rg_status_string", "rg_version",
+    "rg_plan_set_option", "rg_plan_get_option", "rg_status_string", "rg_version",
 )
 
 
@@ -88,6 +88,8 @@ def lib():
         "rg_plan_synchronize": [vp],
         "rg_plan_profile": [vp, i32],
         "rg_plan_kernel_times": [vp, i32, C.c_char_p, dp, i64p, i32p],
+        "rg_plan_set_option": [vp, C.c_char_p, i32],
+        "rg_plan_get_option": [vp, C.c_char_p, i32p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -286,6 +288,20 @@ class Plan:
 
     def synchronize(self):
         check(lib().rg_plan_synchronize(self._h))
+
+    # -- launch options (the autotuner's choices) -------------------------------------------
+    OPTIONS = ("inverse_fp64", "inverse_mixed", "dyson", "project_parts", "lanes", "eigen")
+
+    def set_option(self, key: str, value: int) -> None:
+        check(lib().rg_plan_set_option(self._h, key.encode(), int(value)))
+
+    def get_option(self, key: str) -> int:
+        v = C.c_int32(0)
+        check(lib().rg_plan_get_option(self._h, key.encode(), C.byref(v)))
+        return v.value
+
+    def options(self) -> dict:
+        return {k: self.get_option(k) for k in self.OPTIONS}
 
     def profile(self, enable: bool):
         check(lib().rg_plan_profile(self._h, int(bool(enable))))

@@ -146,6 +146,8 @@ struct rg_plan {
 
 // Smallest half-batch worth a lane of its own.
 constexpr int kLaneMin = 128;
+// Most partial sums of the projection (rg_load_cq adds at most eight).
+constexpr int kMaxCqParts = 8;
 
 // rg_project.hip
 void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double *d_upot, int upot_nreal);

@@ -190,7 +190,7 @@ rg_status alloc_batch(rg_plan *P)
     const size_t B = (size_t)P->batch_max, mat = (size_t)P->np * P->np;
     rg_status st;
     if ((st = dalloc(&P->d_gss, B * mat)) || (st = dalloc(&P->d_gs1[0], B * mat)) || (st = dalloc(&P->d_gs1[1], B * mat)) ||
-        (st = dalloc(&P->d_tmat, B * mat)) || (st = dalloc(&P->d_cq, B * P->ntab * P->cq_parts)) ||
+        (st = dalloc(&P->d_tmat, B * mat)) || (st = dalloc(&P->d_cq, B * P->ntab * kMaxCqParts)) ||
         (st = dalloc(&P->d_teig, B * P->neigv)) || (st = dalloc(&P->d_status, B)) || (st = dalloc(&P->d_redo, B)) || (st = dalloc(&P->d_us, B * P->ny)) ||
         (st = dalloc(&P->d_gss32, B * mat)) || (st = dalloc(&P->d_gs1_32[0], B * mat)) ||
         (st = dalloc(&P->d_gs1_32[1], B * mat)))
@@ -453,6 +453,25 @@ void run_adjoint_f64(rg_plan *P, int nb, int loss_kind, const double *d_upot, do
     }
 }
 
+// Streams, events and plan views of the two lanes (created once, on first use).
+rg_status ensure_lanes(rg_plan *P)
+{
+    if (P->lane[0]) return RG_OK;
+    for (hipEvent_t &e : P->lane_ev)
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return RG_EDEVICE;
+    for (rg_plan *&L : P->lane) {
+        L = new (std::nothrow) rg_plan(*P);
+        if (!L) return RG_ENOMEM;
+        L->lanes = 1;
+        L->lane[0] = L->lane[1] = nullptr;
+        for (hipEvent_t &e : L->lane_ev) e = nullptr;
+        for (auto &t : L->timers) { t.pending.clear(); t.pool.clear(); t.ms = 0.0; t.launches = 0; }
+        L->stream = nullptr;
+        if (hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess) return RG_EDEVICE;
+    }
+    return RG_OK;
+}
+
 int inverse_choice(const char *v, int dflt, bool fp64)
 {
     if (!v) return dflt;
@@ -580,20 +599,9 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
     if ((st = upload_constants(P)) || (st = alloc_batch(P))) { free_plan(P); return st; }
     // Concurrent lanes (RG_LANES=1 or 2): views of this plan on streams of their own.
     const char *ln = std::getenv("RG_LANES");
-    P->lanes = ln ? std::max(1, std::min(2, std::atoi(ln))) : P->lanes;
-    if (P->lanes > 1) {
-        for (hipEvent_t &e : P->lane_ev)
-            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { free_plan(P); return RG_EDEVICE; }
-        for (rg_plan *&L : P->lane) {
-            L = new (std::nothrow) rg_plan(*P);
-            if (!L) { free_plan(P); return RG_ENOMEM; }
-            L->lanes = 1;
-            L->lane[0] = L->lane[1] = nullptr;
-            for (hipEvent_t &e : L->lane_ev) e = nullptr;
-            L->stream = nullptr;
-            if (hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess) { free_plan(P); return RG_EDEVICE; }
-        }
-    }
+    const int lanes = ln ? std::max(1, std::min(2, std::atoi(ln))) : P->lanes;
+    if (lanes > 1 && (st = ensure_lanes(P))) { free_plan(P); return st; }
+    P->lanes = lanes;
     *plan = P;
     return RG_OK;
 }
@@ -908,6 +916,52 @@ rg_status rg_plan_profile(rg_plan *P, int32_t enable)
     harvest_timers(P);
     for (auto &t : P->timers) { t.ms = 0.0; t.launches = 0; }
     P->profiling = enable != 0;
+    return RG_OK;
+}
+
+rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
+{
+    if (!P || !key) return RG_EINVAL;
+    DeviceGuard g(P->device);
+    (void)hipStreamSynchronize(P->stream);
+    if (std::strcmp(key, "inverse_fp64") == 0) {
+        if (value != RG_INV_REGISTER && value != RG_INV_LOOKAHEAD && value != RG_INV_BLOCKED) return RG_EINVAL;
+        P->inv_fp64 = value;
+    } else if (std::strcmp(key, "inverse_mixed") == 0) {
+        if (value != RG_INV_LOOKAHEAD && value != RG_INV_BLOCKED && value != RG_INV_NOPIV) return RG_EINVAL;
+        P->inv_mixed = value;
+    } else if (std::strcmp(key, "dyson") == 0) {
+        if (value != 0 && value != 1) return RG_EINVAL;
+        P->gemm_wg = value == 1;
+    } else if (std::strcmp(key, "project_parts") == 0) {
+        if (value < 1 || value > kMaxCqParts) return RG_EINVAL;
+        P->cq_parts = value;
+    } else if (std::strcmp(key, "lanes") == 0) {
+        if (value != 1 && value != 2) return RG_EINVAL;
+        if (value == 2) {
+            rg_status st = ensure_lanes(P);
+            if (st) return st;
+        }
+        P->lanes = value;
+    } else if (std::strcmp(key, "eigen") == 0) {
+        if (value != 0 && value != 1) return RG_EINVAL;
+        P->eig_jacobi = value == 1;
+    } else {
+        return RG_EINVAL;
+    }
+    return RG_OK;
+}
+
+rg_status rg_plan_get_option(const rg_plan *P, const char *key, int32_t *value)
+{
+    if (!P || !key || !value) return RG_EINVAL;
+    if (std::strcmp(key, "inverse_fp64") == 0) *value = P->inv_fp64;
+    else if (std::strcmp(key, "inverse_mixed") == 0) *value = P->inv_mixed;
+    else if (std::strcmp(key, "dyson") == 0) *value = P->gemm_wg ? 1 : 0;
+    else if (std::strcmp(key, "project_parts") == 0) *value = P->cq_parts;
+    else if (std::strcmp(key, "lanes") == 0) *value = P->lanes;
+    else if (std::strcmp(key, "eigen") == 0) *value = P->eig_jacobi ? 1 : 0;
+    else return RG_EINVAL;
     return RG_OK;
 }
 

@@ -1,0 +1,93 @@
+"""Launch options and the autotuner (no counterpart in the reference).
+
+Every option value runs the same sweep with a different kernel or launch shape, so each must give
+the oracle's answer: complex128 sweeps to 1e-10 relative (floor 1e-13), complex64 sweeps within
+the absolute bound of the mixed-precision tests; the two-lane batch split must be bit-identical
+to one stream (each realization sees the same kernels and data).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from recurgreen_amd import autotune, binding
+
+from test_gpu_parity import CASES, MIXED_TOL, assert_T_close, say, upot_batch
+
+pytestmark = pytest.mark.gpu
+
+
+def test_option_round_trip_and_rejection():
+    say("options round trip")
+    plan = binding.Plan(CASES["c1_N20_ny205"], batch_max=2)
+    for key, values in (("inverse_fp64", (0, 1, 2)), ("inverse_mixed", (1, 2, 3)), ("dyson", (0, 1)),
+                        ("project_parts", (1, 3, 8)), ("lanes", (2, 1)), ("eigen", (1, 0))):
+        for v in values:
+            plan.set_option(key, v)
+            assert plan.get_option(key) == v
+    for key, v in (("inverse_fp64", 3), ("inverse_mixed", 0), ("project_parts", 9), ("lanes", 3), ("nonsense", 0)):
+        with pytest.raises(binding.RecurGreenError):
+            plan.set_option(key, v)
+    plan.close()
+
+
+@pytest.mark.parametrize("name", ["N37_absorb", "N100_filtered", "N128"])
+def test_every_option_value_matches_oracle(name):
+    p = CASES[name]
+    say(f"option values against the oracle {name}")
+    ow = O.Waveguide(**p)
+    U = upot_batch(ow, [4, 5])
+    To = np.stack([ow.tdistrib_potential(U[:, :, k].T) for k in range(2)])
+    plan = binding.Plan(p, batch_max=2)
+    for key, values in (("inverse_fp64", (0, 1, 2)), ("project_parts", (1, 2, 4, 8)), ("eigen", (1, 0))):
+        for v in values:
+            plan.set_option(key, v)
+            Tg, st, code = plan.tdistrib_potential(U, "fp64")
+            assert code == 0 and not st.any(), (key, v)
+            assert_T_close(Tg.reshape(2, -1).ravel(), To.ravel())
+    for key, values in (("inverse_mixed", (1, 2, 3)), ("dyson", (0, 1))):
+        for v in values:
+            plan.set_option(key, v)
+            Tg, st, code = plan.tdistrib_potential(U, "mixed_fp32")
+            assert code == 0 and not st.any(), (key, v)
+            err = np.max(np.abs(Tg.reshape(2, -1) - To))
+            print(f"[gpu-case] {name} mixed {key}={v}: max |dT| = {err:.3e}", flush=True)
+            assert err <= MIXED_TOL["mixed_fp32"], (key, v)
+    plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed_fp32"])
+def test_two_lanes_bit_identical(precision):
+    import torch
+
+    p = dict(CASES["N64"], nxdiso=12)
+    say(f"two lanes vs one stream {precision}")
+    B = 300  # two lanes need at least 2 x 128 realizations
+    plan = binding.Plan(p, batch_max=B)
+    outs = []
+    for lanes in (1, 2):
+        plan.set_option("lanes", lanes)
+        out = torch.empty(B * plan.neigv, dtype=torch.float64, device="cuda")
+        st = torch.zeros(B, dtype=torch.int32, device="cuda")
+        plan.tdistrib_device(11, B, precision, out, st)
+        plan.synchronize()
+        assert int(st.abs().sum()) == 0
+        outs.append(out.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    plan.close()
+
+
+def test_autotune_cache_round_trip(tmp_path):
+    p = CASES["c1_N20_ny205"]
+    say("autotune tiny shape")
+    path = str(tmp_path / "tune.json")
+    entry = autotune.tune(p, 16, "mixed_fp32", probe_slices=8, reps=1, path=path)
+    for key, values in autotune.CANDIDATES["mixed_fp32"]:
+        if key in entry:
+            assert entry[key] in values
+    plan = binding.Plan(p, batch_max=16)
+    got = autotune.apply(plan, p, 16, "mixed_fp32", path=path)
+    assert got is not None
+    assert plan.get_option("inverse_mixed") == entry["inverse_mixed"]
+    assert plan.get_option("project_parts") == entry["project_parts"]
+    assert autotune.apply(plan, p, 32, "mixed_fp32", path=path) is None  # other shape: not cached
+    plan.close()
