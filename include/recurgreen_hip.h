@@ -145,7 +145,9 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *   "dyson"         0 64x64 output tiles, 1 one workgroup per realization (default, NP >= 80);
  *   "project_parts" 1..8 partial sums of the projection over transverse cells;
  *   "lanes"         1 one stream, 2 the batch split in halves on two streams;
- *   "eigen"         0 Hermitian eigensolver of t^H t (default), 1 one-sided Jacobi SVD of t.
+ *   "eigen"         0 Hermitian eigensolver of t^H t (default), 1 one-sided Jacobi SVD of t;
+ *   "profile_mask"  bit k set: rg_plan_profile times the launches of entry k of
+ *                   rg_plan_kernel_times (default: all bits).
  * RG_EINVAL for an unknown key or value.  Waits for the plan's stream first. */
 rg_status rg_plan_set_option(rg_plan *plan, const char *key, int32_t value);
 rg_status rg_plan_get_option(const rg_plan *plan, const char *key, int32_t *value);

@@ -132,6 +132,7 @@ struct rg_plan {
     int hist_cap = 0;
 
     bool profiling = false;
+    int profile_mask = ~0;  // bit k: record launches of timer k (rg_kernel_id) while profiling
     // slice-inversion kernel per state precision (RG_INV_*): FP64 state, complex64 state
     int inv_fp64 = 0, inv_mixed = 3;
     bool eig_jacobi = false;  // transmission eigenvalues by one-sided Jacobi SVD of t instead of eig(t^H t)

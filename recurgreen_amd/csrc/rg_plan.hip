@@ -290,6 +290,7 @@ static void lane_view(const rg_plan *P, rg_plan *L, int off, int nb)
     L->inv_mixed = P->inv_mixed;
     L->eig_jacobi = P->eig_jacobi;
     L->profiling = P->profiling;
+    L->profile_mask = P->profile_mask;
     (void)nb;
 }
 
@@ -513,7 +514,7 @@ static const char *kTimerNames[RGK_COUNT] = {"project", "slice_inverse", "dyson_
 void rg_timer_begin(rg_plan *P, int id, hipEvent_t *stop_out)
 {
     *stop_out = nullptr;
-    if (!P->profiling) return;
+    if (!P->profiling || !((P->profile_mask >> id) & 1)) return;
     rg_timer &t = P->timers[id];
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
     if (!t.pool.empty()) {
@@ -946,6 +947,8 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
     } else if (std::strcmp(key, "eigen") == 0) {
         if (value != 0 && value != 1) return RG_EINVAL;
         P->eig_jacobi = value == 1;
+    } else if (std::strcmp(key, "profile_mask") == 0) {
+        P->profile_mask = value;
     } else {
         return RG_EINVAL;
     }
@@ -961,6 +964,7 @@ rg_status rg_plan_get_option(const rg_plan *P, const char *key, int32_t *value)
     else if (std::strcmp(key, "project_parts") == 0) *value = P->cq_parts;
     else if (std::strcmp(key, "lanes") == 0) *value = P->lanes;
     else if (std::strcmp(key, "eigen") == 0) *value = P->eig_jacobi ? 1 : 0;
+    else if (std::strcmp(key, "profile_mask") == 0) *value = P->profile_mask;
     else return RG_EINVAL;
     return RG_OK;
 }
