@@ -84,6 +84,16 @@ rg_status rg_tdistrib(rg_plan *plan, int64_t seed0, int32_t nseed, int32_t preci
  * int32, may be NULL); the call is asynchronous on the plan's stream. */
 rg_status rg_tdistrib_device(rg_plan *plan, int64_t seed0, int32_t nseed, int32_t precision,
                              double *tallvals_dev, int32_t *status_dev);
+/* Uneven batch (no counterpart in the reference, whose OpenMP loop runs identical copies of one
+ * waveguide_t): realization r has its own parameters params[r] and seed seeds[r] -- any length,
+ * absorption and disorder strength, and any width.  Realizations sharing boundary type, mode count,
+ * transverse cells, filter ranges and free length are swept together (sorted by length, the
+ * finished ones dropping out); other widths form further groups.  Output in the caller's order:
+ * the eigenvalues of realization r (descending) at tallvals[offsets[r] .. offsets[r+1]), offsets
+ * has nreal+1 entries; status (may be NULL) one rg_status per realization (RG_ENOTSUP for mode
+ * counts above 128).  Allocates one plan per group of at most batch_max realizations. */
+rg_status rg_tdistrib_uneven(const rg_params *params, const int64_t *seeds, int32_t nreal, int32_t precision,
+                             int32_t device, int32_t batch_max, double *tallvals, int64_t *offsets, int32_t *status);
 /* Explicit potentials (host upot, nxdiso*ny*nreal doubles): the path used for parity against
  * a CPU implementation fed the same potentials. */
 rg_status rg_tdistrib_potential(rg_plan *plan, const double *upot, int32_t nreal, int32_t precision,
@@ -147,7 +157,11 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *   "lanes"         1 one stream, 2 the batch split in halves on two streams;
  *   "eigen"         0 Hermitian eigensolver of t^H t (default), 1 one-sided Jacobi SVD of t;
  *   "profile_mask"  bit k set: rg_plan_profile times the launches of entry k of
- *                   rg_plan_kernel_times (default: all bits).
+ *                   rg_plan_kernel_times (default: all bits);
+ *   "graph"         1: seeded and explicit-potential sweeps run as captured HIP graphs (captured
+ *                   on first use per batch size, precision, output pointer and options; replayed
+ *                   with the seed read from device memory), when profiling is off and one lane is
+ *                   active; 0 (default): direct launches.
  * RG_EINVAL for an unknown key or value.  Waits for the plan's stream first. */
 rg_status rg_plan_set_option(rg_plan *plan, const char *key, int32_t value);
 rg_status rg_plan_get_option(const rg_plan *plan, const char *key, int32_t *value);

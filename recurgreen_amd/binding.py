@@ -22,6 +22,7 @@ BOUNDTYPES = {"dirichlet": 0, "periodic": 1}
 EXPORTS = (
     "rg_plan_create", "rg_plan_destroy", "rg_plan_dims", "rg_tdistrib", "rg_tdistrib_device",
     "rg_tdistrib_potential", "rg_transmission_potential", "rg_refl_trans", "rg_refl_trans_potential",
+    "rg_tdistrib_uneven",
     "rg_adjoint_reserve", "rg_adjoint_potential", "rg_adjoint_potential_device",
     "rg_generate_potential", "rg_histogram",
     "rg_plan_stream", "rg_plan_synchronize", "rg_plan_profile", "rg_plan_kernel_times",
@@ -89,6 +90,7 @@ def lib():
         "rg_plan_profile": [vp, i32],
         "rg_plan_kernel_times": [vp, i32, C.c_char_p, dp, i64p, i32p],
         "rg_plan_set_option": [vp, C.c_char_p, i32],
+        "rg_tdistrib_uneven": [C.POINTER(rg_params), C.POINTER(C.c_int64), i32, i32, i32, i32, vp, i64p, i32p],
         "rg_plan_get_option": [vp, C.c_char_p, i32p],
     }
     for name, args in sig.items():
@@ -124,20 +126,59 @@ def ptr(a) -> C.c_void_p:
     return C.c_void_p(int(a.data_ptr()))
 
 
+def make_params(params: dict) -> rg_params:
+    """rg_params from the keys of the &waveguide namelist (plus ny, dos_model)."""
+    return rg_params(
+        boundtype=BOUNDTYPES[params.get("boundtype", "dirichlet")],
+        nxdiso=int(params["nxdiso"]), nxfree=int(params.get("nxfree", 0)),
+        wol=float(params["wol"]), exfac=float(params.get("exfac", 1.0)),
+        aratio=float(params.get("aratio", 1.0)), dscat=float(params.get("dscat", 0.0)),
+        dabso=float(params.get("dabso", 0.0)), napera=float(params.get("napera", 1.0)),
+        naperb=float(params.get("naperb", 1.0)), ny=int(params.get("ny") or 0),
+        dos_model=int(params.get("dos_model", 0)),
+    )
+
+
+def tdistrib_uneven(params: list, seeds, precision: str = "fp64", device: int = 0, batch_max: int = 4096):
+    """rg_tdistrib_uneven: one call over realizations with their own parameters and seeds.
+    Returns (list of per-realization eigenvalue arrays in the given order, status array, code)."""
+    n = len(params)
+    arr = (rg_params * max(1, n))(*[make_params(p) for p in params])
+    sd = np.ascontiguousarray(seeds, dtype=np.int64)
+    if sd.shape != (n,):
+        raise ValueError("one seed per realization")
+    off = np.zeros(n + 1, np.int64)
+    st = np.zeros(max(1, n), np.int32)
+    # offsets first (eigenvalue counts), then the sweep: the counts follow from init alone
+    total = 0
+    for p in params:
+        probe = make_params(p)
+        total += _neigv_of(probe)
+    out = np.zeros(max(1, total), np.float64)
+    code = lib().rg_tdistrib_uneven(arr, sd.ctypes.data_as(C.POINTER(C.c_int64)), n, PRECISIONS[precision],
+                                    int(device), int(batch_max), out.ctypes.data, off.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    st.ctypes.data_as(C.POINTER(C.c_int32)))
+    vals = [out[off[r]:off[r + 1]].copy() for r in range(n)]
+    return vals, st[:n], code
+
+
+def _neigv_of(p: rg_params) -> int:
+    """Eigenvalue count of one waveguide (src/waveguide.f90:208 Dirichlet, :165 periodic), or 0
+    when the parameters are rejected (the library reports the status)."""
+    import math
+
+    m = min(p.napera, p.naperb)
+    if p.boundtype == BOUNDTYPES["dirichlet"]:
+        return max(0, int(math.floor(2.0 * m * p.wol)))
+    return max(0, 2 * int(math.floor(m * p.wol)) + 1)
+
+
 class Plan:
     """Owner of one rg_plan (device buffers, stream, mode constants)."""
 
     def __init__(self, params: dict, batch_max: int = 256, device: int = 0):
         L = lib()
-        p = rg_params(
-            boundtype=BOUNDTYPES[params.get("boundtype", "dirichlet")],
-            nxdiso=int(params["nxdiso"]), nxfree=int(params.get("nxfree", 0)),
-            wol=float(params["wol"]), exfac=float(params.get("exfac", 1.0)),
-            aratio=float(params.get("aratio", 1.0)), dscat=float(params.get("dscat", 0.0)),
-            dabso=float(params.get("dabso", 0.0)), napera=float(params.get("napera", 1.0)),
-            naperb=float(params.get("naperb", 1.0)), ny=int(params.get("ny") or 0),
-            dos_model=int(params.get("dos_model", 0)),
-        )
+        p = make_params(params)
         h = C.c_void_p()
         check(L.rg_plan_create(C.byref(p), int(batch_max), int(device), C.byref(h)))
         self._h = h
@@ -290,7 +331,7 @@ class Plan:
         check(lib().rg_plan_synchronize(self._h))
 
     # -- launch options (the autotuner's choices) -------------------------------------------
-    OPTIONS = ("inverse_fp64", "inverse_mixed", "dyson", "project_parts", "lanes", "eigen")
+    OPTIONS = ("inverse_fp64", "inverse_mixed", "dyson", "project_parts", "lanes", "eigen", "graph")
 
     def set_option(self, key: str, value: int) -> None:
         check(lib().rg_plan_set_option(self._h, key.encode(), int(value)))

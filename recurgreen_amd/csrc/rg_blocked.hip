@@ -262,9 +262,12 @@ __device__ __attribute__((always_inline)) inline void panel_factor(ACC (&ar)[WV]
 template <int WV, int PREC>
 __global__ __launch_bounds__(64 * WV, PREC == RG_FP64 ? 2 : 4) void k_inverse_blocked(
     typename PT<PREC>::S *__restrict__ gss, const rg_cqv cqv, int boundtype,
-    const double2 *__restrict__ adiag, const double2 *__restrict__ bvec, int n, int first, int32_t *__restrict__ status)
+    const double2 *__restrict__ adiag, const double2 *__restrict__ bvec, int n, int first, int32_t *__restrict__ status,
+    int cs)
 {
     constexpr int NP = 16 * WV, NT = 64 * WV;
+    adiag += (size_t)blockIdx.x * 3 * cs;  // cs = np: constants of each realization, else shared
+    bvec += (size_t)blockIdx.x * cs;
     using S = typename PT<PREC>::S;
     using T = typename PT<PREC>::T;
     using ACC = typename PT<PREC>::ACC;
@@ -384,8 +387,10 @@ __global__ __launch_bounds__(64 * WV, PREC == RG_FP64 ? 2 : 4) void k_inverse_bl
 // ------------------------------------------------------------------------------------------------
 template <int NP, int PREC>
 __global__ __launch_bounds__(256) void k_dyson_gemm_sp(const float2 *__restrict__ gss, const float2 *__restrict__ gs1,
-                                                       float2 *__restrict__ out, const double2 *__restrict__ bvec, int n)
+                                                       float2 *__restrict__ out, const double2 *__restrict__ bvec, int n,
+                                                       int cs)
 {
+    bvec += (size_t)blockIdx.y * cs;
     constexpr int TT = (NP + 63) / 64;
     // FP32: A^T [k][row], B [k][col] planes; bf16: A [row][k] and B^T [col][k] hi/lo planes
     __shared__ float AsR[16][66], AsI[16][66], BsR[16][66], BsI[16][66];
@@ -533,7 +538,7 @@ static void launch_inverse_blocked(rg_plan *P, int nb, void *gss, const rg_cqv c
 #define RG_BINV(W)                                                                                          \
     case W:                                                                                                 \
         hipLaunchKernelGGL((k_inverse_blocked<W, PREC>), dim3(nb), dim3(64 * W), 0, P->stream, g, cq, \
-                           P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status);                        \
+                           P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status, P->cstride);           \
         break;
         RG_BINV(1)
         RG_BINV(2)
@@ -568,7 +573,7 @@ static void launch_gemm_sp(rg_plan *P, int nb, const float2 *gss, const float2 *
 #define RG_SGEMM(T)                                                                                            \
     case 16 * T:                                                                                               \
         hipLaunchKernelGGL((k_dyson_gemm_sp<16 * T, PREC>), dim3(tt * tt, nb), dim3(256), 0, P->stream, gss, gs1_old, \
-                           gs1_new, P->d_b, P->nmu);                                                           \
+                           gs1_new, P->d_b, P->nmu, P->cstride);                                               \
         break;
         RG_SGEMM(1)
         RG_SGEMM(2)

@@ -29,8 +29,9 @@ struct DysonShared {
 template <int NP>
 __global__ __launch_bounds__(512, 4) void k_dyson_gemm_wg(const float2 *__restrict__ gss, const float2 *__restrict__ gs1,
                                                           float2 *__restrict__ out, const double2 *__restrict__ bvec,
-                                                          int n)
+                                                          int n, int cs)
 {
+    bvec += (size_t)blockIdx.x * cs;  // cs = np: constants of each realization, else shared
     constexpr int T = NP / 16, NK = NP / 16, NQ = NP * 8;  // tiles, k chunks, float4 per operand chunk
     constexpr int LOADS = (NQ + 511) / 512;
     __shared__ DysonShared<NP> sh;
@@ -137,16 +138,16 @@ bool rg_launch_gemm_wg(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_
 {
     switch (P->np) {
         case 80:
-            hipLaunchKernelGGL(k_dyson_gemm_wg<80>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b, P->nmu);
+            hipLaunchKernelGGL(k_dyson_gemm_wg<80>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b, P->nmu, P->cstride);
             return true;
         case 96:
-            hipLaunchKernelGGL(k_dyson_gemm_wg<96>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b, P->nmu);
+            hipLaunchKernelGGL(k_dyson_gemm_wg<96>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b, P->nmu, P->cstride);
             return true;
         case 112:
-            hipLaunchKernelGGL(k_dyson_gemm_wg<112>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b, P->nmu);
+            hipLaunchKernelGGL(k_dyson_gemm_wg<112>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b, P->nmu, P->cstride);
             return true;
         case 128:
-            hipLaunchKernelGGL(k_dyson_gemm_wg<128>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b, P->nmu);
+            hipLaunchKernelGGL(k_dyson_gemm_wg<128>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b, P->nmu, P->cstride);
             return true;
         default: return false;
     }

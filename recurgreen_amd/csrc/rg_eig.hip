@@ -61,8 +61,9 @@ template <class E, bool VEC>
 __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1, int NP, int imin, int imax, int jmin,
                                                       int jmax, const double2 *__restrict__ sqrtmu,
                                                       double *__restrict__ teig, int neigv,
-                                                      int32_t *__restrict__ status, double2 *__restrict__ xvec)
+                                                      int32_t *__restrict__ status, double2 *__restrict__ xvec, int cs)
 {
+    sqrtmu += (size_t)blockIdx.x * cs;  // cs = np: constants of each realization, else shared
     __shared__ double2 sA[EG_MAXN * (EG_MAXN + 1) / 2];
     __shared__ double2 s_taus[VEC ? EG_MAXN : 1];
     __shared__ double2 s_x[VEC ? EG_MAXN : 1];
@@ -375,13 +376,13 @@ template <class E>
 static void launch_eig(rg_plan *P, int nb, const E *gs1, double *teig, int32_t *status)
 {
     hipLaunchKernelGGL((k_tdt_eig<E, false>), dim3(nb), dim3(EG_NT), 0, P->stream, gs1, P->np, P->imin, P->imax,
-                       P->jmin, P->jmax, P->d_sqrtmu, teig, P->neigv, status, nullptr);
+                       P->jmin, P->jmax, P->d_sqrtmu, teig, P->neigv, status, nullptr, P->cstride);
 }
 
 void rg_launch_tdt_topvec_f64(rg_plan *P, int nb, const double2 *gs1, int32_t *status, double2 *xvec)
 {
     hipLaunchKernelGGL((k_tdt_eig<double2, true>), dim3(nb), dim3(EG_NT), 0, P->stream, gs1, P->np, P->imin, P->imax,
-                       P->jmin, P->jmax, P->d_sqrtmu, nullptr, P->neigv, status, xvec);
+                       P->jmin, P->jmax, P->d_sqrtmu, nullptr, P->neigv, status, xvec, P->cstride);
 }
 
 void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status)

@@ -138,6 +138,25 @@ struct rg_plan {
     bool eig_jacobi = false;  // transmission eigenvalues by one-sided Jacobi SVD of t instead of eig(t^H t)
     bool gemm_wg = true;      // FP32 Dyson product: one realization per workgroup (NP >= 80) or 64 x 64 tiles
     rg_timer timers[RGK_COUNT];
+    // Captured sweeps (option "graph"): one executable graph per shape/options/output, replayed with
+    // the seed read from the device word d_seed (seed_dev points at it while a sweep is captured).
+    struct graph_entry {
+        int nb, precision, opts;
+        const double *upot;
+        double *teig;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    // Uneven batches: cstride = np when every realization has its own mode constants (d_adiag
+    // [r][3][np], d_b / d_sqrtmu [r][np]), 0 when they are shared; per-realization seeds and
+    // disorder strengths for the noise (null: seed0 + r and stdu).
+    int cstride = 0;
+    const int64_t *d_seeds_r = nullptr;
+    const double *d_stdu_r = nullptr;
+    bool use_graph = false;
+    std::vector<graph_entry> graphs;
+    int64_t *d_seed = nullptr;
+    const int64_t *seed_dev = nullptr;
     // Concurrent lanes: the batch split in halves, each swept on a stream of its own by a view of
     // this plan (same buffers, re-based pointers).  lanes = 1: one stream.
     int lanes = 1;

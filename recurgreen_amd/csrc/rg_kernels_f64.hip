@@ -136,9 +136,11 @@ __global__ __launch_bounds__(64 * TR) void k_slice_inverse(double2 *__restrict__
                                                              int boundtype,
                                                              const double2 *__restrict__ adiag,
                                                              const double2 *__restrict__ bvec, int n, int first,
-                                                             int32_t *__restrict__ status)
+                                                             int32_t *__restrict__ status, int cs)
 {
     constexpr int NP = 16 * TR, TC = 4, NT = 64 * TR;
+    adiag += (size_t)blockIdx.x * 3 * cs;  // cs = np: constants of each realization, else shared
+    bvec += (size_t)blockIdx.x * cs;
     // staging height for the write-out: a power of two with STAGE_ROWS * NP <= 2048 (32 KiB)
     constexpr int STAGE_ROWS = NP <= 16 ? 16 : (NP <= 32 ? 64 : (NP <= 64 ? 32 : 16));
     constexpr int RT = 2 * NP;  // two pivot-row buffers
@@ -223,12 +225,14 @@ __global__ __launch_bounds__(64 * TR) void k_slice_inverse(double2 *__restrict__
 // ------------------------------------------------------------------------------------------------
 template <int NP>
 __global__ __launch_bounds__(256) void k_dyson_gemm(const double2 *__restrict__ gss, const double2 *__restrict__ gs1,
-                                                     double2 *__restrict__ out, const double2 *__restrict__ bvec, int n)
+                                                     double2 *__restrict__ out, const double2 *__restrict__ bvec, int n,
+                                                     int cs)
 {
     constexpr int TT = (NP + 63) / 64;
     __shared__ double2 As[16][64 + 1];
     __shared__ double2 Bs[16][64 + 1];
     const int b = blockIdx.y, ti = blockIdx.x / TT, tj = blockIdx.x % TT;
+    bvec += (size_t)b * cs;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const double2 *A = gss + (size_t)b * NP * NP;
     const double2 *B = gs1 + (size_t)b * NP * NP;
@@ -293,12 +297,14 @@ typedef double rg_d4 __attribute__((ext_vector_type(4)));
 
 template <int NP>
 __global__ __launch_bounds__(256) void k_dyson_gemm_mfma(const double2 *__restrict__ gss, const double2 *__restrict__ gs1,
-                                                          double2 *__restrict__ out, const double2 *__restrict__ bvec, int n)
+                                                          double2 *__restrict__ out, const double2 *__restrict__ bvec, int n,
+                                                          int cs)
 {
     constexpr int TT = (NP + 63) / 64, LD = 64 + 2;  // padded LDS rows (bank spread)
     __shared__ double AsR[16][LD], AsI[16][LD];      // A^T chunk: [k][row]
     __shared__ double BsR[16][LD], BsI[16][LD];      // (b B) chunk: [k][col]
     const int b = blockIdx.y, ti0 = (blockIdx.x / TT) * 64, tj0 = (blockIdx.x % TT) * 64;
+    bvec += (size_t)b * cs;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
@@ -392,11 +398,12 @@ template <class E>
 __global__ __launch_bounds__(256) void k_fisher_lee_svd(const E *__restrict__ gs1, int NP, int imin, int imax,
                                                          int jmin, int jmax, const double2 *__restrict__ sqrtmu,
                                                          double2 *__restrict__ work, double *__restrict__ teig,
-                                                         int neigv, int32_t *__restrict__ status, int max_sweeps)
+                                                         int neigv, int32_t *__restrict__ status, int max_sweeps, int cs)
 {
     __shared__ int s_rot;
     __shared__ double s_t[512];
     const int b = blockIdx.x, tid = threadIdx.x;
+    sqrtmu += (size_t)b * cs;
     const int m = imax - imin + 1, nn = jmax - jmin + 1;
     const bool tr = nn > m;
     const int rows = tr ? nn : m, cols = tr ? m : nn;
@@ -491,9 +498,10 @@ __global__ __launch_bounds__(256) void k_fisher_lee_svd(const E *__restrict__ gs
 template <class E>
 __global__ __launch_bounds__(256) void k_fisher_lee_out(const E *__restrict__ gs1, int NP, int imin, int imax,
                                                         int jmin, int jmax, const double2 *__restrict__ sqrtmu,
-                                                        double2 *__restrict__ tout)
+                                                        double2 *__restrict__ tout, int cs)
 {
     const int b = blockIdx.x;
+    sqrtmu += (size_t)b * cs;
     const int m = imax - imin + 1, nn = jmax - jmin + 1;
     const E *G = gs1 + (size_t)b * NP * NP;
     for (int e = threadIdx.x; e < m * nn; e += blockDim.x) {
@@ -565,7 +573,7 @@ void rg_launch_inverse_f64(rg_plan *P, int nb, int slot, int first, bool with_di
 #define RG_INV_CASE(T)                                                                                    \
     case T:                                                                                               \
         hipLaunchKernelGGL(k_slice_inverse<T>, dim3(nb), dim3(64 * T), 0, P->stream, P->d_gss, cq, \
-                           P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status);                     \
+                           P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status, P->cstride);        \
         break;
     switch (TR) {
         RG_INV_CASE(1)
@@ -591,7 +599,7 @@ void rg_launch_gemm_f64(rg_plan *P, int nb, const double2 *gss, const double2 *g
 #define RG_GEMM_CASE(T)                                                                                     \
     case 16 * T:                                                                                            \
         hipLaunchKernelGGL(k_dyson_gemm_mfma<16 * T>, dim3(tt * tt, nb), dim3(256), 0, P->stream, gss, gs1_old, gs1_new, \
-                           P->d_b, P->nmu);                                                                 \
+                           P->d_b, P->nmu, P->cstride);                                                     \
         break;
     switch (np) {
         RG_GEMM_CASE(1)
@@ -615,10 +623,10 @@ static void launch_fisher_lee_svd(rg_plan *P, int nb, const E *gs1, double *teig
     rg_timer_begin(P, RGK_SVD, &stop);
     if (tmat_out)
         hipLaunchKernelGGL(k_fisher_lee_out<E>, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->imin, P->imax,
-                           P->jmin, P->jmax, P->d_sqrtmu, tmat_out);
+                           P->jmin, P->jmax, P->d_sqrtmu, tmat_out, P->cstride);
     else
         hipLaunchKernelGGL(k_fisher_lee_svd<E>, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->imin, P->imax,
-                           P->jmin, P->jmax, P->d_sqrtmu, P->d_tmat, teig, P->neigv, status, 60);
+                           P->jmin, P->jmax, P->d_sqrtmu, P->d_tmat, teig, P->neigv, status, 60, P->cstride);
     rg_timer_end(P, RGK_SVD, stop);
 }
 

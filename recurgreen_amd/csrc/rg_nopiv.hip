@@ -204,8 +204,10 @@ template <int WV>
 __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict__ gss, const rg_cqv cqv, int boundtype,
                                                               const double2 *__restrict__ adiag,
                                                               const double2 *__restrict__ bvec, int n, int first,
-                                                              int32_t *__restrict__ redo)
+                                                              int32_t *__restrict__ redo, int cs)
 {
+    adiag += (size_t)blockIdx.x * 3 * cs;  // cs = np: constants of each realization, else shared
+    bvec += (size_t)blockIdx.x * cs;
     constexpr int NP = 16 * WV, NT = 64 * WV;
     __shared__ NopivShared<WV> sh;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, c = lane & 15, g = lane >> 4;
@@ -344,7 +346,7 @@ void rg_launch_inverse_nopiv(rg_plan *P, int nb, int slot, int first, bool with_
 #define RG_NP(W)                                                                                              \
     case W:                                                                                                   \
         hipLaunchKernelGGL(k_inverse_nopiv<W>, dim3(nb), dim3(64 * W), 0, P->stream, P->d_gss32, cq,           \
-                           P->p.boundtype, ad, P->d_b, P->nmu, first, redo);                                 \
+                           P->p.boundtype, ad, P->d_b, P->nmu, first, redo, P->cstride);                      \
         break;
         RG_NP(2)
         RG_NP(3)

@@ -4,11 +4,13 @@
 // constants of init (src/waveguide.f90:81-255) are computed once on the host in FP64, uploaded,
 // and every batch buffer is allocated at creation, so the sweep entry points never allocate.
 #include <algorithm>
+#include <array>
 #include <complex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <new>
 
 #include "rg_internal.h"
@@ -38,6 +40,8 @@ rg_status dalloc(T **p, size_t count)
     return RG_OK;
 }
 
+void free_graphs(rg_plan *P);
+
 void free_plan(rg_plan *P)
 {
     if (!P) return;
@@ -50,6 +54,8 @@ void free_plan(rg_plan *P)
                     P->d_sqrtremu, P->d_redo, P->d_us, P->d_ck, P->d_seg, P->d_adj, P->d_loss, P->d_grad, P->d_bconj};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    free_graphs(P);
+    if (P->d_seed) (void)hipFree(P->d_seed);
     for (auto &t : P->timers) {
         for (auto &e : t.pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         for (auto &e : t.pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
@@ -214,18 +220,29 @@ struct DeviceGuard {
     ~DeviceGuard() { (void)hipSetDevice(prev); }
 };
 
+void sweep_step(rg_plan *P, int nb, int ix, int slot, int ixd, int64_t seed0, const double *d_upot, int precision,
+                int *cur, bool flip);
+
 // One slice of the sweep over nb realizations (src/waveguide.f90:424-440): projection of the
 // slice's disorder, inversion of the slice matrix into Gss, and Gs1 <- -Gss B Gs1 (a copy of Gss
 // on the first slice).  *cur tracks which Gs1 buffer holds the state.  FP64 keeps complex128
 // state; the mixed modes keep complex64 state with FP64 projection, pivot checks and eigenvalues.
 void sweep_slice(rg_plan *P, int nb, int ix, int64_t seed0, const double *d_upot, int precision, int *cur)
 {
-    const int nx = P->nx;
-    const size_t mat = (size_t)nb * P->np * P->np;
     const int id = ix - P->nxfree;  // 1-based index in the disordered region
     const bool dis = id >= 1 && id <= P->nxdiso;
-    if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
-    const int slot = ix == 1 ? 0 : (ix == nx ? 2 : 1);
+    sweep_step(P, nb, ix, ix == 1 ? 0 : (ix == P->nx ? 2 : 1), dis ? id - 1 : -1, seed0, d_upot, precision, cur, true);
+}
+
+// One slice step with the slice's diagonal (slot 0 first, 1 bulk, 2 last) and disordered-slice index
+// ixd (-1: no disorder) given; flip = false leaves *cur for the caller to flip (several ranges of
+// one batch stepping together).
+void sweep_step(rg_plan *P, int nb, int ix, int slot, int ixd, int64_t seed0, const double *d_upot, int precision,
+                int *cur, bool flip)
+{
+    const size_t mat = (size_t)nb * P->np * P->np;
+    const bool dis = ixd >= 0;
+    if (dis) rg_launch_project(P, nb, ixd, seed0, d_upot, nb);
     int inv = precision == RG_FP64 ? P->inv_fp64 : P->inv_mixed;
     if (inv == RG_INV_NOPIV && P->np < 32) inv = RG_INV_LOOKAHEAD;
     if (inv == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
@@ -236,13 +253,14 @@ void sweep_slice(rg_plan *P, int nb, int ix, int64_t seed0, const double *d_upot
     } else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, precision);
     hipEvent_t stop = nullptr;
     if (precision == RG_FP64) {
-        if (ix == 1) {
+        if (ix == 1) {  // a copy kernel (complex128 as pairs of float2), so captured sweeps hold kernels only
             rg_timer_begin(P, RGK_COPY, &stop);
-            (void)hipMemcpyAsync(P->d_gs1[0], P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+            rg_launch_copy_f2(P, reinterpret_cast<const float2 *>(P->d_gss), reinterpret_cast<float2 *>(P->d_gs1[0]),
+                              2 * mat);
             rg_timer_end(P, RGK_COPY, stop);
         } else {
             rg_launch_gemm_f64(P, nb, P->d_gss, P->d_gs1[*cur], P->d_gs1[*cur ^ 1]);
-            *cur ^= 1;
+            if (flip) *cur ^= 1;
         }
     } else {
         if (ix == 1) {
@@ -251,7 +269,7 @@ void sweep_slice(rg_plan *P, int nb, int ix, int64_t seed0, const double *d_upot
             rg_timer_end(P, RGK_COPY, stop);
         } else {
             rg_launch_gemm_sp(P, nb, P->d_gss32, P->d_gs1_32[*cur], P->d_gs1_32[*cur ^ 1], precision);
-            *cur ^= 1;
+            if (flip) *cur ^= 1;
         }
     }
 }
@@ -286,12 +304,79 @@ static void lane_view(const rg_plan *P, rg_plan *L, int off, int nb)
     L->d_us = P->d_us + (size_t)off * P->ny;
     L->d_status = P->d_status + off;
     L->d_redo = P->d_redo + off;
+    if (P->cstride) {  // constants of each realization
+        L->d_adiag = P->d_adiag + (size_t)off * 3 * P->cstride;
+        L->d_b = P->d_b + (size_t)off * P->cstride;
+        L->d_sqrtmu = P->d_sqrtmu + (size_t)off * P->cstride;
+    }
+    L->cstride = P->cstride;
+    L->d_seeds_r = P->d_seeds_r ? P->d_seeds_r + off : nullptr;
+    L->d_stdu_r = P->d_stdu_r ? P->d_stdu_r + off : nullptr;
     L->inv_fp64 = P->inv_fp64;
     L->inv_mixed = P->inv_mixed;
     L->eig_jacobi = P->eig_jacobi;
     L->profiling = P->profiling;
     L->profile_mask = P->profile_mask;
     (void)nb;
+}
+
+namespace {
+__global__ void k_set_seed(int64_t *__restrict__ dst, int64_t v) { *dst = v; }
+}  // namespace
+
+// Options that change the kernels a sweep launches (part of a captured sweep's key).
+static int graph_opts(const rg_plan *P)
+{
+    return P->inv_fp64 | P->inv_mixed << 4 | (P->gemm_wg ? 1 : 0) << 8 | P->cq_parts << 9 | (P->eig_jacobi ? 1 : 0) << 13;
+}
+
+void free_graphs(rg_plan *P)
+{
+    for (auto &e : P->graphs) {
+        if (e.exec) (void)hipGraphExecDestroy(e.exec);
+        if (e.graph) (void)hipGraphDestroy(e.graph);
+    }
+    P->graphs.clear();
+}
+
+// The sweep as one executable graph: captured on first use for (nb, precision, potentials,
+// output, options), then replayed.  The seed is written to the device word d_seed by a kernel
+// launched before the replay, so one graph serves every seed range.  Returns false (nothing
+// launched) if the capture or instantiation fails; the caller then launches the sweep directly.
+bool run_sweep_graph(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int precision, double *teig)
+{
+    if (!P->d_seed && hipMalloc(&P->d_seed, sizeof(int64_t)) != hipSuccess) {
+        P->d_seed = nullptr;
+        return false;
+    }
+    const int opts = graph_opts(P);
+    rg_plan::graph_entry *hit = nullptr;
+    for (auto &e : P->graphs)
+        if (e.nb == nb && e.precision == precision && e.opts == opts && e.upot == d_upot && e.teig == teig) hit = &e;
+    if (!hit) {
+        if (P->graphs.size() >= 8) {  // keep a few shapes: drop the oldest
+            (void)hipGraphExecDestroy(P->graphs.front().exec);
+            (void)hipGraphDestroy(P->graphs.front().graph);
+            P->graphs.erase(P->graphs.begin());
+        }
+        rg_plan::graph_entry e{nb, precision, opts, d_upot, teig, nullptr, nullptr};
+        if (hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+        P->seed_dev = P->d_seed;
+        int cur = 0;
+        for (int ix = 1; ix <= P->nx; ix++) sweep_slice(P, nb, ix, 0, d_upot, precision, &cur);
+        sweep_finish(P, nb, precision, cur, teig, nullptr);
+        P->seed_dev = nullptr;
+        const hipError_t ce = hipStreamEndCapture(P->stream, &e.graph);
+        if (ce != hipSuccess || !e.graph) return false;
+        if (hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(e.graph);
+            return false;
+        }
+        P->graphs.push_back(e);
+        hit = &P->graphs.back();
+    }
+    hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(1), 0, P->stream, P->d_seed, seed0);
+    return hipGraphLaunch(hit->exec, P->stream) == hipSuccess;
 }
 
 // The sweep over nb realizations into teig (and tmat_out when given).  With two lanes the batch is
@@ -301,6 +386,9 @@ static void lane_view(const rg_plan *P, rg_plan *L, int off, int nb)
 void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int precision, double *teig,
                double2 *tmat_out)
 {
+    if (P->use_graph && !P->profiling && !tmat_out && (P->lanes < 2 || nb < 2 * kLaneMin) &&
+        run_sweep_graph(P, nb, seed0, d_upot, precision, teig))
+        return;
     if (P->lanes < 2 || tmat_out || nb < 2 * kLaneMin) {
         int cur = 0;
         for (int ix = 1; ix <= P->nx; ix++) sweep_slice(P, nb, ix, seed0, d_upot, precision, &cur);
@@ -467,6 +555,8 @@ rg_status ensure_lanes(rg_plan *P)
         L->lane[0] = L->lane[1] = nullptr;
         for (hipEvent_t &e : L->lane_ev) e = nullptr;
         for (auto &t : L->timers) { t.pending.clear(); t.pool.clear(); t.ms = 0.0; t.launches = 0; }
+        L->graphs.clear();  // graphs belong to the plan; lanes launch directly
+        L->use_graph = false;
         L->stream = nullptr;
         if (hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess) return RG_EDEVICE;
     }
@@ -500,6 +590,117 @@ rg_status harvest_status(rg_plan *P, int nb, int32_t *status_host, int32_t *wors
         if (status_host) status_host[i] = st[i];
         if (st[i] && !*worst) *worst = st[i];
     }
+    return RG_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Uneven batches: realizations with their own length, absorption, disorder strength and seed.
+// ------------------------------------------------------------------------------------------------
+struct uneven_real {
+    int idx;  // position in the caller's arrays
+    int nx, nxdiso;
+    double stdu;
+    int64_t seed;
+    std::vector<double2> adiag, b, sqrtmu;  // 3 np (first, bulk, last slice), np, np
+};
+
+// One group (same boundary, mode count, cells, filter ranges and free length) through plan P:
+// realizations sorted by slice count, longest first, in chunks of P->batch_max.  Slice ix runs
+// over the prefix of realizations still active (nx >= ix), in contiguous ranges of equal slice
+// diagonal and disorder index; those whose last slice it was get their eigenvalues at once.
+rg_status run_uneven_group(rg_plan *P, std::vector<uneven_real> &rs, int precision, double *tallvals,
+                           const int64_t *offsets, int32_t *status, int32_t *worst)
+{
+    const int np = P->np, neigv = P->neigv, cap = P->batch_max;
+    std::stable_sort(rs.begin(), rs.end(), [](const uneven_real &a, const uneven_real &b) { return a.nx > b.nx; });
+    double2 *d_ad = nullptr, *d_b = nullptr, *d_sm = nullptr;
+    int64_t *d_seeds = nullptr;
+    double *d_sd = nullptr;
+    auto release = [&]() {
+        for (void *q : {(void *)d_ad, (void *)d_b, (void *)d_sm, (void *)d_seeds, (void *)d_sd})
+            if (q) (void)hipFree(q);
+    };
+    rg_status st;
+    if ((st = dalloc(&d_ad, (size_t)cap * 3 * np)) || (st = dalloc(&d_b, (size_t)cap * np)) ||
+        (st = dalloc(&d_sm, (size_t)cap * np)) || (st = dalloc(&d_seeds, (size_t)cap)) || (st = dalloc(&d_sd, (size_t)cap))) {
+        release();
+        return st;
+    }
+    // base view: the plan's buffers with per-realization constants; seg: a range of it
+    rg_plan V(*P);
+    V.lanes = 1;
+    V.lane[0] = V.lane[1] = nullptr;
+    V.graphs.clear();
+    V.use_graph = false;
+    V.profiling = false;
+    for (auto &t : V.timers) { t.pending.clear(); t.pool.clear(); }
+    V.cstride = np;
+    V.d_adiag = d_ad;
+    V.d_b = d_b;
+    V.d_sqrtmu = d_sm;
+    V.d_seeds_r = d_seeds;
+    V.d_stdu_r = d_sd;
+    rg_plan S(V);
+    std::vector<double2> had((size_t)cap * 3 * np), hb((size_t)cap * np), hsm((size_t)cap * np);
+    std::vector<int64_t> hseed(cap);
+    std::vector<double> hsd(cap), te((size_t)cap * neigv);
+    std::vector<int32_t> sv(cap);
+    for (size_t c0 = 0; c0 < rs.size(); c0 += cap) {
+        const int nb = (int)std::min<size_t>(cap, rs.size() - c0);
+        for (int i = 0; i < nb; i++) {
+            const uneven_real &r = rs[c0 + i];
+            std::copy(r.adiag.begin(), r.adiag.end(), had.begin() + (size_t)i * 3 * np);
+            std::copy(r.b.begin(), r.b.end(), hb.begin() + (size_t)i * np);
+            std::copy(r.sqrtmu.begin(), r.sqrtmu.end(), hsm.begin() + (size_t)i * np);
+            hseed[i] = r.seed;
+            hsd[i] = r.stdu;
+        }
+        HIPCHK(hipMemcpy(d_ad, had.data(), sizeof(double2) * (size_t)nb * 3 * np, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_b, hb.data(), sizeof(double2) * (size_t)nb * np, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_sm, hsm.data(), sizeof(double2) * (size_t)nb * np, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_seeds, hseed.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_sd, hsd.data(), sizeof(double) * nb, hipMemcpyHostToDevice));
+        HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
+        int cur = 0;
+        const int nxmax = rs[c0].nx;
+        for (int ix = 1; ix <= nxmax; ix++) {
+            int nact = nb;
+            while (nact > 0 && rs[c0 + nact - 1].nx < ix) nact--;
+            auto key = [&](int i) {
+                const uneven_real &r = rs[c0 + i];
+                const int id = ix - P->nxfree;
+                return std::make_pair(ix == 1 ? 0 : (ix == r.nx ? 2 : 1), id >= 1 && id <= r.nxdiso ? id - 1 : -1);
+            };
+            for (int a = 0; a < nact;) {
+                const auto k = key(a);
+                int e = a + 1;
+                while (e < nact && key(e) == k) e++;
+                lane_view(&V, &S, a, e - a);
+                sweep_step(&S, e - a, ix, k.first, k.second, 0, nullptr, precision, &cur, false);
+                a = e;
+            }
+            if (ix > 1) cur ^= 1;
+            int f0 = nact;
+            while (f0 > 0 && rs[c0 + f0 - 1].nx == ix) f0--;
+            if (f0 < nact) {
+                lane_view(&V, &S, f0, nact - f0);
+                double *out = P->d_teig + (size_t)f0 * neigv;
+                if (precision == RG_FP64) rg_launch_tdt_eig_f64(&S, nact - f0, S.d_gs1[cur], out, S.d_status);
+                else rg_launch_tdt_eig_f32(&S, nact - f0, S.d_gs1_32[cur], out, S.d_status);
+            }
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(te.data(), P->d_teig, sizeof(double) * (size_t)nb * neigv, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(sv.data(), P->d_status, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        for (int i = 0; i < nb; i++) {
+            const int idx = rs[c0 + i].idx;
+            if (status) status[idx] = sv[i];
+            if (sv[i] && !*worst) *worst = sv[i];
+            std::copy(te.begin() + (size_t)i * neigv, te.begin() + (size_t)(i + 1) * neigv, tallvals + offsets[idx]);
+        }
+    }
+    release();
     return RG_OK;
 }
 
@@ -659,6 +860,62 @@ rg_status rg_tdistrib(rg_plan *P, int64_t seed0, int32_t nseed, int32_t precisio
         if ((st = harvest_status(P, nb, status ? status + off : nullptr, &worst))) return st;
     }
     if (P->profiling) harvest_timers(P);
+    return worst ? (rg_status)worst : RG_OK;
+}
+
+rg_status rg_tdistrib_uneven(const rg_params *params, const int64_t *seeds, int32_t nreal, int32_t precision,
+                             int32_t device, int32_t batch_max, double *tallvals, int64_t *offsets, int32_t *status)
+{
+    if (!params || !seeds || !offsets || nreal < 0 || batch_max < 1) return RG_EINVAL;
+    if (precision != RG_FP64 && precision != RG_MIXED_FP32 && precision != RG_MIXED_BF16X3) return RG_EINVAL;
+    // per-realization constants on the host; groups of realizations the kernels can batch
+    std::map<std::array<int, 9>, std::vector<uneven_real>> groups;
+    int32_t worst = 0;
+    offsets[0] = 0;
+    for (int32_t r = 0; r < nreal; r++) {
+        rg_plan t;
+        t.p = params[r];
+        const rg_status st = init_modes(&t);
+        offsets[r + 1] = offsets[r] + (st ? 0 : t.neigv);
+        if (st) {
+            if (status) status[r] = st;
+            if (!worst) worst = st;
+            continue;
+        }
+        uneven_real u;
+        u.idx = r;
+        u.nx = t.nx;
+        u.nxdiso = t.nxdiso;
+        u.stdu = t.stdu;
+        u.seed = seeds[r];
+        u.adiag.assign(3 * (size_t)t.np, make_double2(0, 0));
+        for (int m = 0; m < t.nmu; m++) {  // as upload_constants
+            u.adiag[m] = t.h_abound[m];
+            u.adiag[t.np + m] = t.h_abulk[m];
+            u.adiag[2 * t.np + m] = t.nx == 2 ? cadd(t.h_abound[m], t.h_abulk[m]) : t.h_abound[m];
+        }
+        u.b = t.h_b;
+        u.sqrtmu = t.h_sqrtmu;
+        groups[{t.p.boundtype, t.nmu, t.ny, t.imin, t.imax, t.jmin, t.jmax, t.neigv, t.nxfree}].push_back(std::move(u));
+    }
+    if (!tallvals && offsets[nreal] > 0) return RG_EINVAL;
+    for (auto &kv : groups) {
+        std::vector<uneven_real> &rs = kv.second;
+        rg_plan *P = nullptr;
+        const int cap = (int)std::min<size_t>((size_t)batch_max, rs.size());
+        rg_status st = rg_plan_create(&params[rs[0].idx], cap, device, &P);
+        if (!st && P->np > 128) st = RG_ENOTSUP;
+        if (!st) {
+            DeviceGuard g(P->device);
+            st = run_uneven_group(P, rs, precision, tallvals, offsets, status, &worst);
+        }
+        if (P) rg_plan_destroy(P);
+        if (st) {  // the whole group: set-up or device failure
+            for (const uneven_real &u : rs)
+                if (status) status[u.idx] = st;
+            if (!worst) worst = st;
+        }
+    }
     return worst ? (rg_status)worst : RG_OK;
 }
 
@@ -949,6 +1206,9 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
         P->eig_jacobi = value == 1;
     } else if (std::strcmp(key, "profile_mask") == 0) {
         P->profile_mask = value;
+    } else if (std::strcmp(key, "graph") == 0) {
+        if (value != 0 && value != 1) return RG_EINVAL;
+        P->use_graph = value == 1;
     } else {
         return RG_EINVAL;
     }
@@ -965,6 +1225,7 @@ rg_status rg_plan_get_option(const rg_plan *P, const char *key, int32_t *value)
     else if (std::strcmp(key, "lanes") == 0) *value = P->lanes;
     else if (std::strcmp(key, "eigen") == 0) *value = P->eig_jacobi ? 1 : 0;
     else if (std::strcmp(key, "profile_mask") == 0) *value = P->profile_mask;
+    else if (std::strcmp(key, "graph") == 0) *value = P->use_graph ? 1 : 0;
     else return RG_EINVAL;
     return RG_OK;
 }

@@ -43,11 +43,17 @@ __device__ inline void stage_chunk(double (*s_u)[PR_RB + 1], int chunk, int b0, 
 
 // Disorder of one slice for a batch of seeds: us[b ny + iy] = stdu * N(seed0 + b, ixd, iy), the
 // Philox4x32-10 / Box-Muller normal of the cell (generated here, at full occupancy, so that the
-// projection kernel's matrix cores are not held up by the transcendental functions).
-__global__ __launch_bounds__(256) void k_noise_slice(int ixd, int64_t seed0, int ny, double stdu, double *__restrict__ us)
+// projection kernel's matrix cores are not held up by the transcendental functions).  With
+// seedp != null the first seed is *seedp + seed0 (a captured sweep reads it from device memory).
+// An uneven batch gives every realization its own seed and strength: seeds[b], stdus[b].
+__global__ __launch_bounds__(256) void k_noise_slice(int ixd, int64_t seed0, const int64_t *__restrict__ seedp, int ny,
+                                                     double stdu, const int64_t *__restrict__ seeds,
+                                                     const double *__restrict__ stdus, double *__restrict__ us)
 {
     const int b = blockIdx.y, iy = blockIdx.x * 256 + threadIdx.x;
-    if (iy < ny) us[(size_t)b * ny + iy] = stdu * rg_normal_cell((uint64_t)(seed0 + b), 0u, ixd, iy);
+    const int64_t s = seeds ? seeds[b] : (seedp ? *seedp + seed0 : seed0) + b;
+    const double sd = stdus ? stdus[b] : stdu;
+    if (iy < ny) us[(size_t)b * ny + iy] = sd * rg_normal_cell((uint64_t)s, 0u, ixd, iy);
 }
 
 __global__ __launch_bounds__(256) void k_project_mfma(const double *__restrict__ us, int64_t sb, int si, int ny,
@@ -125,8 +131,8 @@ void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double 
     int64_t sb = (int64_t)P->nxdiso * P->ny;
     int si = P->nxdiso;
     if (!d_upot) {  // seeded: this slice's disorder first
-        hipLaunchKernelGGL(k_noise_slice, dim3((P->ny + 255) / 256, nb), dim3(256), 0, P->stream, ixd, seed0, P->ny,
-                           P->stdu, P->d_us);
+        hipLaunchKernelGGL(k_noise_slice, dim3((P->ny + 255) / 256, nb), dim3(256), 0, P->stream, ixd, seed0,
+                           P->seed_dev, P->ny, P->stdu, P->d_seeds_r, P->d_stdu_r, P->d_us);
         us = P->d_us;
         sb = P->ny;
         si = 1;

@@ -91,3 +91,41 @@ def test_autotune_cache_round_trip(tmp_path):
     assert plan.get_option("project_parts") == entry["project_parts"]
     assert autotune.apply(plan, p, 32, "mixed_fp32", path=path) is None  # other shape: not cached
     plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed_fp32"])
+def test_graph_replays_match_direct_launches(precision):
+    """Captured sweeps replayed several times, with direct sweeps and synchronisations between the
+    replays: every replay equals the direct sweep of the same seeds bit for bit."""
+    import torch
+
+    p = dict(CASES["N64"], nxdiso=12)
+    say(f"graph replays {precision}")
+    B = 64
+    plan = binding.Plan(p, batch_max=B)
+
+    out = torch.empty(B * plan.neigv, dtype=torch.float64, device="cuda")  # one output: one graph
+    st = torch.zeros(B, dtype=torch.int32, device="cuda")
+
+    def run(seed0, graph):
+        plan.set_option("graph", graph)
+        out.fill_(-1.0)
+        plan.tdistrib_device(seed0, B, precision, out, st)
+        plan.synchronize()
+        assert int(st.abs().sum()) == 0
+        return out.cpu().numpy()
+
+    direct = {s: run(s, 0) for s in (1, 101, 201)}
+    for s in (1, 101, 1, 201, 101):
+        assert np.array_equal(run(s, 1), direct[s]), s
+        assert np.array_equal(run(s + 100 if s < 201 else 1, 0), direct[s + 100 if s < 201 else 1])
+    # explicit potentials through the staging buffer
+    ow = O.Waveguide(**p)
+    U = upot_batch(ow, [3, 4])
+    plan.set_option("graph", 0)
+    T0, st0, c0 = plan.tdistrib_potential(U, precision)
+    plan.set_option("graph", 1)
+    for _ in range(3):
+        T1, st1, c1 = plan.tdistrib_potential(U, precision)
+        assert c1 == 0 and np.array_equal(T1, T0)
+    plan.close()
