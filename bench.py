@@ -115,6 +115,69 @@ def accuracy(plan, t_ref, t_mix):
     return out
 
 
+def uneven_workload(nreal, seed=12345):
+    """Uneven batch: N ~ U{16,32,64,128,256}, W = N/2 + 1/8, L/l_s ~ U[0.5, 10] with l_s = 10,
+    eps ~ U[0, 0.05 k^2] (dabso = eps L / k), dx = dy = 0.05, white-noise disorder; seeds 1..nreal."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    k, dx, ls = 2 * np.pi, 0.05, 10.0
+    out = []
+    for _ in range(nreal):
+        N = int(rng.choice([16, 32, 64, 128, 256]))
+        W = N / 2 + 0.125
+        L = ls * rng.uniform(0.5, 10.0)
+        nx = max(2, int(round(L / dx)))
+        L = nx * dx
+        eps = rng.uniform(0.0, 0.05 * k * k)
+        out.append(dict(boundtype="dirichlet", nxdiso=nx, nxfree=0, wol=W, exfac=1.0, aratio=L / W, dscat=L / ls,
+                        dabso=eps * L / k, napera=1.0, naperb=1.0, ny=int(round(W / dx)), dos_model=1))
+    return out
+
+
+def uneven_bench(args):
+    """One rg_tdistrib_uneven call per precision over the same uneven batch (wall clock around the
+    call: host set-up, one plan per width, the sweeps and the copies back)."""
+    import numpy as np
+    import torch
+
+    from recurgreen_amd import binding
+
+    torch.cuda.init()
+    ps = uneven_workload(args.uneven)
+    seeds = np.arange(1, len(ps) + 1, dtype=np.int64)
+    Ns = np.array([int(np.floor(2 * p["wol"])) for p in ps])
+    res = {}
+    for prec in ("fp64", args.precision):
+        t0 = time.perf_counter()
+        vals, st, code = binding.tdistrib_uneven(ps, seeds, prec, batch_max=args.batch)
+        dt = time.perf_counter() - t0
+        res[prec] = (vals, st, dt)
+        print(f"[bench-uneven] {prec}: {dt:.1f} s", file=sys.stderr, flush=True)
+    v64, s64, t64 = res["fp64"]
+    vm, sm, tm = res[args.precision]
+    ok = (s64 == 0) & (sm == 0)
+    flops = sum(16.0 * int(Ns[r]) ** 3 * (p["nxdiso"] - 1) + 8.0 * int(Ns[r]) ** 3 for r, p in enumerate(ps) if ok[r])
+    per_n = {}
+    for N in sorted(set(Ns.tolist())):
+        idx = [r for r in range(len(ps)) if Ns[r] == N]
+        good = [r for r in idx if ok[r]]
+        err = max((float(np.max(np.abs(vm[r] - v64[r]))) for r in good), default=None)
+        per_n[str(N)] = {"realizations": len(idx), "swept": len(good), "max_abs_dT_mixed_vs_fp64": err,
+                         "status": sorted(set(int(s64[r]) for r in idx))}
+    line = {
+        "metric": "realizations/s, uneven batch (c5: N in {16..256}, L/ls in [0.5, 10], eps in [0, 0.05 k^2])",
+        "value": int(ok.sum()) / tm, "unit": "realizations/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": DTYPE[args.precision], "data": "synthetic",
+        "config": {"workload": f"uneven batch of {len(ps)} realizations, seeds 1..{len(ps)}, batch_max {args.batch}, "
+                               f"precision {args.precision}"},
+        "fp64_value": int(ok.sum()) / t64, "speedup_vs_fp64": t64 / tm,
+        "algorithmic_tflops": flops / tm / 1e12, "fp64_algorithmic_tflops": flops / t64 / 1e12,
+        "per_mode_count": per_n,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(mib):
     """Plain device-to-device copies of `mib` MiB (torch's copy kernel): the reference kernel whose
     FETCH_SIZE / WRITE_SIZE counters scale the sweep kernels' counters to bytes."""
@@ -141,6 +204,8 @@ def main():
                     help="sweep reported as `value`; the FP64 sweep is always measured beside it")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--uneven", type=int, default=0, metavar="K",
+                    help="measure an uneven batch of K realizations (rg_tdistrib_uneven) instead")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch options instead of the autotuner's (recurgreen_amd.autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -150,6 +215,9 @@ def main():
         print(f"bench.py measures one GPU only (realizations are independent and would shard by seed "
               f"range across GPUs); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.uneven:
+        uneven_bench(args)
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

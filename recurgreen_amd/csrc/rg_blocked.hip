@@ -263,10 +263,11 @@ template <int WV, int PREC>
 __global__ __launch_bounds__(64 * WV, PREC == RG_FP64 ? 2 : 4) void k_inverse_blocked(
     typename PT<PREC>::S *__restrict__ gss, const rg_cqv cqv, int boundtype,
     const double2 *__restrict__ adiag, const double2 *__restrict__ bvec, int n, int first, int32_t *__restrict__ status,
-    int cs)
+    int cs, const int32_t *__restrict__ nxr, int ix)
 {
     constexpr int NP = 16 * WV, NT = 64 * WV;
     adiag += (size_t)blockIdx.x * 3 * cs;  // cs = np: constants of each realization, else shared
+    if (nxr && nxr[blockIdx.x] == ix) adiag += cs;  // this realization's last slice: its last diagonal
     bvec += (size_t)blockIdx.x * cs;
     using S = typename PT<PREC>::S;
     using T = typename PT<PREC>::T;
@@ -538,7 +539,7 @@ static void launch_inverse_blocked(rg_plan *P, int nb, void *gss, const rg_cqv c
 #define RG_BINV(W)                                                                                          \
     case W:                                                                                                 \
         hipLaunchKernelGGL((k_inverse_blocked<W, PREC>), dim3(nb), dim3(64 * W), 0, P->stream, g, cq, \
-                           P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status, P->cstride);           \
+                           P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status, P->cstride, P->d_nx_r, P->cur_ix); \
         break;
         RG_BINV(1)
         RG_BINV(2)

@@ -138,11 +138,12 @@ template <int TR, class T>
 __global__ __launch_bounds__(64 * TR, sizeof(T) == 8 ? 2 : 4) void k_gj_lookahead(
     typename CT<T>::C *__restrict__ gss, const rg_cqv cqv, int boundtype,
     const double2 *__restrict__ adiag, const double2 *__restrict__ bvec, int n, int first, int32_t *__restrict__ status,
-    const int32_t *__restrict__ only, int cs)
+    const int32_t *__restrict__ only, int cs, const int32_t *__restrict__ nxr, int ix)
 {
     using C = typename CT<T>::C;
     if (only && only[blockIdx.x] == 0) return;
     adiag += (size_t)blockIdx.x * 3 * cs;  // cs = np: constants of each realization, else shared
+    if (nxr && nxr[blockIdx.x] == ix) adiag += cs;  // this realization's last slice: its last diagonal
     bvec += (size_t)blockIdx.x * cs;
     constexpr int NP = 16 * TR, TC = 4, NT = 64 * TR;
     constexpr int STAGE_ROWS = NP <= 16 ? 16 : (NP <= 32 ? 64 : (NP <= 64 ? 32 : 16));
@@ -269,7 +270,7 @@ static void launch_gj(rg_plan *P, int nb, typename CT<T>::C *g, const rg_cqv cq,
 #define RG_GJ(W)                                                                                                \
     case W:                                                                                                     \
         hipLaunchKernelGGL((k_gj_lookahead<W, T>), dim3(nb), dim3(64 * W), 0, P->stream, g, cq,                   \
-                           P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status, only, P->cstride);         \
+                           P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status, only, P->cstride, P->d_nx_r, P->cur_ix); \
         break;
         RG_GJ(1)
         RG_GJ(2)

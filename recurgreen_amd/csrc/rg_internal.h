@@ -151,6 +151,8 @@ struct rg_plan {
     // [r][3][np], d_b / d_sqrtmu [r][np]), 0 when they are shared; per-realization seeds and
     // disorder strengths for the noise (null: seed0 + r and stdu).
     int cstride = 0;
+    const int32_t *d_nx_r = nullptr;  // slice counts: slice cur_ix uses the last diagonal where nx_r == cur_ix
+    int cur_ix = 0;
     const int64_t *d_seeds_r = nullptr;
     const double *d_stdu_r = nullptr;
     bool use_graph = false;

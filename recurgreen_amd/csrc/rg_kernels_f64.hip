@@ -136,10 +136,12 @@ __global__ __launch_bounds__(64 * TR) void k_slice_inverse(double2 *__restrict__
                                                              int boundtype,
                                                              const double2 *__restrict__ adiag,
                                                              const double2 *__restrict__ bvec, int n, int first,
-                                                             int32_t *__restrict__ status, int cs)
+                                                             int32_t *__restrict__ status, int cs,
+                                                             const int32_t *__restrict__ nxr, int ix)
 {
     constexpr int NP = 16 * TR, TC = 4, NT = 64 * TR;
     adiag += (size_t)blockIdx.x * 3 * cs;  // cs = np: constants of each realization, else shared
+    if (nxr && nxr[blockIdx.x] == ix) adiag += cs;  // this realization's last slice: its last diagonal
     bvec += (size_t)blockIdx.x * cs;
     // staging height for the write-out: a power of two with STAGE_ROWS * NP <= 2048 (32 KiB)
     constexpr int STAGE_ROWS = NP <= 16 ? 16 : (NP <= 32 ? 64 : (NP <= 64 ? 32 : 16));
@@ -573,7 +575,7 @@ void rg_launch_inverse_f64(rg_plan *P, int nb, int slot, int first, bool with_di
 #define RG_INV_CASE(T)                                                                                    \
     case T:                                                                                               \
         hipLaunchKernelGGL(k_slice_inverse<T>, dim3(nb), dim3(64 * T), 0, P->stream, P->d_gss, cq, \
-                           P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status, P->cstride);        \
+                           P->p.boundtype, ad, P->d_b, P->nmu, first, P->d_status, P->cstride, P->d_nx_r, P->cur_ix); \
         break;
     switch (TR) {
         RG_INV_CASE(1)

@@ -204,9 +204,11 @@ template <int WV>
 __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict__ gss, const rg_cqv cqv, int boundtype,
                                                               const double2 *__restrict__ adiag,
                                                               const double2 *__restrict__ bvec, int n, int first,
-                                                              int32_t *__restrict__ redo, int cs)
+                                                              int32_t *__restrict__ redo, int cs,
+                                                              const int32_t *__restrict__ nxr, int ix)
 {
     adiag += (size_t)blockIdx.x * 3 * cs;  // cs = np: constants of each realization, else shared
+    if (nxr && nxr[blockIdx.x] == ix) adiag += cs;  // this realization's last slice: its last diagonal
     bvec += (size_t)blockIdx.x * cs;
     constexpr int NP = 16 * WV, NT = 64 * WV;
     __shared__ NopivShared<WV> sh;
@@ -346,7 +348,7 @@ void rg_launch_inverse_nopiv(rg_plan *P, int nb, int slot, int first, bool with_
 #define RG_NP(W)                                                                                              \
     case W:                                                                                                   \
         hipLaunchKernelGGL(k_inverse_nopiv<W>, dim3(nb), dim3(64 * W), 0, P->stream, P->d_gss32, cq,           \
-                           P->p.boundtype, ad, P->d_b, P->nmu, first, redo, P->cstride);                      \
+                           P->p.boundtype, ad, P->d_b, P->nmu, first, redo, P->cstride, P->d_nx_r, P->cur_ix); \
         break;
         RG_NP(2)
         RG_NP(3)

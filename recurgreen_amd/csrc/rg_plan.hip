@@ -311,6 +311,8 @@ static void lane_view(const rg_plan *P, rg_plan *L, int off, int nb)
     }
     L->cstride = P->cstride;
     L->d_seeds_r = P->d_seeds_r ? P->d_seeds_r + off : nullptr;
+    L->d_nx_r = P->d_nx_r ? P->d_nx_r + off : nullptr;
+    L->cur_ix = P->cur_ix;
     L->d_stdu_r = P->d_stdu_r ? P->d_stdu_r + off : nullptr;
     L->inv_fp64 = P->inv_fp64;
     L->inv_mixed = P->inv_mixed;
@@ -606,8 +608,10 @@ struct uneven_real {
 
 // One group (same boundary, mode count, cells, filter ranges and free length) through plan P:
 // realizations sorted by slice count, longest first, in chunks of P->batch_max.  Slice ix runs
-// over the prefix of realizations still active (nx >= ix), in contiguous ranges of equal slice
-// diagonal and disorder index; those whose last slice it was get their eigenvalues at once.
+// over the prefix of realizations still active (nx >= ix) -- one launch per kernel, each
+// realization taking its last diagonal on its own last slice (nx_r) -- split only where the
+// disorder region ends (free slices).  Realizations whose last slice it was have their G(L;0)
+// copied aside; one eigenvalue launch per chunk follows the sweep.
 rg_status run_uneven_group(rg_plan *P, std::vector<uneven_real> &rs, int precision, double *tallvals,
                            const int64_t *offsets, int32_t *status, int32_t *worst)
 {
@@ -616,13 +620,15 @@ rg_status run_uneven_group(rg_plan *P, std::vector<uneven_real> &rs, int precisi
     double2 *d_ad = nullptr, *d_b = nullptr, *d_sm = nullptr;
     int64_t *d_seeds = nullptr;
     double *d_sd = nullptr;
+    int32_t *d_nxr = nullptr;
     auto release = [&]() {
-        for (void *q : {(void *)d_ad, (void *)d_b, (void *)d_sm, (void *)d_seeds, (void *)d_sd})
+        for (void *q : {(void *)d_ad, (void *)d_b, (void *)d_sm, (void *)d_seeds, (void *)d_sd, (void *)d_nxr})
             if (q) (void)hipFree(q);
     };
     rg_status st;
     if ((st = dalloc(&d_ad, (size_t)cap * 3 * np)) || (st = dalloc(&d_b, (size_t)cap * np)) ||
-        (st = dalloc(&d_sm, (size_t)cap * np)) || (st = dalloc(&d_seeds, (size_t)cap)) || (st = dalloc(&d_sd, (size_t)cap))) {
+        (st = dalloc(&d_sm, (size_t)cap * np)) || (st = dalloc(&d_seeds, (size_t)cap)) || (st = dalloc(&d_sd, (size_t)cap)) ||
+        (st = dalloc(&d_nxr, (size_t)cap))) {
         release();
         return st;
     }
@@ -640,11 +646,13 @@ rg_status run_uneven_group(rg_plan *P, std::vector<uneven_real> &rs, int precisi
     V.d_sqrtmu = d_sm;
     V.d_seeds_r = d_seeds;
     V.d_stdu_r = d_sd;
+    V.d_nx_r = d_nxr;
     rg_plan S(V);
+    const size_t mat = (size_t)np * np;
     std::vector<double2> had((size_t)cap * 3 * np), hb((size_t)cap * np), hsm((size_t)cap * np);
     std::vector<int64_t> hseed(cap);
     std::vector<double> hsd(cap), te((size_t)cap * neigv);
-    std::vector<int32_t> sv(cap);
+    std::vector<int32_t> sv(cap), hnx(cap);
     for (size_t c0 = 0; c0 < rs.size(); c0 += cap) {
         const int nb = (int)std::min<size_t>(cap, rs.size() - c0);
         for (int i = 0; i < nb; i++) {
@@ -654,41 +662,47 @@ rg_status run_uneven_group(rg_plan *P, std::vector<uneven_real> &rs, int precisi
             std::copy(r.sqrtmu.begin(), r.sqrtmu.end(), hsm.begin() + (size_t)i * np);
             hseed[i] = r.seed;
             hsd[i] = r.stdu;
+            hnx[i] = r.nx;
         }
         HIPCHK(hipMemcpy(d_ad, had.data(), sizeof(double2) * (size_t)nb * 3 * np, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_b, hb.data(), sizeof(double2) * (size_t)nb * np, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_sm, hsm.data(), sizeof(double2) * (size_t)nb * np, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_seeds, hseed.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_sd, hsd.data(), sizeof(double) * nb, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_nxr, hnx.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
         HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
         int cur = 0;
         const int nxmax = rs[c0].nx;
         for (int ix = 1; ix <= nxmax; ix++) {
             int nact = nb;
             while (nact > 0 && rs[c0 + nact - 1].nx < ix) nact--;
-            auto key = [&](int i) {
-                const uneven_real &r = rs[c0 + i];
+            V.cur_ix = ix;
+            auto ixd_of = [&](int i) {
                 const int id = ix - P->nxfree;
-                return std::make_pair(ix == 1 ? 0 : (ix == r.nx ? 2 : 1), id >= 1 && id <= r.nxdiso ? id - 1 : -1);
+                return id >= 1 && id <= rs[c0 + i].nxdiso ? id - 1 : -1;
             };
             for (int a = 0; a < nact;) {
-                const auto k = key(a);
+                const int k = ixd_of(a);
                 int e = a + 1;
-                while (e < nact && key(e) == k) e++;
+                while (e < nact && ixd_of(e) == k) e++;
                 lane_view(&V, &S, a, e - a);
-                sweep_step(&S, e - a, ix, k.first, k.second, 0, nullptr, precision, &cur, false);
+                sweep_step(&S, e - a, ix, ix == 1 ? 0 : 1, k, 0, nullptr, precision, &cur, false);
                 a = e;
             }
             if (ix > 1) cur ^= 1;
             int f0 = nact;
             while (f0 > 0 && rs[c0 + f0 - 1].nx == ix) f0--;
-            if (f0 < nact) {
-                lane_view(&V, &S, f0, nact - f0);
-                double *out = P->d_teig + (size_t)f0 * neigv;
-                if (precision == RG_FP64) rg_launch_tdt_eig_f64(&S, nact - f0, S.d_gs1[cur], out, S.d_status);
-                else rg_launch_tdt_eig_f32(&S, nact - f0, S.d_gs1_32[cur], out, S.d_status);
+            if (f0 < nact) {  // G(L;0) of the realizations that ended here, kept in the t buffer
+                const size_t n2 = (size_t)(nact - f0) * mat, o = (size_t)f0 * mat;
+                if (precision == RG_FP64)
+                    rg_launch_copy_f2(P, reinterpret_cast<const float2 *>(P->d_gs1[cur] + o),
+                                      reinterpret_cast<float2 *>(P->d_tmat + o), 2 * n2);
+                else
+                    rg_launch_copy_f2(P, P->d_gs1_32[cur] + o, reinterpret_cast<float2 *>(P->d_tmat) + o, n2);
             }
         }
+        if (precision == RG_FP64) rg_launch_tdt_eig_f64(&V, nb, P->d_tmat, P->d_teig, P->d_status);
+        else rg_launch_tdt_eig_f32(&V, nb, reinterpret_cast<const float2 *>(P->d_tmat), P->d_teig, P->d_status);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(te.data(), P->d_teig, sizeof(double) * (size_t)nb * neigv, hipMemcpyDeviceToHost, P->stream));
         HIPCHK(hipMemcpyAsync(sv.data(), P->d_status, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, P->stream));
@@ -903,8 +917,8 @@ rg_status rg_tdistrib_uneven(const rg_params *params, const int64_t *seeds, int3
         std::vector<uneven_real> &rs = kv.second;
         rg_plan *P = nullptr;
         const int cap = (int)std::min<size_t>((size_t)batch_max, rs.size());
-        rg_status st = rg_plan_create(&params[rs[0].idx], cap, device, &P);
-        if (!st && P->np > 128) st = RG_ENOTSUP;
+        // mode counts above 128 are refused before any device memory is taken
+        rg_status st = rs[0].b.size() > 128 ? RG_ENOTSUP : rg_plan_create(&params[rs[0].idx], cap, device, &P);
         if (!st) {
             DeviceGuard g(P->device);
             st = run_uneven_group(P, rs, precision, tallvals, offsets, status, &worst);
