@@ -584,6 +584,14 @@ static void launch_gemm_sp(rg_plan *P, int nb, const float2 *gss, const float2 *
         RG_SGEMM(6)
         RG_SGEMM(7)
         RG_SGEMM(8)
+        RG_SGEMM(9)
+        RG_SGEMM(10)
+        RG_SGEMM(11)
+        RG_SGEMM(12)
+        RG_SGEMM(13)
+        RG_SGEMM(14)
+        RG_SGEMM(15)
+        RG_SGEMM(16)
 #undef RG_SGEMM
         default: break;
     }

@@ -385,8 +385,11 @@ void rg_launch_tdt_topvec_f64(rg_plan *P, int nb, const double2 *gs1, int32_t *s
                        P->jmin, P->jmax, P->d_sqrtmu, nullptr, P->neigv, status, xvec, P->cstride);
 }
 
+// The Gram matrix of order min(m, nn) must fit the kernel's LDS (EG_MAXN); larger orders take the
+// one-sided Jacobi SVD instead, so no launch indexes past the LDS arrays.
 void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status)
 {
+    if (!rg_eig_supported(P)) return rg_launch_fisher_lee_svd_f64(P, nb, gs1, teig, status, nullptr);
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_SVD, &stop);
     launch_eig<double2>(P, nb, gs1, teig, status);
@@ -395,6 +398,7 @@ void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig,
 
 void rg_launch_tdt_eig_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status)
 {
+    if (!rg_eig_supported(P)) return rg_launch_fisher_lee_svd_f32(P, nb, gs1, teig, status, nullptr);
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_SVD, &stop);
     launch_eig<float2>(P, nb, gs1, teig, status);

@@ -150,6 +150,10 @@ struct rg_plan {
     // Uneven batches: cstride = np when every realization has its own mode constants (d_adiag
     // [r][3][np], d_b / d_sqrtmu [r][np]), 0 when they are shared; per-realization seeds and
     // disorder strengths for the noise (null: seed0 + r and stdu).
+    // Slice inversion above np = 128 (rg_schur.hip): block work area of batch_max realizations,
+    // views start schur_off realizations in; a zero diagonal and b = i for the block inversions.
+    double2 *d_schur = nullptr, *d_schur_zero = nullptr, *d_schur_i = nullptr;
+    int schur_off = 0;
     int cstride = 0;
     const int32_t *d_nx_r = nullptr;  // slice counts: slice cur_ix uses the last diagonal where nx_r == cur_ix
     int cur_ix = 0;
@@ -209,6 +213,10 @@ void rg_launch_inverse_nopiv(rg_plan *P, int nb, int slot, int first, bool with_
 // rg_blocked.hip
 void rg_launch_inverse_blocked(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
 void rg_launch_gemm_sp(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new, int precision);
+// rg_schur.hip: slice inversion for 128 < np <= 256 (2 x 2 Schur complement on the smaller kernels);
+// rg_schur_elems: complex128 elements of block work area per realization.
+void rg_launch_inverse_schur(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
+size_t rg_schur_elems(int np);
 // rg_dyson32.hip: the FP32 Dyson product with one realization per workgroup; false if NP is not
 // one of 80, 96, 112, 128
 bool rg_launch_gemm_wg(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new);

@@ -612,6 +612,14 @@ void rg_launch_gemm_f64(rg_plan *P, int nb, const double2 *gss, const double2 *g
         RG_GEMM_CASE(6)
         RG_GEMM_CASE(7)
         RG_GEMM_CASE(8)
+        RG_GEMM_CASE(9)
+        RG_GEMM_CASE(10)
+        RG_GEMM_CASE(11)
+        RG_GEMM_CASE(12)
+        RG_GEMM_CASE(13)
+        RG_GEMM_CASE(14)
+        RG_GEMM_CASE(15)
+        RG_GEMM_CASE(16)
         default: break;
     }
 #undef RG_GEMM_CASE

@@ -51,7 +51,8 @@ void free_plan(rg_plan *P)
     void *bufs[] = {P->d_b, P->d_adiag, P->d_sqrtmu, P->d_tab, P->d_gss, P->d_gs1[0], P->d_gs1[1],
                     P->d_tmat, P->d_cq, P->d_teig, P->d_upot, P->d_status, P->d_hist, P->d_bins,
                     P->d_gss32, P->d_gs1_32[0], P->d_gs1_32[1], P->d_g11, P->d_g1s[0], P->d_g1s[1],
-                    P->d_sqrtremu, P->d_redo, P->d_us, P->d_ck, P->d_seg, P->d_adj, P->d_loss, P->d_grad, P->d_bconj};
+                    P->d_sqrtremu, P->d_redo, P->d_us, P->d_ck, P->d_seg, P->d_adj, P->d_loss, P->d_grad, P->d_bconj,
+                    P->d_schur, P->d_schur_zero, P->d_schur_i};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_graphs(P);
@@ -209,6 +210,14 @@ rg_status alloc_batch(rg_plan *P)
     if ((st = dalloc(&P->d_g11, (size_t)P->refl_cap * mat)) || (st = dalloc(&P->d_g1s[0], (size_t)P->refl_cap * mat)) ||
         (st = dalloc(&P->d_g1s[1], (size_t)P->refl_cap * mat)))
         return st;
+    if (P->np > 128) {  // block work area of the Schur-complement inversion
+        if ((st = dalloc(&P->d_schur, B * rg_schur_elems(P->np))) || (st = dalloc(&P->d_schur_zero, 3 * 128)) ||
+            (st = dalloc(&P->d_schur_i, 128)))
+            return st;
+        std::vector<double2> zero(3 * 128, make_double2(0, 0)), iv(128, make_double2(0, 1));
+        HIPCHK(hipMemcpy(P->d_schur_zero, zero.data(), sizeof(double2) * zero.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(P->d_schur_i, iv.data(), sizeof(double2) * iv.size(), hipMemcpyHostToDevice));
+    }
     P->hist_cap = 4096;
     if ((st = dalloc(&P->d_hist, (size_t)P->hist_cap + 2)) || (st = dalloc(&P->d_bins, (size_t)P->hist_cap + 1))) return st;
     return RG_OK;
@@ -245,7 +254,8 @@ void sweep_step(rg_plan *P, int nb, int ix, int slot, int ixd, int64_t seed0, co
     if (dis) rg_launch_project(P, nb, ixd, seed0, d_upot, nb);
     int inv = precision == RG_FP64 ? P->inv_fp64 : P->inv_mixed;
     if (inv == RG_INV_NOPIV && P->np < 32) inv = RG_INV_LOOKAHEAD;
-    if (inv == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
+    if (P->np > 128) rg_launch_inverse_schur(P, nb, slot, ix == 1, dis, precision);
+    else if (inv == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
     else if (inv == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision);
     else if (inv == RG_INV_NOPIV) {
         rg_launch_inverse_nopiv(P, nb, slot, ix == 1, dis, P->d_redo);
@@ -312,6 +322,7 @@ static void lane_view(const rg_plan *P, rg_plan *L, int off, int nb)
     L->cstride = P->cstride;
     L->d_seeds_r = P->d_seeds_r ? P->d_seeds_r + off : nullptr;
     L->d_nx_r = P->d_nx_r ? P->d_nx_r + off : nullptr;
+    L->schur_off = P->schur_off + off;
     L->cur_ix = P->cur_ix;
     L->d_stdu_r = P->d_stdu_r ? P->d_stdu_r + off : nullptr;
     L->inv_fp64 = P->inv_fp64;
@@ -391,7 +402,7 @@ void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int prec
     if (P->use_graph && !P->profiling && !tmat_out && (P->lanes < 2 || nb < 2 * kLaneMin) &&
         run_sweep_graph(P, nb, seed0, d_upot, precision, teig))
         return;
-    if (P->lanes < 2 || tmat_out || nb < 2 * kLaneMin) {
+    if (P->lanes < 2 || tmat_out || nb < 2 * kLaneMin || P->np > 128) {
         int cur = 0;
         for (int ix = 1; ix <= P->nx; ix++) sweep_slice(P, nb, ix, seed0, d_upot, precision, &cur);
         sweep_finish(P, nb, precision, cur, teig, tmat_out);
@@ -579,7 +590,7 @@ rg_status check_common(rg_plan *P, int32_t n, int32_t precision)
 {
     if (!P || n < 0) return RG_EINVAL;
     if (precision != RG_FP64 && precision != RG_MIXED_FP32 && precision != RG_MIXED_BF16X3) return RG_EINVAL;
-    if (P->np > 128) return RG_ENOTSUP;
+    if (P->np > 256) return RG_ENOTSUP;
     return RG_OK;
 }
 
@@ -611,7 +622,7 @@ struct uneven_real {
 // over the prefix of realizations still active (nx >= ix) -- one launch per kernel, each
 // realization taking its last diagonal on its own last slice (nx_r) -- split only where the
 // disorder region ends (free slices).  Realizations whose last slice it was have their G(L;0)
-// copied aside; one eigenvalue launch per chunk follows the sweep.
+// copied into their own (now unused) Gss slot; one eigenvalue launch per chunk follows the sweep.
 rg_status run_uneven_group(rg_plan *P, std::vector<uneven_real> &rs, int precision, double *tallvals,
                            const int64_t *offsets, int32_t *status, int32_t *worst)
 {
@@ -692,17 +703,19 @@ rg_status run_uneven_group(rg_plan *P, std::vector<uneven_real> &rs, int precisi
             if (ix > 1) cur ^= 1;
             int f0 = nact;
             while (f0 > 0 && rs[c0 + f0 - 1].nx == ix) f0--;
-            if (f0 < nact) {  // G(L;0) of the realizations that ended here, kept in the t buffer
+            if (f0 < nact) {  // G(L;0) of the realizations that ended here, kept in their own Gss slot
+                // (no later slice touches a finished realization's slot; the t buffer stays free
+                // as the work area of the Jacobi singular values)
                 const size_t n2 = (size_t)(nact - f0) * mat, o = (size_t)f0 * mat;
                 if (precision == RG_FP64)
                     rg_launch_copy_f2(P, reinterpret_cast<const float2 *>(P->d_gs1[cur] + o),
-                                      reinterpret_cast<float2 *>(P->d_tmat + o), 2 * n2);
+                                      reinterpret_cast<float2 *>(P->d_gss + o), 2 * n2);
                 else
-                    rg_launch_copy_f2(P, P->d_gs1_32[cur] + o, reinterpret_cast<float2 *>(P->d_tmat) + o, n2);
+                    rg_launch_copy_f2(P, P->d_gs1_32[cur] + o, P->d_gss32 + o, n2);
             }
         }
-        if (precision == RG_FP64) rg_launch_tdt_eig_f64(&V, nb, P->d_tmat, P->d_teig, P->d_status);
-        else rg_launch_tdt_eig_f32(&V, nb, reinterpret_cast<const float2 *>(P->d_tmat), P->d_teig, P->d_status);
+        if (precision == RG_FP64) rg_launch_tdt_eig_f64(&V, nb, P->d_gss, P->d_teig, P->d_status);
+        else rg_launch_tdt_eig_f32(&V, nb, P->d_gss32, P->d_teig, P->d_status);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(te.data(), P->d_teig, sizeof(double) * (size_t)nb * neigv, hipMemcpyDeviceToHost, P->stream));
         HIPCHK(hipMemcpyAsync(sv.data(), P->d_status, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, P->stream));
@@ -918,7 +931,7 @@ rg_status rg_tdistrib_uneven(const rg_params *params, const int64_t *seeds, int3
         rg_plan *P = nullptr;
         const int cap = (int)std::min<size_t>((size_t)batch_max, rs.size());
         // mode counts above 128 are refused before any device memory is taken
-        rg_status st = rs[0].b.size() > 128 ? RG_ENOTSUP : rg_plan_create(&params[rs[0].idx], cap, device, &P);
+        rg_status st = rs[0].b.size() > 256 ? RG_ENOTSUP : rg_plan_create(&params[rs[0].idx], cap, device, &P);
         if (!st) {
             DeviceGuard g(P->device);
             st = run_uneven_group(P, rs, precision, tallvals, offsets, status, &worst);

@@ -7,8 +7,9 @@
 // whole batch and one slice this is the product  C[b][q] = sum_i U[b][i] tab[i][q]:
 // M = realizations, N = table columns (2N+1 or 2N), K = ny transverse cells.
 //
-// Each workgroup (4 wavefronts) owns 32 realizations (two 16-row MFMA tiles) and one contiguous
-// range of 16-cell chunks of K; wavefront w owns the table column tiles {w, w+4, ...}.  The slice
+// Each workgroup (4 wavefronts) owns 32 realizations (two 16-row MFMA tiles), one contiguous range
+// of 16-cell chunks of K and one group of up to 20 table column tiles (blockIdx.z); wavefront w owns
+// the tiles {w, w+4, ...} of its group.  The slice
 // values U (generated for the slice by k_noise_slice, or read from an explicit potential batch) are
 // staged through LDS one chunk ahead of the v_mfma_f64_16x16x4 products; table tiles stream from
 // L2.
@@ -62,9 +63,9 @@ __global__ __launch_bounds__(256) void k_project_mfma(const double *__restrict__
 {
     __shared__ double s_u[2][PR_KC][PR_RB + 1];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int b0 = blockIdx.x * PR_RB, part = blockIdx.y;
+    const int b0 = blockIdx.x * PR_RB, part = blockIdx.y, nt0 = PR_MAXNT * blockIdx.z;  // column tile group
     const int c0 = (int)((int64_t)part * nchunk / parts), c1 = (int)((int64_t)(part + 1) * nchunk / parts);
-    const int ntiles = ntp / 16;
+    const int ntiles = ntp / 16 < nt0 + PR_MAXNT ? ntp / 16 : nt0 + PR_MAXNT;
     const int li = lane & 15, lk = lane >> 4;
 
     dbl4 acc[2][PR_TPW];
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(256) void k_project_mfma(const double *__restrict__
             const double a1 = s_u[buf][4 * kk + lk][16 + li];
 #pragma unroll
             for (int j = 0; j < PR_TPW; j++) {
-                const int nt = w + 4 * j;
+                const int nt = nt0 + w + 4 * j;
                 if (nt < ntiles) {
                     const double bv = trow[(size_t)(4 * kk) * ntp + 16 * nt];
                     acc[0][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[0][j], 0, 0, 0);
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(256) void k_project_mfma(const double *__restrict__
     for (int m = 0; m < 2; m++)
 #pragma unroll
         for (int j = 0; j < PR_TPW; j++) {
-            const int nt = w + 4 * j;
+            const int nt = nt0 + w + 4 * j;
             if (nt >= ntiles) continue;
             const int q = 16 * nt + li;
 #pragma unroll
@@ -120,7 +121,7 @@ int rg_project_parts(int ny)
     return parts < 1 ? 1 : (parts > 8 ? 8 : parts);
 }
 
-bool rg_project_supported(int ntab) { return (ntab + 15) / 16 <= PR_MAXNT; }
+bool rg_project_supported(int ntab) { return (ntab + 15) / 16 <= 2 * PR_MAXNT; }
 
 void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double *d_upot, int upot_nreal)
 {
@@ -137,7 +138,8 @@ void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double 
         sb = P->ny;
         si = 1;
     }
-    hipLaunchKernelGGL(k_project_mfma, dim3((nb + PR_RB - 1) / PR_RB, P->cq_parts), dim3(256), 0, P->stream, us, sb, si,
+    const int groups = (P->ntp / 16 + PR_MAXNT - 1) / PR_MAXNT;  // column tiles in groups of PR_MAXNT
+    hipLaunchKernelGGL(k_project_mfma, dim3((nb + PR_RB - 1) / PR_RB, P->cq_parts, groups), dim3(256), 0, P->stream, us, sb, si,
                        P->ny, P->d_tab, P->ntp, P->ntab, P->nchunk, P->cq_parts, P->d_cq, nb);
     rg_timer_end(P, RGK_PROJECT, stop);
 }

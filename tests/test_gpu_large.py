@@ -1,0 +1,70 @@
+"""Mode counts above 128 (up to 256): the slice inversion by a 2 x 2 Schur complement on the
+kernels of order <= 128, the Dyson products at np = 144..256, the projection in two groups of
+column tiles, the Jacobi singular values.  Against the FP64 oracle on identical potentials:
+complex128 eigenvalues to 1e-10 relative (floor 1e-13); complex64 within an absolute bound.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from recurgreen_amd import binding
+
+from test_gpu_parity import assert_T_close, say, upot_batch
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # N = 256 in reference mode (ny = N): 513 projection coefficients, blocks 128 + 128
+    "N256": dict(boundtype="dirichlet", nxdiso=8, wol=128.25, aratio=0.4 / 128.25, dscat=0.3),
+    # N = 200 (np = 208: blocks 128 + 80), absorption, ny = 400
+    "N200_absorb": dict(boundtype="dirichlet", nxdiso=10, wol=100.25, aratio=0.5 / 100.25, dscat=0.5, dabso=0.3,
+                        ny=400, dos_model=1),
+    # periodic, N = 161 (np = 176)
+    "periodic_N161": dict(boundtype="periodic", nxdiso=6, wol=80.3, aratio=0.3 / 80.3, dscat=0.2, ny=200),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_large_mode_counts_fp64_parity(name):
+    p = CASES[name]
+    say(f"large N fp64 {name}")
+    ow = O.Waveguide(**p)
+    plan = binding.Plan(p, batch_max=2)
+    assert plan.nmu > 128
+    U = upot_batch(ow, [1, 2])
+    Tg, st, code = plan.tdistrib_potential(U, "fp64")
+    assert code == 0 and not st.any()
+    Tg = Tg.reshape(2, -1)
+    for k in range(2):
+        assert_T_close(Tg[k], ow.tdistrib_potential(U[:, :, k].T))
+    plan.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_large_mode_counts_mixed(name):
+    p = CASES[name]
+    say(f"large N mixed {name}")
+    ow = O.Waveguide(**p)
+    plan = binding.Plan(p, batch_max=2)
+    U = upot_batch(ow, [3, 4])
+    Tg, st, code = plan.tdistrib_potential(U, "mixed_fp32")
+    assert code == 0 and not st.any()
+    To = np.stack([ow.tdistrib_potential(U[:, :, k].T) for k in range(2)])
+    err = np.max(np.abs(Tg.reshape(2, -1) - To))
+    print(f"[gpu-case] {name} mixed: max |dT| = {err:.3e}", flush=True)
+    assert err <= 5e-4
+    plan.close()
+
+
+def test_seeded_n256_batch_matches_explicit():
+    """Seeded sweep of a batch at N = 256 equals the explicit-potential sweep of the generated
+    disorder (generator, batching and the large-order path together)."""
+    p = CASES["N256"]
+    say("seeded N256")
+    plan = binding.Plan(p, batch_max=3)
+    T1, st1, c1 = plan.tdistrib(5, 3, "fp64")
+    U = plan.generate_potential(5, 3)
+    T2, st2, c2 = plan.tdistrib_potential(U, "fp64")
+    assert c1 == 0 and c2 == 0
+    assert np.array_equal(T1, T2)
+    plan.close()

@@ -1,5 +1,5 @@
 """Uneven batches (rg_tdistrib_uneven): realizations with their own length, absorption, disorder
-strength, seed and width in one call.  Each realization must equal, bit for bit, the uniform
+strength, seed and width (up to N = 256) in one call.  Each realization must equal, bit for bit, the uniform
 one-realization sweep of the same parameters and seed (same kernels, same data, constants taken
 per realization instead of shared); rejected parameters and unsupported widths are reported per
 realization.  One absorbing realization is also checked against the FP64 oracle (1e-10 relative).
@@ -30,9 +30,9 @@ def uneven_set():
         guide(10.25, 55, 0.0, ny=205), guide(10.25, 2, 0.0, ny=205),   # nx = 2: last diagonal = bound + bulk
         guide(4.3, 10, 0.2, exfac=1.2, nxfree=5, dos_model=0), guide(10.25, 80, 1.0, ny=205),
         guide(10.0, 20),                                            # wol a multiple of 0.5: rejected
-        guide(100.25, 10, ny=201),                                  # N = 200 > 128: not supported
+        guide(150.25, 10, ny=301),                                  # N = 300 > 256: not supported
         guide(18.7, 40, 0.0, ny=80), guide(4.3, 25, 0.0, exfac=1.2, nxfree=5, dos_model=0),
-        guide(10.25, 12, 0.1, ny=205),
+        guide(10.25, 12, 0.1, ny=205), guide(100.25, 6, 0.2, ny=201), guide(100.25, 9, 0.0, ny=201),  # N = 200
     ]
     seeds = np.arange(100, 100 + len(ps), dtype=np.int64)
     return ps, seeds
