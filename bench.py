@@ -25,10 +25,24 @@ PEAK_TFLOPS = {"fp64": 78.6, "mixed_fp32": 157.3, "mixed_bf16x3": 2500.0}
 DTYPE = {"fp64": "f64", "mixed_fp32": "f32+f64", "mixed_bf16x3": "bf16x3+f64"}
 
 
+# Waveguides of the benchmark: width W, slices nx (L = nx dx), l_s, batch.  n128 is the headline
+# workload; c1..c4 are the other configurations of the task (c1 CPU-sized, c2 FP64 N=100, c3 mixed
+# N=256, c4 the adjoint's N=64).
+CONFIGS = {
+    "n128": (64.25, 1000, 10.0, 4096),
+    "c1": (10.25, 200, 5.0, 1000),
+    "c2": (50.25, 1000, 10.0, 8192),
+    "c3": (128.25, 2000, 20.0, 4096),
+    "c4": (32.25, 400, 10.0, 256),
+}
+
+
 def workload_params(args):
-    """W=64.25, dx=dy=0.05, l_s=10; L=50 (nx=1000) unless --nx shortens the guide at fixed dx."""
-    W, dx, ls = 64.25, 0.05, 10.0
-    nxdiso = 1000 if args.nx is None else args.nx
+    """Dirichlet guide of the chosen configuration at dx=dy=0.05 (L = nx dx, dscat = L / l_s);
+    --nx shortens it at fixed dx."""
+    W, nx0, ls, _ = CONFIGS[getattr(args, "config", "n128")]
+    dx = 0.05
+    nxdiso = nx0 if args.nx is None else args.nx
     L = nxdiso * dx
     return dict(boundtype="dirichlet", nxdiso=nxdiso, nxfree=0, wol=W, exfac=1.0, aratio=L / W,
                 dscat=L / ls, dabso=0.0, napera=1.0, naperb=1.0, ny=int(round(W / dx)), dos_model=1)
@@ -198,7 +212,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--config", default="n128", choices=list(CONFIGS),
+                    help="waveguide configuration (default: the N=128 headline workload)")
+    ap.add_argument("--batch", type=int, default=None, help="realizations per step (default: the configuration's)")
     ap.add_argument("--nx", type=int, default=None, help="override slice count (profiling runs)")
     ap.add_argument("--precision", default="mixed_fp32", choices=list(PEAK_TFLOPS),
                     help="sweep reported as `value`; the FP64 sweep is always measured beside it")
@@ -215,6 +231,8 @@ def main():
         print(f"bench.py measures one GPU only (realizations are independent and would shard by seed "
               f"range across GPUs); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.batch is None:
+        args.batch = CONFIGS[args.config][3]
     if args.uneven:
         uneven_bench(args)
         return
@@ -283,8 +301,8 @@ def main():
         "value": r["value"], "unit": "realizations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": DTYPE[main_prec], "data": "synthetic",
-        "config": {"workload": f"dirichlet W=64.25 N={n} L={nx * 0.05:g} nx={nx} ny={plan.ny} ls=10 eps=0, "
-                               f"batch {B} realizations, precision {main_prec}",
+        "config": {"workload": f"{args.config}: dirichlet W={params['wol']} N={n} L={nx * 0.05:g} nx={nx} ny={plan.ny} "
+                               f"ls={CONFIGS[args.config][2]:g} eps=0, batch {B} realizations, precision {main_prec}",
                    "global_batch": B, "parallelism": "realizations"},
         "slice_updates_per_s": r["value"] * (nx - 1),
         "algorithmic_tflops": r["value"] * flops_real / 1e12,

@@ -5,7 +5,8 @@
 // Here the T are computed as the eigenvalues of the smaller Gram matrix A = t^H t (or t t^H),
 // in FP64 whatever the precision of G, one workgroup per realization:
 //   1. A = Y^H Y in registers (4x4 tiles per thread), Y the scaled aperture block of G, then to LDS
-//      as a packed lower triangle (column-major packed, 132 KiB at n = 128);
+//      as a packed lower triangle (column-major packed, 132 KiB at n = 128); for 128 < n <= 256 the
+//      packed triangle lives in a global work area instead (one per realization, through L2);
 //   2. Householder reduction to a real symmetric tridiagonal (d, e) in LDS (the zhetd2 sequence:
 //      v from zlarfg, p = tau A v, w = p - tau/2 (p^H v) v, A -= v w^H + w v^H);
 //   3. the eigenvalues of (d, e) by Sturm-count multisection, two threads per eigenvalue, each
@@ -17,7 +18,8 @@
 namespace {
 
 constexpr int EG_NT = 512;  // threads per workgroup
-constexpr int EG_MAXN = 128;
+constexpr int EG_MAXN = 128;   // largest order with the packed triangle in LDS
+constexpr int EG_MAXN2 = 256;  // largest order (packed triangle in global memory)
 
 __device__ inline double2 ldg2(const double2 *p) { return *p; }
 __device__ inline double2 ldg2(const float2 *p)
@@ -57,21 +59,25 @@ __device__ inline int sturm_count(const double *d, const double *e2, int n, doub
 // eigenvector (unit norm, n complex) to xvec[b * EG_MAXN ..]: inverse iteration on the tridiagonal
 // (partial-pivoting LU, as LAPACK's dlagtf/dlagts) and back-transformation by the Householder
 // reflectors, which are kept below the subdiagonal of the packed triangle.
-template <class E, bool VEC>
+// MAXN = EG_MAXN keeps the packed triangle in LDS (132 KiB); MAXN = EG_MAXN2 keeps it in the global
+// work area `work` (one packed triangle of MAXN per realization, streamed through L2).
+template <class E, bool VEC, int MAXN>
 __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1, int NP, int imin, int imax, int jmin,
                                                       int jmax, const double2 *__restrict__ sqrtmu,
                                                       double *__restrict__ teig, int neigv,
-                                                      int32_t *__restrict__ status, double2 *__restrict__ xvec, int cs)
+                                                      int32_t *__restrict__ status, double2 *__restrict__ xvec, int cs,
+                                                      double2 *__restrict__ work)
 {
     sqrtmu += (size_t)blockIdx.x * cs;  // cs = np: constants of each realization, else shared
-    __shared__ double2 sA[EG_MAXN * (EG_MAXN + 1) / 2];
-    __shared__ double2 s_taus[VEC ? EG_MAXN : 1];
-    __shared__ double2 s_x[VEC ? EG_MAXN : 1];
+    __shared__ double2 sA_lds[MAXN == EG_MAXN ? EG_MAXN * (EG_MAXN + 1) / 2 : 1];
+    double2 *sA = MAXN == EG_MAXN ? sA_lds : work + (size_t)blockIdx.x * NP * NP;  // n (n + 1) / 2 <= NP^2
+    __shared__ double2 s_taus[VEC ? MAXN : 1];
+    __shared__ double2 s_x[VEC ? MAXN : 1];
     __shared__ double s_lam;
-    __shared__ double2 s_v[EG_MAXN], s_p[EG_MAXN];
-    __shared__ double s_d[EG_MAXN], s_e[EG_MAXN], s_e2[EG_MAXN];
-    __shared__ double s_w[EG_MAXN];     // |sqrt(mu_l)|^2 weights
-    __shared__ double2 s_sa[EG_MAXN];  // column scales of A
+    __shared__ double2 s_v[MAXN], s_p[MAXN];
+    __shared__ double s_d[MAXN], s_e[MAXN], s_e2[MAXN];
+    __shared__ double s_w[MAXN];     // |sqrt(mu_l)|^2 weights
+    __shared__ double2 s_sa[MAXN];  // column scales of A
     __shared__ double2 s_tau;
     __shared__ int s_bad;
 
@@ -179,9 +185,10 @@ __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1,
             __syncthreads();
             continue;
         }
-        // p = tau A22 v; four threads per row, each a quarter of the columns
-        {
-            const int r = tid >> 2, h = tid & 3;
+        // p = tau A22 v; four threads per row, each a quarter of the columns (rows in passes of
+        // EG_NT / 4, the same count for every lane)
+        for (int r0 = 0; r0 < Lk; r0 += EG_NT / 4) {
+            const int r = r0 + (tid >> 2), h = tid & 3;
             double2 acc = cmk(0.0, 0.0);
             if (r < Lk) {
                 const int c0 = (h * Lk) >> 2, c1 = ((h + 1) * Lk) >> 2;
@@ -209,8 +216,8 @@ __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1,
         }
         const double2 alpha2 = cmul(cmk(-0.5 * tau.x, -0.5 * tau.y), dot);
         // A22 -= v w^H + w v^H on the lower triangle: column pairs (c, Lk-1-c), eight threads a pair
-        {
-            const int pr = tid >> 3, sub = tid & 7, npair = (Lk + 1) >> 1;
+        for (int pr0 = 0, npair = (Lk + 1) >> 1; pr0 < npair; pr0 += EG_NT / 8) {
+            const int pr = pr0 + (tid >> 3), sub = tid & 7;
             if (pr < npair) {
                 const int c1 = pr, c2 = Lk - 1 - pr;
                 const int len1 = Lk - c1, len = c1 == c2 ? len1 : Lk + 1;
@@ -289,8 +296,8 @@ __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1,
         if (tid == 0) {
             // (T - sigma I) z = r by Gaussian elimination with row interchanges; rows kept as
             // (u0, u1, u2) = entries at columns i, i+1, i+2.  Three solves from r = 1.
-            double *u0 = reinterpret_cast<double *>(s_p), *u1 = u0 + EG_MAXN, *u2 = s_e2;
-            double *zz = reinterpret_cast<double *>(s_v), *rh = zz + EG_MAXN;
+            double *u0 = reinterpret_cast<double *>(s_p), *u1 = u0 + MAXN, *u2 = s_e2;
+            double *zz = reinterpret_cast<double *>(s_v), *rh = zz + MAXN;
             const double sigma = s_lam + 1e-14 * fmax(1.0, fabs(s_lam));
             for (int i = 0; i < n; i++) zz[i] = 1.0;
             for (int it = 0; it < 3; it++) {
@@ -369,24 +376,32 @@ __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1,
 bool rg_eig_supported(const rg_plan *P)
 {
     const int m = P->imax - P->imin + 1, nn = P->jmax - P->jmin + 1;
-    return (m < nn ? m : nn) <= EG_MAXN;
+    return (m < nn ? m : nn) <= EG_MAXN2;
 }
 
+// Orders up to EG_MAXN in LDS; up to EG_MAXN2 with the packed triangle in the t buffer (free while
+// eigenvalues are computed: the sweep's state is in Gss/Gs1).
 template <class E>
 static void launch_eig(rg_plan *P, int nb, const E *gs1, double *teig, int32_t *status)
 {
-    hipLaunchKernelGGL((k_tdt_eig<E, false>), dim3(nb), dim3(EG_NT), 0, P->stream, gs1, P->np, P->imin, P->imax,
-                       P->jmin, P->jmax, P->d_sqrtmu, teig, P->neigv, status, nullptr, P->cstride);
+    const int m = P->imax - P->imin + 1, nn = P->jmax - P->jmin + 1;
+    if ((m < nn ? m : nn) <= EG_MAXN)
+        hipLaunchKernelGGL((k_tdt_eig<E, false, EG_MAXN>), dim3(nb), dim3(EG_NT), 0, P->stream, gs1, P->np, P->imin,
+                           P->imax, P->jmin, P->jmax, P->d_sqrtmu, teig, P->neigv, status, nullptr, P->cstride, nullptr);
+    else
+        hipLaunchKernelGGL((k_tdt_eig<E, false, EG_MAXN2>), dim3(nb), dim3(EG_NT), 0, P->stream, gs1, P->np, P->imin,
+                           P->imax, P->jmin, P->jmax, P->d_sqrtmu, teig, P->neigv, status, nullptr, P->cstride,
+                           P->d_tmat);
 }
 
+// Order <= EG_MAXN only (the adjoint's entry points refuse np > 128).
 void rg_launch_tdt_topvec_f64(rg_plan *P, int nb, const double2 *gs1, int32_t *status, double2 *xvec)
 {
-    hipLaunchKernelGGL((k_tdt_eig<double2, true>), dim3(nb), dim3(EG_NT), 0, P->stream, gs1, P->np, P->imin, P->imax,
-                       P->jmin, P->jmax, P->d_sqrtmu, nullptr, P->neigv, status, xvec, P->cstride);
+    hipLaunchKernelGGL((k_tdt_eig<double2, true, EG_MAXN>), dim3(nb), dim3(EG_NT), 0, P->stream, gs1, P->np, P->imin,
+                       P->imax, P->jmin, P->jmax, P->d_sqrtmu, nullptr, P->neigv, status, xvec, P->cstride, nullptr);
 }
 
-// The Gram matrix of order min(m, nn) must fit the kernel's LDS (EG_MAXN); larger orders take the
-// one-sided Jacobi SVD instead, so no launch indexes past the LDS arrays.
+// Orders beyond EG_MAXN2 take the one-sided Jacobi SVD, so no launch indexes past the arrays.
 void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status)
 {
     if (!rg_eig_supported(P)) return rg_launch_fisher_lee_svd_f64(P, nb, gs1, teig, status, nullptr);
