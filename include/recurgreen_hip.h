@@ -161,7 +161,9 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *   "graph"         1: seeded and explicit-potential sweeps run as captured HIP graphs (captured
  *                   on first use per batch size, precision, output pointer and options; replayed
  *                   with the seed read from device memory), when profiling is off and one lane is
- *                   active; 0 (default): direct launches.
+ *                   active; 0 (default): direct launches;
+ *   "adjoint_precision" RG_FP64 (default): the adjoint sweep in complex128; RG_MIXED_FP32: complex64
+ *                   state and products (FP32 MFMA), loss, terminal adjoint and gradient in FP64.
  * RG_EINVAL for an unknown key or value.  Waits for the plan's stream first. */
 rg_status rg_plan_set_option(rg_plan *plan, const char *key, int32_t value);
 rg_status rg_plan_get_option(const rg_plan *plan, const char *key, int32_t *value);

@@ -15,15 +15,16 @@
 namespace {
 
 // Loss L = Tr(t^H t) = sum |t_ab|^2 and its terminal adjoint Sbar_ij = -4i t_ab conj(s_i s_j).
-__global__ __launch_bounds__(256) void k_adj_terminal(const double2 *__restrict__ gs1, int NP, int n, int imin, int imax,
+template <class E>
+__global__ __launch_bounds__(256) void k_adj_terminal(const E *__restrict__ gs1, int NP, int n, int imin, int imax,
                                                       int jmin, int jmax, const double2 *__restrict__ sqrtmu,
-                                                      double *__restrict__ loss, double2 *__restrict__ sbar)
+                                                      double *__restrict__ loss, E *__restrict__ sbar)
 {
     __shared__ double s_red[256];
     const int b = blockIdx.x, tid = threadIdx.x;
     const int m = imax - imin + 1, nn = jmax - jmin + 1;
-    const double2 *G = gs1 + (size_t)b * NP * NP;
-    double2 *SB = sbar + (size_t)b * NP * NP;
+    const E *G = gs1 + (size_t)b * NP * NP;
+    E *SB = sbar + (size_t)b * NP * NP;
     double acc = 0.0;
     for (int e = tid; e < NP * NP; e += 256) {
         const int i = e / NP, j = e % NP;
@@ -31,13 +32,13 @@ __global__ __launch_bounds__(256) void k_adj_terminal(const double2 *__restrict_
         const int a = i - (imin - 1), c = j - (jmin - 1);
         if (i < n && j < n && a >= 0 && a < m && c >= 0 && c < nn) {
             const double2 sij = cmul(sqrtmu[i], sqrtmu[j]);
-            const double2 v = cmul(sij, G[e]);
+            const double2 v = cmul(sij, rg_mt<E>::ld(G[e]));
             const double2 t = cmk(-2.0 * v.y, 2.0 * v.x);  // 2i s_i G_ij s_j
             acc += t.x * t.x + t.y * t.y;
             const double2 q = cmul(t, cconj(sij));
             out = cmk(4.0 * q.y, -4.0 * q.x);  // -4i q
         }
-        SB[e] = out;
+        SB[e] = rg_mt<E>::make(out.x, out.y);
     }
     s_red[tid] = acc;
     __syncthreads();
@@ -51,11 +52,12 @@ __global__ __launch_bounds__(256) void k_adj_terminal(const double2 *__restrict_
 // Loss T_max and its terminal adjoint from the top eigenvector x (unit norm) of the Gram matrix:
 // gram_cols (A = t^H t, x = v): y = t v, T_max = |y|^2, t v v^H = y v^H;
 // otherwise (A = t t^H, x = u): z = t^H u, T_max = |z|^2, u u^H t = u z^H.
-__global__ __launch_bounds__(256) void k_adj_terminal_tmax(const double2 *__restrict__ gs1, int NP, int n, int imin,
+template <class E>
+__global__ __launch_bounds__(256) void k_adj_terminal_tmax(const E *__restrict__ gs1, int NP, int n, int imin,
                                                            int imax, int jmin, int jmax,
                                                            const double2 *__restrict__ sqrtmu,
                                                            const double2 *__restrict__ xvec, int xstride,
-                                                           double *__restrict__ loss, double2 *__restrict__ sbar)
+                                                           double *__restrict__ loss, E *__restrict__ sbar)
 {
     __shared__ double2 s_x[256], s_y[256];
     __shared__ double s_red[256];
@@ -63,11 +65,11 @@ __global__ __launch_bounds__(256) void k_adj_terminal_tmax(const double2 *__rest
     const int m = imax - imin + 1, nn = jmax - jmin + 1;
     const bool gram_cols = nn <= m;
     const int nx_ = gram_cols ? nn : m, ny_ = gram_cols ? m : nn;
-    const double2 *G = gs1 + (size_t)b * NP * NP;
-    double2 *SB = sbar + (size_t)b * NP * NP;
+    const E *G = gs1 + (size_t)b * NP * NP;
+    E *SB = sbar + (size_t)b * NP * NP;
     auto tval = [&](int a, int c) {  // t_ac = 2i s_i G_ij s_j
         const int i = imin - 1 + a, j = jmin - 1 + c;
-        const double2 v = cmul(cmul(sqrtmu[i], G[(size_t)i * NP + j]), sqrtmu[j]);
+        const double2 v = cmul(cmul(sqrtmu[i], rg_mt<E>::ld(G[(size_t)i * NP + j])), sqrtmu[j]);
         return cmk(-2.0 * v.y, 2.0 * v.x);
     };
     for (int i = tid; i < nx_; i += 256) s_x[i] = xvec[(size_t)b * xstride + i];
@@ -99,30 +101,33 @@ __global__ __launch_bounds__(256) void k_adj_terminal_tmax(const double2 *__rest
             const double2 q = cmul(w, cconj(cmul(sqrtmu[i], sqrtmu[j])));
             out = cmk(4.0 * q.y, -4.0 * q.x);
         }
-        SB[e] = out;
+        SB[e] = rg_mt<E>::make(out.x, out.y);
     }
 }
 
 // gbar += sbar on the active n x n block.
-__global__ __launch_bounds__(256) void k_adj_add(double2 *__restrict__ gbar, const double2 *__restrict__ sbar, int NP,
-                                                 int n)
+template <class E>
+__global__ __launch_bounds__(256) void k_adj_add(E *__restrict__ gbar, const E *__restrict__ sbar, int NP, int n)
 {
     const size_t base = (size_t)blockIdx.x * NP * NP;
     for (int e = threadIdx.x; e < n * n; e += 256) {
         const size_t o = base + (size_t)(e / n) * NP + (e % n);
-        gbar[o] = cadd(gbar[o], sbar[o]);
+        const double2 r = cadd(rg_mt<E>::ld(gbar[o]), rg_mt<E>::ld(sbar[o]));
+        gbar[o] = rg_mt<E>::make(r.x, r.y);
     }
 }
 
 // gbar = -conj(b)_m Cbar_mk conj(b)_k on the active block.
-__global__ __launch_bounds__(256) void k_adj_scale(double2 *__restrict__ gbar, const double2 *__restrict__ cbar,
+template <class E>
+__global__ __launch_bounds__(256) void k_adj_scale(E *__restrict__ gbar, const E *__restrict__ cbar,
                                                    const double2 *__restrict__ bvec, int NP, int n)
 {
     const size_t base = (size_t)blockIdx.x * NP * NP;
     for (int e = threadIdx.x; e < n * n; e += 256) {
         const int m = e / n, k = e % n;
         const size_t o = base + (size_t)m * NP + k;
-        gbar[o] = cneg(cmul(cmul(cconj(bvec[m]), cbar[o]), cconj(bvec[k])));
+        const double2 r = cneg(cmul(cmul(cconj(bvec[m]), rg_mt<E>::ld(cbar[o])), cconj(bvec[k])));
+        gbar[o] = rg_mt<E>::make(r.x, r.y);
     }
 }
 
@@ -132,31 +137,32 @@ __global__ __launch_bounds__(256) void k_adj_scale(double2 *__restrict__ gbar, c
 //   periodic (K_mn = (1/ny) exp(2 pi i (n-m) y), table cos/sin pairs):
 //       e_2d = sum_{|n-m|=d} Re Cbar_mn,  e_2d+1 = sum_{n-m=d} Im Cbar_mn - sum_{m-n=d} Im Cbar_mn.
 // grad has the potential layout (nxdiso, ny, r): grad[(b * ny + iy) * nxdiso + ixd].
-__global__ __launch_bounds__(256) void k_adj_grad(const double2 *__restrict__ cbar, int NP, int n, int boundtype,
+template <class E>
+__global__ __launch_bounds__(256) void k_adj_grad(const E *__restrict__ cbar, int NP, int n, int boundtype,
                                                   const double *__restrict__ tab, int ntp, int ntab, int ny, int nxdiso,
                                                   int ixd, double *__restrict__ grad)
 {
     __shared__ double s_e[2 * 256 + 8];
     const int b = blockIdx.x, tid = threadIdx.x;
-    const double2 *Cb = cbar + (size_t)b * NP * NP;
+    const E *Cb = cbar + (size_t)b * NP * NP;
     for (int q = tid; q < ntab; q += 256) {
         double e = 0.0;
         if (boundtype == RG_DIRICHLET) {
             for (int m = 0; m < n; m++) {
-                if (m + q < n) e += Cb[(size_t)m * NP + m + q].x;           // n - m = q
-                if (q > 0 && m - q >= 0) e += Cb[(size_t)m * NP + m - q].x;  // m - n = q
+                if (m + q < n) e += rg_mt<E>::ld(Cb[(size_t)m * NP + m + q]).x;           // n - m = q
+                if (q > 0 && m - q >= 0) e += rg_mt<E>::ld(Cb[(size_t)m * NP + m - q]).x;  // m - n = q
                 const int k = q - 2 - m;
-                if (k >= 0 && k < n) e -= Cb[(size_t)m * NP + k].x;
+                if (k >= 0 && k < n) e -= rg_mt<E>::ld(Cb[(size_t)m * NP + k]).x;
             }
         } else {
             const int d = q >> 1;
             for (int m = 0; m < n; m++) {
                 if (q & 1) {
-                    if (m + d < n) e += Cb[(size_t)m * NP + m + d].y;
-                    if (m - d >= 0) e -= Cb[(size_t)m * NP + m - d].y;
+                    if (m + d < n) e += rg_mt<E>::ld(Cb[(size_t)m * NP + m + d]).y;
+                    if (m - d >= 0) e -= rg_mt<E>::ld(Cb[(size_t)m * NP + m - d]).y;
                 } else {
-                    if (m + d < n) e += Cb[(size_t)m * NP + m + d].x;
-                    if (d > 0 && m - d >= 0) e += Cb[(size_t)m * NP + m - d].x;
+                    if (m + d < n) e += rg_mt<E>::ld(Cb[(size_t)m * NP + m + d]).x;
+                    if (d > 0 && m - d >= 0) e += rg_mt<E>::ld(Cb[(size_t)m * NP + m - d]).x;
                 }
             }
         }
@@ -172,36 +178,59 @@ __global__ __launch_bounds__(256) void k_adj_grad(const double2 *__restrict__ cb
     }
 }
 
-}  // namespace
-
-void rg_launch_adj_terminal(rg_plan *P, int nb, const double2 *gs1, double *loss, double2 *sbar)
+template <class E>
+void adj_terminal(rg_plan *P, int nb, const E *gs1, double *loss, E *sbar)
 {
-    hipLaunchKernelGGL(k_adj_terminal, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->nmu, P->imin, P->imax, P->jmin,
+    hipLaunchKernelGGL(k_adj_terminal<E>, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->nmu, P->imin, P->imax, P->jmin,
                        P->jmax, P->d_sqrtmu, loss, sbar);
 }
 
-void rg_launch_adj_terminal_tmax(rg_plan *P, int nb, const double2 *gs1, const double2 *xvec, int xstride, double *loss,
-                                 double2 *sbar)
+template <class E>
+void adj_terminal_tmax(rg_plan *P, int nb, const E *gs1, const double2 *xvec, int xstride, double *loss, E *sbar)
 {
-    hipLaunchKernelGGL(k_adj_terminal_tmax, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->nmu, P->imin, P->imax,
+    hipLaunchKernelGGL(k_adj_terminal_tmax<E>, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->nmu, P->imin, P->imax,
                        P->jmin, P->jmax, P->d_sqrtmu, xvec, xstride, loss, sbar);
 }
 
-void rg_launch_adj_add(rg_plan *P, int nb, double2 *gbar, const double2 *sbar)
+template <class E>
+void adj_add(rg_plan *P, int nb, E *gbar, const E *sbar)
 {
-    hipLaunchKernelGGL(k_adj_add, dim3(nb), dim3(256), 0, P->stream, gbar, sbar, P->np, P->nmu);
+    hipLaunchKernelGGL(k_adj_add<E>, dim3(nb), dim3(256), 0, P->stream, gbar, sbar, P->np, P->nmu);
 }
 
-void rg_launch_adj_scale(rg_plan *P, int nb, double2 *gbar, const double2 *cbar)
+template <class E>
+void adj_scale(rg_plan *P, int nb, E *gbar, const E *cbar)
 {
-    hipLaunchKernelGGL(k_adj_scale, dim3(nb), dim3(256), 0, P->stream, gbar, cbar, P->d_b, P->np, P->nmu);
+    hipLaunchKernelGGL(k_adj_scale<E>, dim3(nb), dim3(256), 0, P->stream, gbar, cbar, P->d_b, P->np, P->nmu);
 }
 
-void rg_launch_adj_grad(rg_plan *P, int nb, const double2 *cbar, int ixd, double *grad)
+template <class E>
+void adj_grad(rg_plan *P, int nb, const E *cbar, int ixd, double *grad)
 {
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_PROJECT, &stop);
-    hipLaunchKernelGGL(k_adj_grad, dim3(nb), dim3(256), 0, P->stream, cbar, P->np, P->nmu, P->p.boundtype, P->d_tab,
+    hipLaunchKernelGGL(k_adj_grad<E>, dim3(nb), dim3(256), 0, P->stream, cbar, P->np, P->nmu, P->p.boundtype, P->d_tab,
                        P->ntp, P->ntab, P->ny, P->nxdiso, ixd, grad);
     rg_timer_end(P, RGK_PROJECT, stop);
 }
+
+}  // namespace
+
+void rg_launch_adj_terminal(rg_plan *P, int nb, const double2 *gs1, double *loss, double2 *sbar) { adj_terminal(P, nb, gs1, loss, sbar); }
+void rg_launch_adj_terminal(rg_plan *P, int nb, const float2 *gs1, double *loss, float2 *sbar) { adj_terminal(P, nb, gs1, loss, sbar); }
+void rg_launch_adj_terminal_tmax(rg_plan *P, int nb, const double2 *gs1, const double2 *xvec, int xstride, double *loss,
+                                 double2 *sbar)
+{
+    adj_terminal_tmax(P, nb, gs1, xvec, xstride, loss, sbar);
+}
+void rg_launch_adj_terminal_tmax(rg_plan *P, int nb, const float2 *gs1, const double2 *xvec, int xstride, double *loss,
+                                 float2 *sbar)
+{
+    adj_terminal_tmax(P, nb, gs1, xvec, xstride, loss, sbar);
+}
+void rg_launch_adj_add(rg_plan *P, int nb, double2 *gbar, const double2 *sbar) { adj_add(P, nb, gbar, sbar); }
+void rg_launch_adj_add(rg_plan *P, int nb, float2 *gbar, const float2 *sbar) { adj_add(P, nb, gbar, sbar); }
+void rg_launch_adj_scale(rg_plan *P, int nb, double2 *gbar, const double2 *cbar) { adj_scale(P, nb, gbar, cbar); }
+void rg_launch_adj_scale(rg_plan *P, int nb, float2 *gbar, const float2 *cbar) { adj_scale(P, nb, gbar, cbar); }
+void rg_launch_adj_grad(rg_plan *P, int nb, const double2 *cbar, int ixd, double *grad) { adj_grad(P, nb, cbar, ixd, grad); }
+void rg_launch_adj_grad(rg_plan *P, int nb, const float2 *cbar, int ixd, double *grad) { adj_grad(P, nb, cbar, ixd, grad); }

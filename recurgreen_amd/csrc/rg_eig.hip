@@ -401,6 +401,12 @@ void rg_launch_tdt_topvec_f64(rg_plan *P, int nb, const double2 *gs1, int32_t *s
                        P->imax, P->jmin, P->jmax, P->d_sqrtmu, nullptr, P->neigv, status, xvec, P->cstride, nullptr);
 }
 
+void rg_launch_tdt_topvec_f32(rg_plan *P, int nb, const float2 *gs1, int32_t *status, double2 *xvec)
+{
+    hipLaunchKernelGGL((k_tdt_eig<float2, true, EG_MAXN>), dim3(nb), dim3(EG_NT), 0, P->stream, gs1, P->np, P->imin,
+                       P->imax, P->jmin, P->jmax, P->d_sqrtmu, nullptr, P->neigv, status, xvec, P->cstride, nullptr);
+}
+
 // Orders beyond EG_MAXN2 take the one-sided Jacobi SVD, so no launch indexes past the arrays.
 void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status)
 {

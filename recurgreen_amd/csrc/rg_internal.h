@@ -53,6 +53,27 @@ __host__ __device__ inline double2 crecip(double2 a)
 }
 __host__ __device__ inline double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
 
+// Element traits of the complex128 / complex64 batched products: the real type, the MFMA
+// accumulator, the 16x16x4 MFMA (both take A[i][k] from lane i + 16k and B[k][j] from lane j + 16k)
+// and where result register v of lane group g lands (f64: row 4v + g; f32: row 4g + v).
+template <class E> struct rg_mt;
+template <> struct rg_mt<double2> {
+    typedef double R;
+    typedef double V4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V4 mfma(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int v, int g) { return 4 * v + g; }
+    static __device__ __forceinline__ double2 make(double x, double y) { return make_double2(x, y); }
+    static __device__ __forceinline__ double2 ld(double2 v) { return v; }
+};
+template <> struct rg_mt<float2> {
+    typedef float R;
+    typedef float V4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V4 mfma(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int v, int g) { return 4 * g + v; }
+    static __device__ __forceinline__ float2 make(double x, double y) { return make_float2((float)x, (float)y); }
+    static __device__ __forceinline__ double2 ld(float2 v) { return make_double2(v.x, v.y); }
+};
+
 // ------------------------------------------------------------------------------------------------
 // Plan.
 // ------------------------------------------------------------------------------------------------
@@ -120,6 +141,7 @@ struct rg_plan {
     // adjoint sweep workspace (rg_adjoint_reserve): checkpoints every adj_seg slices, one segment
     // of recomputed states, five adjoint matrices, the gradient staging, for adj_cap realizations
     int adj_cap = 0, adj_seg = 0, adj_nck = 0;
+    int adj_prec = 0;  // RG_FP64: complex128 adjoint; RG_MIXED_FP32: complex64 state and products
     double2 *d_ck = nullptr, *d_seg = nullptr, *d_adj = nullptr;
     double *d_loss = nullptr, *d_grad = nullptr;
     double2 *d_bconj = nullptr;  // conj(b) (np)
@@ -191,7 +213,16 @@ void rg_launch_fisher_lee_svd_f32(rg_plan *P, int nb, const float2 *gs1, double 
 void rg_zgemm(rg_plan *P, int nb, double2 alpha, const double2 *A, int opA, const double2 *B, int opB, double2 beta,
               double2 *C, const double2 *dl, const double2 *dk, const double2 *dr);
 // rg_adjoint.hip
+void rg_cgemm32(rg_plan *P, int nb, double2 alpha, const float2 *A, int opA, const float2 *B, int opB, double2 beta,
+                float2 *C, const double2 *dl, const double2 *dk, const double2 *dr);
+// rg_adjoint.hip (complex128 and complex64 adjoint arrays)
 void rg_launch_adj_terminal(rg_plan *P, int nb, const double2 *gs1, double *loss, double2 *sbar);
+void rg_launch_adj_terminal(rg_plan *P, int nb, const float2 *gs1, double *loss, float2 *sbar);
+void rg_launch_adj_terminal_tmax(rg_plan *P, int nb, const float2 *gs1, const double2 *xvec, int xstride, double *loss,
+                                 float2 *sbar);
+void rg_launch_adj_add(rg_plan *P, int nb, float2 *gbar, const float2 *sbar);
+void rg_launch_adj_scale(rg_plan *P, int nb, float2 *gbar, const float2 *cbar);
+void rg_launch_adj_grad(rg_plan *P, int nb, const float2 *cbar, int ixd, double *grad);
 void rg_launch_adj_terminal_tmax(rg_plan *P, int nb, const double2 *gs1, const double2 *xvec, int xstride, double *loss,
                                  double2 *sbar);
 void rg_launch_adj_add(rg_plan *P, int nb, double2 *gbar, const double2 *sbar);
@@ -205,6 +236,7 @@ void rg_launch_tdt_eig_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, 
 bool rg_eig_supported(const rg_plan *P);
 // top eigenvector of t^H t (or t t^H when t has fewer rows), 128 complex slots per realization
 void rg_launch_tdt_topvec_f64(rg_plan *P, int nb, const double2 *gs1, int32_t *status, double2 *xvec);
+void rg_launch_tdt_topvec_f32(rg_plan *P, int nb, const float2 *gs1, int32_t *status, double2 *xvec);
 // rg_gj.hip (only != nullptr: invert just the realizations with only[b] != 0)
 void rg_launch_inverse_gj(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision,
                           const int32_t *only = nullptr);

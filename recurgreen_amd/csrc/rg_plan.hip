@@ -486,73 +486,131 @@ void fwd_step_f64(rg_plan *P, int nb, int ix, const double *d_upot, const double
         rg_launch_gemm_f64(P, nb, P->d_gss, s1_in, s1_out);
 }
 
-// Adjoint of L = Tr(t^H t) over nb realizations with explicit device potentials d_upot (layout
-// (nxdiso, ny, nb)): the loss to d_loss[nb], dL/du to d_grad (same layout as d_upot).  Forward
-// sweep with a checkpoint (Gss, Gs1) after every adj_seg slices; then, segment by segment from
-// the end, the states are recomputed from the checkpoint and the reverse recursion runs through
-// them (every product on rg_zgemm, complex128).
-void run_adjoint_f64(rg_plan *P, int nb, int loss_kind, const double *d_upot, double *d_loss, double *d_grad)
+// The same step on the complex64 state (the kernels of the mixed sweep).
+void fwd_step_f32(rg_plan *P, int nb, int ix, const double *d_upot, const float2 *s1_in, float2 *s1_out)
+{
+    const int id = ix - P->nxfree;
+    const bool dis = d_upot && id >= 1 && id <= P->nxdiso;
+    if (dis) rg_launch_project(P, nb, id - 1, 0, d_upot, nb);
+    const int slot = ix == 1 ? 0 : (ix == P->nx ? 2 : 1);
+    if (P->np >= 32) {
+        rg_launch_inverse_nopiv(P, nb, slot, ix == 1, dis, P->d_redo);
+        rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, RG_MIXED_FP32, P->d_redo);
+    } else {
+        rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, RG_MIXED_FP32);
+    }
+    if (ix == 1) rg_launch_copy_f2(P, P->d_gss32, s1_out, (size_t)nb * P->np * P->np);
+    else rg_launch_gemm_sp(P, nb, P->d_gss32, s1_in, s1_out, RG_MIXED_FP32);
+}
+
+void fwd_step(rg_plan *P, int nb, int ix, const double *d_upot, const double2 *s1_in, double2 *s1_out)
+{
+    fwd_step_f64(P, nb, ix, d_upot, s1_in, s1_out);
+}
+void fwd_step(rg_plan *P, int nb, int ix, const double *d_upot, const float2 *s1_in, float2 *s1_out)
+{
+    fwd_step_f32(P, nb, ix, d_upot, s1_in, s1_out);
+}
+double2 *state_gss(rg_plan *P, double2 *) { return P->d_gss; }
+float2 *state_gss(rg_plan *P, float2 *) { return P->d_gss32; }
+void prod(rg_plan *P, int nb, double2 alpha, const double2 *A, int opA, const double2 *B, int opB, double2 beta, double2 *C,
+          const double2 *dl, const double2 *dk, const double2 *dr)
+{
+    rg_zgemm(P, nb, alpha, A, opA, B, opB, beta, C, dl, dk, dr);
+}
+void prod(rg_plan *P, int nb, double2 alpha, const float2 *A, int opA, const float2 *B, int opB, double2 beta, float2 *C,
+          const double2 *dl, const double2 *dk, const double2 *dr)
+{
+    rg_cgemm32(P, nb, alpha, A, opA, B, opB, beta, C, dl, dk, dr);
+}
+void topvec(rg_plan *P, int nb, const double2 *g, double2 *x) { rg_launch_tdt_topvec_f64(P, nb, g, P->d_status, x); }
+void topvec(rg_plan *P, int nb, const float2 *g, double2 *x) { rg_launch_tdt_topvec_f32(P, nb, g, P->d_status, x); }
+
+// Adjoint of L = Tr(t^H t) (or T_max) over nb realizations with explicit device potentials d_upot
+// (layout (nxdiso, ny, nb)): the loss to d_loss[nb], dL/du to d_grad (same layout as d_upot).
+// Forward sweep with a checkpoint (Gss, Gs1) after every adj_seg slices; then, segment by segment
+// from the end, the states are recomputed from the checkpoint and the reverse recursion runs
+// through them.  E = double2: complex128 state and products (FP64 MFMA); E = float2: complex64
+// state (the mixed sweep's kernels) and products on FP32 MFMA, with the loss, the terminal adjoint
+// and the gradient accumulated in FP64.
+template <class E>
+void run_adjoint(rg_plan *P, int nb, int loss_kind, const double *d_upot, double *d_loss, double *d_grad)
 {
     const size_t mat = (size_t)nb * P->np * P->np, cmat = (size_t)P->adj_cap * P->np * P->np;
     const int nx = P->nx, S = P->adj_seg;
     const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
-    auto ck_g = [&](int c) { return P->d_ck + (size_t)(2 * c) * cmat; };
-    auto ck_s = [&](int c) { return P->d_ck + (size_t)(2 * c + 1) * cmat; };
-    auto seg_g = [&](int j) { return P->d_seg + (size_t)(2 * j) * cmat; };
-    auto seg_s = [&](int j) { return P->d_seg + (size_t)(2 * j + 1) * cmat; };
-    double2 *gbar = P->d_adj, *sbar = P->d_adj + cmat, *sbar2 = P->d_adj + 2 * cmat, *T1 = P->d_adj + 3 * cmat,
-            *cbar = P->d_adj + 4 * cmat;
+    E *ck = reinterpret_cast<E *>(P->d_ck), *sg = reinterpret_cast<E *>(P->d_seg), *ad = reinterpret_cast<E *>(P->d_adj);
+    auto ck_g = [&](int c) { return ck + (size_t)(2 * c) * cmat; };
+    auto ck_s = [&](int c) { return ck + (size_t)(2 * c + 1) * cmat; };
+    auto seg_g = [&](int j) { return sg + (size_t)(2 * j) * cmat; };
+    auto seg_s = [&](int j) { return sg + (size_t)(2 * j + 1) * cmat; };
+    E *gbar = ad, *sbar = ad + cmat, *sbar2 = ad + 2 * cmat, *T1 = ad + 3 * cmat, *cbar = ad + 4 * cmat;
+    E *gss = state_gss(P, (E *)nullptr);
+    E *sa, *sb;
+    if constexpr (std::is_same<E, double2>::value) {
+        sa = P->d_gs1[0];
+        sb = P->d_gs1[1];
+    } else {
+        sa = P->d_gs1_32[0];
+        sb = P->d_gs1_32[1];
+    }
     // forward sweep with checkpoints after slices S, 2S, ...
-    double2 *sa = P->d_gs1[0], *sb = P->d_gs1[1];
     for (int ix = 1; ix <= nx; ix++) {
-        fwd_step_f64(P, nb, ix, d_upot, sa, sb);
+        fwd_step(P, nb, ix, d_upot, sa, sb);
         std::swap(sa, sb);
         if (ix % S == 0 && ix < nx) {
             const int c = ix / S - 1;
-            (void)hipMemcpyAsync(ck_g(c), P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
-            (void)hipMemcpyAsync(ck_s(c), sa, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+            (void)hipMemcpyAsync(ck_g(c), gss, sizeof(E) * mat, hipMemcpyDeviceToDevice, P->stream);
+            (void)hipMemcpyAsync(ck_s(c), sa, sizeof(E) * mat, hipMemcpyDeviceToDevice, P->stream);
         }
     }
     if (loss_kind == 0) {
         rg_launch_adj_terminal(P, nb, sa, d_loss, sbar);
-    } else {  // T_max: top eigenvector of the Gram matrix into T1 (128 slots per realization)
-        rg_launch_tdt_topvec_f64(P, nb, sa, P->d_status, T1);
-        rg_launch_adj_terminal_tmax(P, nb, sa, T1, 128, d_loss, sbar);
+    } else {  // T_max: top eigenvector of the Gram matrix (128 complex128 slots per realization in T1)
+        double2 *xv = reinterpret_cast<double2 *>(T1);
+        topvec(P, nb, sa, xv);
+        rg_launch_adj_terminal_tmax(P, nb, sa, xv, 128, d_loss, sbar);
     }
-    (void)hipMemsetAsync(gbar, 0, sizeof(double2) * mat, P->stream);
+    (void)hipMemsetAsync(gbar, 0, sizeof(E) * mat, P->stream);
     (void)hipMemsetAsync(d_grad, 0, sizeof(double) * (size_t)nb * P->nxdiso * P->ny, P->stream);
     const int nseg = (nx + S - 1) / S;
     for (int c = nseg - 1; c >= 0; c--) {
         const int x0 = c * S + 1, x1 = std::min((c + 1) * S, nx);
-        const double2 *s_in = nullptr;
+        const E *s_in = nullptr;
         if (c >= 1) {
-            (void)hipMemcpyAsync(P->d_gss, ck_g(c - 1), sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+            (void)hipMemcpyAsync(gss, ck_g(c - 1), sizeof(E) * mat, hipMemcpyDeviceToDevice, P->stream);
             s_in = ck_s(c - 1);
         }
         for (int x = x0; x <= x1; x++) {  // recompute the segment's states
             const int j = x - x0;
-            fwd_step_f64(P, nb, x, d_upot, s_in, seg_s(j));
-            (void)hipMemcpyAsync(seg_g(j), P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+            fwd_step(P, nb, x, d_upot, s_in, seg_s(j));
+            (void)hipMemcpyAsync(seg_g(j), gss, sizeof(E) * mat, hipMemcpyDeviceToDevice, P->stream);
             s_in = seg_s(j);
         }
         for (int x = x1; x >= x0; x--) {  // reverse recursion
             const int j = x - x0;
-            const double2 *gx = seg_g(j);
+            const E *gx = seg_g(j);
             if (x >= 2) {
-                const double2 *sprev = j > 0 ? seg_s(j - 1) : ck_s(c - 1);
-                rg_zgemm(P, nb, mone, sbar, 0, sprev, 1, one, gbar, nullptr, nullptr, P->d_bconj);
-                rg_zgemm(P, nb, mone, gx, 1, sbar, 0, zero, sbar2, P->d_bconj, nullptr, nullptr);
+                const E *sprev = j > 0 ? seg_s(j - 1) : ck_s(c - 1);
+                prod(P, nb, mone, sbar, 0, sprev, 1, one, gbar, nullptr, nullptr, P->d_bconj);
+                prod(P, nb, mone, gx, 1, sbar, 0, zero, sbar2, P->d_bconj, nullptr, nullptr);
                 std::swap(sbar, sbar2);
             } else {
                 rg_launch_adj_add(P, nb, gbar, sbar);
             }
-            rg_zgemm(P, nb, one, gbar, 0, gx, 1, zero, T1, nullptr, nullptr, nullptr);
-            rg_zgemm(P, nb, mone, gx, 1, T1, 0, zero, cbar, nullptr, nullptr, nullptr);
+            prod(P, nb, one, gbar, 0, gx, 1, zero, T1, nullptr, nullptr, nullptr);
+            prod(P, nb, mone, gx, 1, T1, 0, zero, cbar, nullptr, nullptr, nullptr);
             const int id = x - P->nxfree;
             if (id >= 1 && id <= P->nxdiso) rg_launch_adj_grad(P, nb, cbar, id - 1, d_grad);
             if (x >= 2) rg_launch_adj_scale(P, nb, gbar, cbar);
         }
     }
+}
+
+void run_adjoint_any(rg_plan *P, int nb, int loss_kind, const double *d_upot, double *d_loss, double *d_grad)
+{
+    if (P->adj_prec == RG_MIXED_FP32) run_adjoint<float2>(P, nb, loss_kind, d_upot, d_loss, d_grad);
+    else run_adjoint<double2>(P, nb, loss_kind, d_upot, d_loss, d_grad);
 }
 
 // Streams, events and plan views of the two lanes (created once, on first use).
@@ -1095,7 +1153,7 @@ rg_status rg_adjoint_potential_device(rg_plan *P, const double *upot_dev, int32_
     for (int32_t off = 0; off < nreal; off += cap) {
         const int nb = std::min(cap, nreal - off);
         HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
-        run_adjoint_f64(P, nb, loss_kind, upot_dev + per * off, loss_dev + off, grad_dev + per * off);
+        run_adjoint_any(P, nb, loss_kind, upot_dev + per * off, loss_dev + off, grad_dev + per * off);
         HIPCHK(hipGetLastError());
         if (status_dev)
             HIPCHK(hipMemcpyAsync(status_dev + off, P->d_status, sizeof(int32_t) * nb, hipMemcpyDeviceToDevice, P->stream));
@@ -1119,7 +1177,7 @@ rg_status rg_adjoint_potential(rg_plan *P, const double *upot, int32_t nreal, in
             if (!std::isfinite(upot[per * off + e])) return RG_ENONFINITE;
         HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
         HIPCHK(hipMemcpyAsync(P->d_upot, upot + per * off, sizeof(double) * per * nb, hipMemcpyHostToDevice, P->stream));
-        run_adjoint_f64(P, nb, loss_kind, P->d_upot, P->d_loss, P->d_grad);
+        run_adjoint_any(P, nb, loss_kind, P->d_upot, P->d_loss, P->d_grad);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(loss + off, P->d_loss, sizeof(double) * nb, hipMemcpyDeviceToHost, P->stream));
         HIPCHK(hipMemcpyAsync(grad + per * off, P->d_grad, sizeof(double) * per * nb, hipMemcpyDeviceToHost, P->stream));
@@ -1236,6 +1294,9 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
     } else if (std::strcmp(key, "graph") == 0) {
         if (value != 0 && value != 1) return RG_EINVAL;
         P->use_graph = value == 1;
+    } else if (std::strcmp(key, "adjoint_precision") == 0) {
+        if (value != RG_FP64 && value != RG_MIXED_FP32) return RG_EINVAL;
+        P->adj_prec = value;
     } else {
         return RG_EINVAL;
     }
@@ -1253,6 +1314,7 @@ rg_status rg_plan_get_option(const rg_plan *P, const char *key, int32_t *value)
     else if (std::strcmp(key, "eigen") == 0) *value = P->eig_jacobi ? 1 : 0;
     else if (std::strcmp(key, "profile_mask") == 0) *value = P->profile_mask;
     else if (std::strcmp(key, "graph") == 0) *value = P->use_graph ? 1 : 0;
+    else if (std::strcmp(key, "adjoint_precision") == 0) *value = P->adj_prec;
     else return RG_EINVAL;
     return RG_OK;
 }

@@ -13,24 +13,6 @@
 
 namespace {
 
-template <class E> struct CG;
-template <> struct CG<double2> {
-    typedef double R;
-    typedef double V4 __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ V4 mfma(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-    // v_mfma_f64_16x16x4f64: result register v of lane l holds D[4v + l/16][l%16]
-    static __device__ __forceinline__ int row(int v, int g) { return 4 * v + g; }
-    static __device__ __forceinline__ double2 make(double x, double y) { return make_double2(x, y); }
-};
-template <> struct CG<float2> {
-    typedef float R;
-    typedef float V4 __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ V4 mfma(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    // v_mfma_f32_16x16x4f32: result register v of lane l holds D[4(l/16) + v][l%16]
-    static __device__ __forceinline__ int row(int v, int g) { return 4 * g + v; }
-    static __device__ __forceinline__ float2 make(double x, double y) { return make_float2((float)x, (float)y); }
-};
-
 // C = alpha A B + beta C for one m x n output per realization (A m x k, B k x n; leading dimensions
 // lda/ldb/ldc, realization strides sA/sB/sC in elements).  64 x 64 output tile per workgroup,
 // 32 x 32 per wavefront, k in chunks of 16 staged through LDS as re/im planes.  beta == 0 does not
@@ -40,8 +22,8 @@ __global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, double2 alph
                                                size_t sA, const E *__restrict__ B, int ldb, size_t sB, double2 beta,
                                                E *__restrict__ C, int ldc, size_t sC)
 {
-    using R = typename CG<E>::R;
-    using V4 = typename CG<E>::V4;
+    using R = typename rg_mt<E>::R;
+    using V4 = typename rg_mt<E>::V4;
     __shared__ R AsR[16][66], AsI[16][66], BsR[16][66], BsI[16][66];
     const int b = blockIdx.z, ti0 = blockIdx.y * 64, tj0 = blockIdx.x * 64;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -96,10 +78,10 @@ __global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, double2 alph
             for (int x = 0; x < 2; x++)
 #pragma unroll
                 for (int y = 0; y < 2; y++) {
-                    cr[x][y] = CG<E>::mfma(ar[x], br[y], cr[x][y]);
-                    cr[x][y] = CG<E>::mfma(nai[x], bi[y], cr[x][y]);
-                    ci[x][y] = CG<E>::mfma(ar[x], bi[y], ci[x][y]);
-                    ci[x][y] = CG<E>::mfma(ai[x], br[y], ci[x][y]);
+                    cr[x][y] = rg_mt<E>::mfma(ar[x], br[y], cr[x][y]);
+                    cr[x][y] = rg_mt<E>::mfma(nai[x], bi[y], cr[x][y]);
+                    ci[x][y] = rg_mt<E>::mfma(ar[x], bi[y], ci[x][y]);
+                    ci[x][y] = rg_mt<E>::mfma(ai[x], br[y], ci[x][y]);
                 }
         }
         __syncthreads();
@@ -112,7 +94,7 @@ __global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, double2 alph
         for (int y = 0; y < 2; y++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int gr = ti0 + wr + 16 * x + CG<E>::row(v, lg), gc = tj0 + wc + 16 * y + li;
+                const int gr = ti0 + wr + 16 * x + rg_mt<E>::row(v, lg), gc = tj0 + wc + 16 * y + li;
                 if (gr >= m || gc >= n) continue;
                 const double xr = cr[x][y][v], xi = ci[x][y][v];
                 double outr = alpha.x * xr - alpha.y * xi, outi = alpha.x * xi + alpha.y * xr;
@@ -121,7 +103,7 @@ __global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, double2 alph
                     outr += beta.x * o.x - beta.y * o.y;
                     outi += beta.x * o.y + beta.y * o.x;
                 }
-                Cb[(size_t)gr * ldc + gc] = CG<E>::make(outr, outi);
+                Cb[(size_t)gr * ldc + gc] = rg_mt<E>::make(outr, outi);
             }
 }
 
@@ -197,7 +179,7 @@ __global__ __launch_bounds__(256) void k_form_blocks(const E *__restrict__ gss, 
         } else {
             m = cmk(row == col ? 1.0 : 0.0, 0.0);
         }
-        const E v = CG<E>::make(m.x, m.y);
+        const E v = rg_mt<E>::make(m.x, m.y);
         if (row < n1) {
             if (col < n1) Wa[row * n1 + col] = v;
             else Wb[row * n2 + (col - n1)] = v;

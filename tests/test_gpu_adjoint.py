@@ -131,3 +131,64 @@ def test_adjoint_tmax_matches_cpu_adjoint(name):
         assert val[k] == pytest.approx(vo, rel=1e-10)
         np.testing.assert_allclose(grad[:, :, k].T, go, rtol=0, atol=1e-8 * np.max(np.abs(go)))
     plan.close()
+
+
+# ---------------------------------------------------------------------------------------------
+# The complex64 adjoint (option adjoint_precision = mixed_fp32): FP32 state and products, FP64
+# loss, terminal adjoint and gradient.  Against the FP64 CPU adjoint on identical potentials:
+# the loss to 1e-4 relative; the gradient of g to 1e-3 of max|grad| (FP32 rounding over the
+# sweep); the gradient of T_max to 1e-3 + 1e-4 T1 / (T1 - T2), because it goes through the top
+# eigenvector of t^H t, whose perturbation by the FP32 error of t grows as 1 / (T1 - T2).  At the
+# c4 shape (N=64, 400 slices, ny=645), against the complex128 GPU adjoint over the whole grid in
+# the relative Frobenius norm, with the same bounds (2e-3 for g).
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("loss", ["g", "tmax"])
+@pytest.mark.parametrize("name", ["dirichlet_absorb", "periodic", "N20_ny205"])
+def test_fp32_adjoint_against_cpu_adjoint(name, loss):
+    p = ADJ[name]
+    say(f"fp32 adjoint vs oracle {name} {loss}")
+    ow = O.Waveguide(**p)
+    plan = binding.Plan(p, batch_max=2)
+    plan.set_option("adjoint_precision", binding.PRECISIONS["mixed_fp32"])
+    U = np.stack([ow.potential(s).T for s in (7, 8)], axis=2)
+    val, grad, st, code = plan.adjoint(U, loss)
+    assert code == 0 and not st.any()
+    for k in range(2):
+        vo, go = ow.adjoint_grad(U[:, :, k].T, loss)
+        assert val[k] == pytest.approx(vo, rel=1e-4)
+        err = np.max(np.abs(grad[:, :, k].T - go)) / np.max(np.abs(go))
+        T = ow.tdistrib_potential(U[:, :, k].T)
+        tol = 1e-3 + (1e-4 * T[0] / (T[0] - T[1]) if loss == "tmax" else 0.0)
+        print(f"[gpu-case] fp32 adjoint {name} {loss}: loss rel err {abs(val[k] - vo) / abs(vo):.2e}, grad err {err:.2e}, "
+              f"T1 {T[0]:.4f} gap {T[0] - T[1]:.2e}, bound {tol:.2e}", flush=True)
+        assert err <= tol
+    plan.close()
+
+
+@pytest.mark.parametrize("loss", ["g", "tmax"])
+def test_fp32_adjoint_c4_shape_against_fp64(loss):
+    import torch
+
+    from recurgreen_amd.autograd import transmission_loss
+
+    p = dict(boundtype="dirichlet", nxdiso=400, wol=32.25, aratio=20.0 / 32.25, dscat=2.0, ny=645, dos_model=1)
+    say(f"fp32 vs fp64 adjoint N64 nx400 {loss}")
+    plan = binding.Plan(p, batch_max=8)
+    plan.adjoint_reserve(8)
+    U = plan.generate_potential(31, 8)  # (nxdiso, ny, r)
+    u = torch.from_numpy(np.ascontiguousarray(U.transpose(2, 1, 0))).cuda().requires_grad_(True)
+    grads, vals = {}, {}
+    for prec in ("fp64", "mixed_fp32"):
+        if u.grad is not None:
+            u.grad = None
+        L = transmission_loss(u, plan, loss, prec)
+        L.sum().backward()
+        grads[prec] = u.grad.detach().cpu().numpy().copy()
+        vals[prec] = L.detach().cpu().numpy()
+    rel = np.linalg.norm(grads["mixed_fp32"] - grads["fp64"]) / np.linalg.norm(grads["fp64"])
+    lrel = np.max(np.abs(vals["mixed_fp32"] - vals["fp64"]) / np.abs(vals["fp64"]))
+    T = plan.tdistrib_potential(U, "fp64")[0].reshape(8, -1)
+    tol = 2e-3 + (1e-4 * np.max(T[:, 0] / (T[:, 0] - T[:, 1])) if loss == "tmax" else 0.0)
+    print(f"[gpu-case] c4 {loss}: gradient rel err {rel:.2e}, loss rel err {lrel:.2e}, bound {tol:.2e}", flush=True)
+    assert lrel <= 1e-4 and rel <= tol
+    plan.close()
