@@ -192,6 +192,47 @@ def uneven_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def adjoint_bench(args):
+    """Gradient of L = Tr(t^H t) and of T_max for the c4 waveguide (N=64, 400 slices, ny=645, batch
+    256 by default): one adjoint call per loss and precision on device potentials, timed after a
+    warmup call; the complex64 gradient is compared with the complex128 one."""
+    import numpy as np
+    import torch
+
+    from recurgreen_amd import binding
+
+    params = workload_params(args)
+    B = args.batch
+    plan = binding.Plan(params, batch_max=B)
+    plan.adjoint_reserve(B)
+    U = plan.generate_potential(1, B)
+    u = torch.from_numpy(np.ascontiguousarray(U.transpose(2, 1, 0))).cuda()
+    out = {}
+    for loss in ("g", "tmax"):
+        res = {}
+        for prec in ("fp64", "mixed_fp32"):
+            plan.set_option("adjoint_precision", binding.PRECISIONS[prec])
+            plan.adjoint_device(u, loss)
+            plan.synchronize()
+            t0 = time.perf_counter()
+            val, grad = plan.adjoint_device(u, loss)
+            plan.synchronize()
+            dt = time.perf_counter() - t0
+            res[prec] = (dt, val.cpu().numpy(), grad.cpu().numpy())
+        d64, v64, g64 = res["fp64"]
+        d32, v32, g32 = res["mixed_fp32"]
+        out[loss] = {"fp64_gradients_per_s": B / d64, "mixed_fp32_gradients_per_s": B / d32, "speedup": d64 / d32,
+                     "grad_rel_err_fro": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+                     "loss_max_rel_err": float(np.max(np.abs(v32 - v64) / np.abs(v64)))}
+    line = {"metric": "adjoint gradients/s (dL/dU over the whole grid), complex128 and complex64",
+            "value": out["g"]["mixed_fp32_gradients_per_s"], "unit": "gradients/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": f"{args.config}: W={params['wol']} nx={params['nxdiso']} ny={params['ny']} batch {B}"},
+            "losses": out}
+    plan.close()
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(mib):
     """Plain device-to-device copies of `mib` MiB (torch's copy kernel): the reference kernel whose
     FETCH_SIZE / WRITE_SIZE counters scale the sweep kernels' counters to bytes."""
@@ -222,6 +263,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uneven", type=int, default=0, metavar="K",
                     help="measure an uneven batch of K realizations (rg_tdistrib_uneven) instead")
+    ap.add_argument("--adjoint", action="store_true", help="time the adjoint (gradient) sweep instead (c4 by default)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch options instead of the autotuner's (recurgreen_amd.autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -231,8 +273,13 @@ def main():
         print(f"bench.py measures one GPU only (realizations are independent and would shard by seed "
               f"range across GPUs); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.adjoint and args.config == "n128":
+        args.config = "c4"
     if args.batch is None:
         args.batch = CONFIGS[args.config][3]
+    if args.adjoint:
+        adjoint_bench(args)
+        return
     if args.uneven:
         uneven_bench(args)
         return
