@@ -373,6 +373,12 @@ __global__ __launch_bounds__(EG_NT, 1) void k_tdt_eig(const E *__restrict__ gs1,
 
 }  // namespace
 
+bool rg_topvec_supported(const rg_plan *P)
+{
+    const int m = P->imax - P->imin + 1, nn = P->jmax - P->jmin + 1;
+    return (m < nn ? m : nn) <= EG_MAXN;
+}
+
 bool rg_eig_supported(const rg_plan *P)
 {
     const int m = P->imax - P->imin + 1, nn = P->jmax - P->jmin + 1;

@@ -234,6 +234,7 @@ void rg_launch_refl_trans_out(rg_plan *P, int nb, const double2 *g11, const doub
 void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status);
 void rg_launch_tdt_eig_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status);
 bool rg_eig_supported(const rg_plan *P);
+bool rg_topvec_supported(const rg_plan *P);  // top eigenvector (adjoint of T_max): order <= 128
 // top eigenvector of t^H t (or t t^H when t has fewer rows), 128 complex slots per realization
 void rg_launch_tdt_topvec_f64(rg_plan *P, int nb, const double2 *gs1, int32_t *status, double2 *xvec);
 void rg_launch_tdt_topvec_f32(rg_plan *P, int nb, const float2 *gs1, int32_t *status, double2 *xvec);

@@ -433,6 +433,16 @@ void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int prec
     }
 }
 
+// complex128 slice inversion of the reflection and adjoint sweeps: the plan's kernel choice, or the
+// Schur complement above np = 128.
+void invert_f64(rg_plan *P, int nb, int slot, int first, bool dis)
+{
+    if (P->np > 128) rg_launch_inverse_schur(P, nb, slot, first, dis, RG_FP64);
+    else if (P->inv_fp64 == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, first, dis);
+    else if (P->inv_fp64 == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, first, dis, RG_FP64);
+    else rg_launch_inverse_blocked(P, nb, slot, first, dis, RG_FP64);
+}
+
 // The reflection/transmission sweep of compute_refl_trans_v1 (src/waveguide.f90:469-519), FP64:
 // besides Gss and Gs1 it carries G11 and G1s, updated after Gs1 in the reference's order
 // (:498-505): Gs1 <- -Gss B Gs1, G11 <- G11 - G1s B Gs1 (new Gs1), G1s <- -G1s B Gss.
@@ -449,9 +459,7 @@ double2 *run_sweep_refl(rg_plan *P, int nb, int64_t seed0, const double *d_upot,
         const bool dis = id >= 1 && id <= P->nxdiso;
         if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
         const int slot = ix == 1 ? 0 : (ix == nx ? 2 : 1);
-        if (P->inv_fp64 == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
-        else if (P->inv_fp64 == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, RG_FP64);
-        else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, RG_FP64);
+        invert_f64(P, nb, slot, ix == 1, dis);
         if (ix == 1) {
             (void)hipMemcpyAsync(P->d_gs1[0], P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
             (void)hipMemcpyAsync(P->d_g11, P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
@@ -476,9 +484,7 @@ void fwd_step_f64(rg_plan *P, int nb, int ix, const double *d_upot, const double
     const bool dis = d_upot && id >= 1 && id <= P->nxdiso;
     if (dis) rg_launch_project(P, nb, id - 1, 0, d_upot, nb);
     const int slot = ix == 1 ? 0 : (ix == P->nx ? 2 : 1);
-    if (P->inv_fp64 == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
-    else if (P->inv_fp64 == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, RG_FP64);
-    else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, RG_FP64);
+    invert_f64(P, nb, slot, ix == 1, dis);
     if (ix == 1)
         (void)hipMemcpyAsync(s1_out, P->d_gss, sizeof(double2) * (size_t)nb * P->np * P->np, hipMemcpyDeviceToDevice,
                              P->stream);
@@ -493,7 +499,9 @@ void fwd_step_f32(rg_plan *P, int nb, int ix, const double *d_upot, const float2
     const bool dis = d_upot && id >= 1 && id <= P->nxdiso;
     if (dis) rg_launch_project(P, nb, id - 1, 0, d_upot, nb);
     const int slot = ix == 1 ? 0 : (ix == P->nx ? 2 : 1);
-    if (P->np >= 32) {
+    if (P->np > 128) {
+        rg_launch_inverse_schur(P, nb, slot, ix == 1, dis, RG_MIXED_FP32);
+    } else if (P->np >= 32) {
         rg_launch_inverse_nopiv(P, nb, slot, ix == 1, dis, P->d_redo);
         rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, RG_MIXED_FP32, P->d_redo);
     } else {
@@ -1106,7 +1114,7 @@ rg_status rg_refl_trans_potential(rg_plan *P, const double *upot, int32_t nreal,
 rg_status rg_adjoint_reserve(rg_plan *P, int32_t nreal_max)
 {
     if (!P || nreal_max < 1) return RG_EINVAL;
-    if (P->np > 128) return RG_ENOTSUP;
+    if (P->np > 256) return RG_ENOTSUP;
     DeviceGuard g(P->device);
     if (P->adj_cap >= nreal_max) return RG_OK;
     for (void *b : {(void *)P->d_ck, (void *)P->d_seg, (void *)P->d_adj, (void *)P->d_loss, (void *)P->d_grad})
@@ -1136,7 +1144,7 @@ static rg_status adjoint_check(rg_plan *P, int32_t nreal, int32_t loss_kind)
     rg_status st = check_common(P, nreal, RG_FP64);
     if (st) return st;
     if (loss_kind != 0 && loss_kind != 1) return RG_EINVAL;
-    if (loss_kind == 1 && !rg_eig_supported(P)) return RG_ENOTSUP;
+    if (loss_kind == 1 && !rg_topvec_supported(P)) return RG_ENOTSUP;
     if (P->adj_cap < 1) return RG_EINVAL;  // rg_adjoint_reserve first
     return RG_OK;
 }

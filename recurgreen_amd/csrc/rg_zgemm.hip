@@ -138,6 +138,14 @@ static void zgemm_any(rg_plan *P, int nb, double2 alpha, const E *A, int opA, co
         RG_ZG(6)
         RG_ZG(7)
         RG_ZG(8)
+        RG_ZG(9)
+        RG_ZG(10)
+        RG_ZG(11)
+        RG_ZG(12)
+        RG_ZG(13)
+        RG_ZG(14)
+        RG_ZG(15)
+        RG_ZG(16)
 #undef RG_ZG
         default: break;
     }

@@ -68,3 +68,41 @@ def test_seeded_n256_batch_matches_explicit():
     assert c1 == 0 and c2 == 0
     assert np.array_equal(T1, T2)
     plan.close()
+
+
+@pytest.mark.parametrize("name", ["N200_absorb", "periodic_N161"])
+def test_large_refl_trans_parity(name):
+    """compute_refl_trans_v1 above N = 128 (Schur inversion, G11/G1s products at np > 128)."""
+    p = CASES[name]
+    say(f"large N refl/trans {name}")
+    ow = O.Waveguide(**p)
+    plan = binding.Plan(p, batch_max=2)
+    U = upot_batch(ow, [5, 6])
+    r, t, st, code = plan.refl_trans(U)
+    assert code == 0 and not st.any()
+    for k in range(2):
+        ro, to = ow.refl_trans(U[:, :, k].T, version=1)
+        assert np.max(np.abs(t[k] - to)) <= 1e-10 * max(1.0, np.max(np.abs(to)))
+        assert np.max(np.abs(r[k] - ro)) <= 1e-10 * max(1.0, np.max(np.abs(ro)))
+    plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed_fp32"])
+def test_large_adjoint_against_cpu_adjoint(precision):
+    """Adjoint of g = Tr(t^H t) at N = 161 (periodic, np = 176) against the FP64 CPU adjoint:
+    complex128 to 1e-9 of max|grad|, complex64 to 1e-3; T_max above order 128 is refused."""
+    p = CASES["periodic_N161"]
+    say(f"large N adjoint {precision}")
+    ow = O.Waveguide(**p)
+    plan = binding.Plan(p, batch_max=1)
+    plan.set_option("adjoint_precision", binding.PRECISIONS[precision])
+    U = upot_batch(ow, [9])
+    val, grad, st, code = plan.adjoint(U, "g")
+    assert code == 0 and not st.any()
+    vo, go = ow.adjoint_grad(U[:, :, 0].T, "g")
+    tol_v, tol_g = (1e-10, 1e-9) if precision == "fp64" else (1e-4, 1e-3)
+    assert val[0] == pytest.approx(vo, rel=tol_v)
+    assert np.max(np.abs(grad[:, :, 0].T - go)) <= tol_g * np.max(np.abs(go))
+    with pytest.raises(binding.RecurGreenError):
+        plan.adjoint(U, "tmax")
+    plan.close()
