@@ -1,5 +1,5 @@
 !!**************************************************************************************************
-!! recurgreen_gpu.f90 -- namelist-compatible `tdistrib` driver whose sweep runs on the GPU.
+!! recurgreen_gpu.f90 -- namelist-compatible `tdistrib` and `tpattern` driver whose sweep runs on the GPU.
 !!
 !! Usage: recurgreen_gpu settings.nml [more.nml ...]
 !! Reads the reference's groups unchanged: &settings task (src/main.f90:45), &waveguide and
@@ -8,7 +8,9 @@
 !! that has no counterpart in the reference (a file without it parses as before).  The OpenMP loop of
 !! compute_eigvals_omp (src/main.f90:134-179) becomes one batched device pass; the histogram chain
 !! follows prepare_distrib (src/main.f90:223-251) and the CSV layout of save_distrib / save_raw_eigvals
-!! (src/main.f90:257-328), without the TikZ step.
+!! (src/main.f90:257-328), without the TikZ step.  Task `tpattern` (src/main.f90:735-901) reads
+!! &tpattern nseed, imode, nthreads, probatype and averages |t|^2, |r|^2 per channel over batched
+!! complex128 reflection/transmission passes.
 !!**************************************************************************************************
 program recurgreen_gpu
     use, intrinsic :: iso_fortran_env, only: real64, error_unit, output_unit
@@ -60,7 +62,9 @@ contains
         select case (task)
             case ("tdistrib")
                 call task_tdistrib(funit)
-            case ("wavefunction", "tprofile", "tprofile_group", "tpattern")
+            case ("tpattern")
+                call task_tpattern(funit)
+            case ("wavefunction", "tprofile", "tprofile_group")
                 write (error_unit, '(a)') "[ERROR] Task '" // trim(task) // "' is not part of the GPU sweep, aborting..."
                 stop errmsg_invalid_arg
             case default
@@ -139,6 +143,125 @@ contains
         rho(:, 2) = histo
         call save_files(wguide, rho, tallvals, nseed, nbins, time, speed, save_eigvals)
         call wguide%del()
+    end subroutine
+
+    !! task_tpattern (src/main.f90:735-856) for probatype 'incident' and 'total': the seed loop of
+    !! compute_refl_trans_v1 (:790-835) runs as batched complex128 device passes of `batch`
+    !! realizations; the per-channel probabilities are summed pass by pass and averaged (:838-840).
+    !! probatype 'deposition' needs the interior Green function of compute_deposition, which the
+    !! transmission sweep does not carry, and stops with an error.  The &gpu group is read as in
+    !! task_tdistrib; its precision and bintype do not apply (reflection needs complex128).
+    subroutine task_tpattern(funit)
+        integer, intent(in) :: funit
+        type(waveguide_gpu_t) :: wguide
+        character(len=50) :: boundtype, probatype, precision, bintype
+        real(wp) :: wol, aratio, exfac, dscat, dabso, napera, naperb, time
+        integer :: nxdiso, nxfree, nseed, imode, nthreads, istat, device, batch, ny, dos_model
+        integer :: nmu, imu, off, nb, k, bsz
+        integer(8) :: c0, c1, crate
+        complex(wp), allocatable :: rmat(:, :, :), tmat(:, :, :)
+        real(wp), allocatable :: tprob(:), rprob(:)
+        namelist /waveguide/ boundtype, wol, exfac, dscat, dabso, aratio, nxdiso, nxfree, napera, naperb
+        namelist /tpattern/ nseed, imode, nthreads, probatype
+        namelist /gpu/ device, precision, batch, ny, dos_model, bintype
+
+        boundtype = "dirichlet"; wol = 4.3_wp; exfac = 1.0_wp; aratio = 1.0_wp; dscat = 1.0_wp; dabso = 0.0_wp
+        nxdiso = 100; nxfree = 0; napera = 1.0_wp; naperb = 1.0_wp
+        nseed = 1; imode = 1; nthreads = 1; probatype = "incident"
+        device = 0; precision = "fp64"; batch = 256; ny = 0; dos_model = 0; bintype = "chebyshev"
+        read(unit=funit, nml=waveguide, iostat=istat)
+        call rewind_or_stop(funit, istat, .false.)
+        read(unit=funit, nml=tpattern, iostat=istat)
+        call rewind_or_stop(funit, istat, .false.)
+        read(unit=funit, nml=gpu, iostat=istat)
+        call rewind_or_stop(funit, istat, .true.)
+        if (nseed <= 0) then
+            write (error_unit, '(a,i0,a)') "[ERROR] Invalid number of disorder realizations (received ", nseed, &
+                "), aborting..."
+            stop errmsg_invalid_arg
+        else if (probatype /= "incident" .and. probatype /= "total" .and. probatype /= "deposition") then
+            write (error_unit, '(a)') "[ERROR] Invalid probatype, received '" // trim(probatype) // &
+                "', expected incident/total/deposition, aborting..."
+            stop errmsg_invalid_arg
+        else if (probatype == "deposition") then
+            write (error_unit, '(a)') "[ERROR] probatype='deposition' is not part of the GPU sweep, aborting..."
+            stop errmsg_invalid_arg
+        end if
+
+        call wguide%set_options(ny, dos_model, RG_FP64)
+        call wguide%init(boundtype, nxdiso, nxfree, wol, exfac, aratio, dscat, dabso, napera, naperb, &
+                         batch_max=max(1, min(batch, nseed)), device=device)
+        nmu = wguide%get_nmu()
+        if (boundtype == "periodic") then   !! imode=1 is the frontal mode (src/main.f90:768-772)
+            imu = mod(imode + nmu/2 - 1, nmu) + 1
+        else
+            imu = imode
+        end if
+        if (probatype == "incident" .and. (imu < 1 .or. imu > nmu)) then
+            write (error_unit, '(2(a,i0),a)') "[ERROR] Invalid incident mode imode=", imode, ", expected in [1, ", nmu, &
+                "], aborting..."
+            stop errmsg_invalid_arg
+        end if
+        call wguide%print_parameters(output_unit, "[INFO]")
+        write (output_unit, '(2(a,i0),3a)') "[INFO] Nseed=", nseed, ", Imode=", imode, ", Probatype='", &
+            trim(probatype), "'"
+
+        bsz = max(1, min(batch, nseed))
+        allocate(rmat(nmu, nmu, bsz), tmat(nmu, nmu, bsz), tprob(nmu), rprob(nmu))
+        tprob = 0.0_wp
+        rprob = 0.0_wp
+        call system_clock(c0, crate)
+        do off = 0, nseed - 1, bsz
+            nb = min(bsz, nseed - off)
+            call wguide%compute_refl_trans_batch(1 + off, nb, rmat(:, :, 1:nb), tmat(:, :, 1:nb))
+            do k = 1, nb
+                if (probatype == "incident") then
+                    tprob = tprob + abs(tmat(:, imu, k))**2
+                    rprob = rprob + abs(rmat(:, imu, k))**2
+                else
+                    tprob = tprob + sum(abs(tmat(:, :, k))**2, dim=1)
+                    rprob = rprob + sum(abs(rmat(:, :, k))**2, dim=1)
+                end if
+            end do
+        end do
+        call system_clock(c1)
+        time = real(c1 - c0, wp)/real(crate, wp)
+        tprob = tprob/nseed
+        rprob = rprob/nseed
+        write (output_unit, '(a,f0.3,a)') "[INFO] Done, computation time is ", time, " s."
+        call save_tpattern(wguide, tprob, rprob, probatype, imode, nseed, time)
+        call wguide%del()
+    end subroutine
+
+    !! save_tpattern (src/main.f90:861-901) without the TikZ step.
+    subroutine save_tpattern(wguide, tprob, rprob, probatype, imode, nseed, time)
+        type(waveguide_gpu_t), intent(in) :: wguide
+        real(wp), intent(in) :: tprob(:), rprob(:), time
+        character(len=*), intent(in) :: probatype
+        integer, intent(in) :: imode, nseed
+        character(len=200) :: filename
+        character(len=40) :: zone
+        integer :: outfp, imu, ifile, dt(8)
+        logical :: fexist
+        call execute_command_line("mkdir -p out_gpu/tpattern")
+        do ifile = 1, 1000000   !! unique_filename (src/base_utils.f90:825-840)
+            write (filename, '(a,i0,a)') "out_gpu/tpattern/result_", ifile, ".csv"
+            inquire(file=filename, exist=fexist)
+            if (.not. fexist) exit
+        end do
+        open(newunit=outfp, file=filename, action='write')
+        call date_and_time(values=dt, zone=zone)   !! print_timestamp (src/base_utils.f90:703-716)
+        write (outfp, '(a,i4,a,5(i0.2,a))') "%% Computed on ", dt(1), "-", dt(2), "-", dt(3), " at ", &
+            dt(5), ":", dt(6), ":", dt(7), " " // trim(zone) // " by RecurGreen-HIP"
+        call wguide%print_parameters(outfp, "%%")
+        write (outfp, '(2(a,i0),a,f0.3,a)') "%% Radiation pattern parameters: probatype='" // trim(probatype) // &
+            "', nseed=", nseed, ", imode=", imode, ", time=", time, " s."
+        write (outfp, '(a)') "imu, tprob, rprob"
+        do imu = 1, size(tprob)
+            write (outfp, '(i0,", ",g0.16,", ",g0.16)') imu, tprob(imu), rprob(imu)
+        end do
+        close(outfp)
+        write (output_unit, '(a)') "[INFO] Average profiles written in file: " // trim(filename)
     end subroutine
 
     subroutine save_files(wguide, rho, tallvals, nseed, nbins, time, speed, save_eigvals)

@@ -51,6 +51,14 @@ module recurgreen_hip
             real(c_double), intent(out) :: tallvals(*)
             integer(c_int32_t), intent(out) :: status(*)
         end function
+        integer(c_int) function rg_refl_trans(plan, seed0, nseed, rmat, tmat, status) bind(C, name="rg_refl_trans")
+            import :: c_int, c_int32_t, c_int64_t, c_double_complex, c_ptr
+            type(c_ptr), value :: plan
+            integer(c_int64_t), value :: seed0
+            integer(c_int32_t), value :: nseed
+            complex(c_double_complex), intent(out) :: rmat(*), tmat(*)
+            integer(c_int32_t), intent(out) :: status(*)
+        end function
         integer(c_int) function rg_histogram(plan, samples, nsamp, on_device, bins, nbins, counts, missing, dropped) &
                 bind(C, name="rg_histogram")
             import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
@@ -85,6 +93,8 @@ module recurgreen_hip
         procedure :: get_nx => get_nx_waveguide_gpu
         procedure :: compute_tdistrib => compute_tdistrib_waveguide_gpu
         procedure :: compute_tdistrib_batch => compute_tdistrib_batch_waveguide_gpu
+        procedure :: compute_refl_trans => compute_refl_trans_waveguide_gpu
+        procedure :: compute_refl_trans_batch => compute_refl_trans_batch_waveguide_gpu
         procedure :: histogram => histogram_waveguide_gpu
         procedure :: print_parameters => print_parameters_waveguide_gpu
         procedure :: del => del_waveguide_gpu
@@ -219,6 +229,43 @@ contains
         end if
         allocate(st(nseed))
         call check_status(rg_tdistrib(self%plan, int(seed0, c_int64_t), int(nseed, c_int32_t), self%precision, tallvals, st))
+    end subroutine
+
+    !! compute_refl_trans_v1 (src/waveguide.f90:469-519) of realization iseed: rmat and tmat are
+    !! nmu x nmu, r = -I + 2i s G11 s and t = 2i s G(L;0) s with s = sqrt(Re mu) (:508-512).
+    subroutine compute_refl_trans_waveguide_gpu(self, iseed, rmat, tmat)
+        class(waveguide_gpu_t), intent(inout) :: self
+        integer, intent(in) :: iseed
+        complex(wp), intent(inout), contiguous :: rmat(:, :), tmat(:, :)
+        integer(c_int32_t) :: st(1)
+        call check_refl_trans_shape(self, size(rmat, 1), size(rmat, 2), 1, shape(tmat), 1)
+        call check_status(rg_refl_trans(self%plan, int(iseed, c_int64_t), 1_c_int32_t, rmat, tmat, st))
+    end subroutine
+
+    !! Realizations seed0 .. seed0+nseed-1 in batched device passes: rmat(:, :, k) and tmat(:, :, k)
+    !! belong to realization seed0+k-1.
+    subroutine compute_refl_trans_batch_waveguide_gpu(self, seed0, nseed, rmat, tmat)
+        class(waveguide_gpu_t), intent(inout) :: self
+        integer, intent(in) :: seed0, nseed
+        complex(wp), intent(inout), contiguous :: rmat(:, :, :), tmat(:, :, :)
+        integer(c_int32_t), allocatable :: st(:)
+        call check_refl_trans_shape(self, size(rmat, 1), size(rmat, 2), size(rmat, 3), shape(tmat), nseed)
+        allocate(st(nseed))
+        call check_status(rg_refl_trans(self%plan, int(seed0, c_int64_t), int(nseed, c_int32_t), rmat, tmat, st))
+    end subroutine
+
+    subroutine check_refl_trans_shape(self, n1, n2, n3, tshape, nseed)
+        class(waveguide_gpu_t), intent(in) :: self
+        integer, intent(in) :: n1, n2, n3, tshape(:), nseed
+        integer :: t3
+        t3 = 1
+        if (size(tshape) == 3) t3 = tshape(3)
+        if (n1 /= self%nmu .or. n2 /= self%nmu .or. tshape(1) /= self%nmu .or. tshape(2) /= self%nmu &
+                .or. n3 < nseed .or. t3 < nseed .or. nseed < 1) then
+            write (error_unit, '(2(a,i0),a)') "[ERROR] Invalid reflection/transmission array shape, expected ", self%nmu, &
+                " x ", self%nmu, " per realization..."
+            stop errmsg_invalid_arg
+        end if
     end subroutine
 
     !! Device histogram with the rules of src/base_utils.f90:491-516 (counts returned as reals).
