@@ -15,7 +15,8 @@
 // Layout: wavefront w owns the 16-column block w as np/16 MFMA accumulator tiles of 16x16 (real
 // and imaginary planes), register v of lane l holding row 4(l/16) + v, column l%16
 // (v_mfma_f32_16x16x4_f32).  Block step K (D = tile (K,K), P = column block K):
-//   wavefront K: Gauss-Jordan on D in registers (16 steps, lane shuffles) -> D^-1, pivots, and
+//   wavefront K: Gauss-Jordan on D in registers (16 unrolled steps, cross-lane broadcasts by
+//   v_permlane32/16_swap and DPP, pivot checks after the last step) -> D^-1, pivots, and
 //   W = D^-1 L_D diag(u) = U_D^-1 diag(u); writes P (rows outside K), D^-1 and W to LDS;
 //   every wavefront: Rt = D^-1 X[K, w] (its own tile K), X[K, w] := Rt, X[i, w] -= P_i Rt (i != K)
 //   (wavefront K: X[i, K] := -P_i D^-1, X[K, K] := D^-1), and checks the rows of its own tile
@@ -24,6 +25,7 @@
 // Look-ahead: wavefront K+1 updates its diagonal tile first and inverts the next D before its
 // other tiles, so one workgroup barrier per block step.
 #include "rg_internal.h"
+#include <utility>
 
 namespace {
 
@@ -43,70 +45,115 @@ struct NopivShared {
     alignas(16) double cq[2 * 256 + 8];
 };
 
-__device__ inline float shfl_f(float v, int src) { return __shfl(v, src, 64); }
-
 __device__ inline float readlane_f(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
 
-// Gauss-Jordan on the 16x16 block (xr, xi) held in accumulator layout by one wavefront, diagonal
-// pivots.  On return (xr, xi) = D^-1, (lr, li) = L_D diag(u) (the column of the reduced block at
-// each step, rows >= step), ua[j] = |u_j|_1; returns false if a pivot fails the zgetrf check (FP64
-// comparisons) or is zero.  The pivot comes from one lane (readlane) and its reciprocal is formed
-// while the pivot row and column travel through the lane shuffles, which keeps one step short.
+// Lane (c, g) <- lane (c, R) of x: row R of the 16 x 4 lane grid broadcast to every row
+// (v_permlane32_swap then v_permlane16_swap of x with itself).
+template <int R>
+__device__ __attribute__((always_inline)) inline float rows_bcast(float x)
+{
+    const unsigned u = __float_as_uint(x);
+    const auto p = __builtin_amdgcn_permlane32_swap(u, u, false, false);  // [r0 r1 r0 r1], [r2 r3 r2 r3]
+    const unsigned t = R < 2 ? p[0] : p[1];
+    const auto q = __builtin_amdgcn_permlane16_swap(t, t, false, false);  // even rows, odd rows
+    return __uint_as_float(R % 2 == 0 ? q[0] : q[1]);
+}
+
+// Lane (c, g) <- lane (J, g): DPP row_newbcast:J.
+template <int J>
+__device__ __attribute__((always_inline)) inline float lane_bcast(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x150 + J, 0xF, 0xF, false));
+}
+
+// Step J of the Gauss-Jordan elimination of a 16x16 block (xr, xi) held in accumulator layout by
+// one wavefront, diagonal pivot.  The pivot comes from its lane (readlane), the pivot row from a
+// row broadcast, the pivot column from a DPP lane broadcast: all register-to-register, with J a
+// constant.  Lane c == J records the pivot magnitude and the reduced column (rows >= J) for the
+// checks and for L_D diag(u).
+template <int J>
+__device__ __attribute__((always_inline)) inline void gj_step(np_f4 &xr, np_f4 &xi, np_f4 &lr, np_f4 &li, double &my_ua,
+                                                              bool &my_den, int lane)
+{
+    constexpr int VJ = J & 3, G = J >> 2;
+    const int c = lane & 15, g = lane >> 4;
+    // pivot D[J][J] from its lane; pivot row D[J][c]; column J of my rows D[4g+v][J]
+    const float pvr = readlane_f(xr[VJ], J + 16 * G), pvi = readlane_f(xi[VJ], J + 16 * G);
+    const float prr = rows_bcast<G>(xr[VJ]), pri = rows_bcast<G>(xi[VJ]);
+    float mr[4], mi[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        mr[v] = lane_bcast<J>(xr[v]);
+        mi[v] = lane_bcast<J>(xi[v]);
+    }
+    const float den = pvr * pvr + pvi * pvi;
+    // 1/den: v_rcp_f32 (1 ulp) and one Newton step, which removes its one-sided error (a
+    // per-slice bias that would otherwise build up over the sweep) at 2 FMAs of latency
+    float inv = 1.0f;
+    if (den > 0.f) {
+        const float r0 = __builtin_amdgcn_rcpf(den);
+        inv = fmaf(r0, fmaf(-den, r0, 1.0f), r0);
+    }
+    const float dr = pvr * inv, di = -pvi * inv;
+    if (c == J) {
+        my_ua = fabs((double)pvr) + fabs((double)pvi);
+        my_den = den > 0.f;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int row = 4 * g + v;
+        const bool cap = c == J && row >= J;
+        lr[v] = cap ? mr[v] : lr[v];
+        li[v] = cap ? mi[v] : li[v];
+    }
+    // scaled pivot row (column J of X reset to e_J first, so its entry is d)
+    float rvr, rvi;
+    if (c == J) { rvr = dr; rvi = di; }
+    else { rvr = prr * dr - pri * di; rvi = prr * di + pri * dr; }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int row = 4 * g + v;
+        float mur = mr[v], mui = mi[v];
+        if (row == J) { mur = pvr - 1.0f; mui = pvi; }
+        float xrv = xr[v], xiv = xi[v];
+        if (c == J) { xrv = row == J ? 1.0f : 0.0f; xiv = 0.0f; }
+        xr[v] = xrv - (mur * rvr - mui * rvi);
+        xi[v] = xiv - (mur * rvi + mui * rvr);
+    }
+}
+
+template <int... Js>
+__device__ __attribute__((always_inline)) inline void gj_steps(std::integer_sequence<int, Js...>, np_f4 &xr, np_f4 &xi,
+                                                               np_f4 &lr, np_f4 &li, double &my_ua, bool &my_den,
+                                                               int lane)
+{
+    (gj_step<Js>(xr, xi, lr, li, my_ua, my_den, lane), ...);
+}
+
+// Gauss-Jordan on the 16x16 block, 16 steps unrolled.  On return (xr, xi) = D^-1, (lr, li) =
+// L_D diag(u) (the column of the reduced block at each step, rows >= step), ua[j] = |u_j|_1;
+// returns false if a pivot fails the zgetrf check (FP64 comparisons) or is zero.
 __device__ __attribute__((always_inline)) inline bool block_gj(np_f4 &xr, np_f4 &xi, np_f4 &lr, np_f4 &li, double *ua,
                                                                int lane)
 {
     const int c = lane & 15, g = lane >> 4;
-    bool ok = true;
     lr = np_f4{0.f, 0.f, 0.f, 0.f};
     li = np_f4{0.f, 0.f, 0.f, 0.f};
     double my_ua = 0.0;  // lane c keeps |u_c|_1
-#pragma unroll 4
-    for (int j = 0; j < 16; j++) {
-        const int vj = j & 3, src_row = c + 16 * (j >> 2);
-        // pivot D[j][j] from its lane; pivot row D[j][c]; column j of my rows D[4g+v][j]
-        const float pvr = readlane_f(xr[vj], j + 16 * (j >> 2)), pvi = readlane_f(xi[vj], j + 16 * (j >> 2));
-        const float prr = shfl_f(xr[vj], src_row), pri = shfl_f(xi[vj], src_row);
-        float mr[4], mi[4];
+    bool my_den = true;  // and whether |u_c|^2 > 0 in FP32
+    // the per-step lane masks are recomputed here rather than hoisted out of the caller's loop
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    gj_steps(std::make_integer_sequence<int, 16>{}, xr, xi, lr, li, my_ua, my_den, ln);
+    // The pivot checks of all 16 steps at once: lane c holds the candidates of step c (column c
+    // of the reduced block, rows >= c, in lr/li) and |u_c|_1; a row below the pivot whose entry
+    // exceeds the margin means zgetrf would have picked another pivot.
+    bool ok = my_ua > 0.0 && my_den;
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            mr[v] = shfl_f(xr[v], j + 16 * g);
-            mi[v] = shfl_f(xi[v], j + 16 * g);
-        }
-        const float den = pvr * pvr + pvi * pvi;
-        // 1/den: v_rcp_f32 (1 ulp) and one Newton step, which removes its one-sided error (a
-        // per-slice bias that would otherwise build up over the sweep) at 2 FMAs of latency
-        float inv = 1.0f;
-        if (den > 0.f) {
-            const float r0 = __builtin_amdgcn_rcpf(den);
-            inv = fmaf(r0, fmaf(-den, r0, 1.0f), r0);
-        }
-        const float dr = pvr * inv, di = -pvi * inv;
-        const double apv = fabs((double)pvr) + fabs((double)pvi);
-        ok = ok && apv > 0.0 && den > 0.f;
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = 4 * g + v;
-            const bool big = fabs((double)mr[v]) + fabs((double)mi[v]) > NP_MARGIN * apv;
-            ok = ok && !(row > j && big);
-            const bool cap = c == j && row >= j;
-            lr[v] = cap ? mr[v] : lr[v];
-            li[v] = cap ? mi[v] : li[v];
-        }
-        my_ua = c == j ? apv : my_ua;
-        // scaled pivot row (column j of X reset to e_j first, so its entry is d)
-        float rvr, rvi;
-        if (c == j) { rvr = dr; rvi = di; }
-        else { rvr = prr * dr - pri * di; rvi = prr * di + pri * dr; }
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = 4 * g + v;
-            float mur = mr[v], mui = mi[v];
-            if (row == j) { mur = pvr - 1.0f; mui = pvi; }
-            float xrv = xr[v], xiv = xi[v];
-            if (c == j) { xrv = row == j ? 1.0f : 0.0f; xiv = 0.0f; }
-            xr[v] = xrv - (mur * rvr - mui * rvi);
-            xi[v] = xiv - (mur * rvi + mui * rvr);
-        }
+    for (int v = 0; v < 4; v++) {
+        const int row = 4 * g + v;
+        const bool big = fabs((double)lr[v]) + fabs((double)li[v]) > NP_MARGIN * my_ua;
+        ok = ok && !(row > c && big);
     }
     if (lane < 16) ua[c] = my_ua;
     // every lane must agree: combine the flags over the wavefront
