@@ -68,6 +68,8 @@ def lib():
         for name in ("rgo_refl_trans_v1", "rgo_refl_trans_v2"):
             getattr(L, name).restype = C.c_int
             getattr(L, name).argtypes = [vp, dp, vp, vp]
+        L.rgo_deposition.restype = C.c_int
+        L.rgo_deposition.argtypes = [vp, dp, C.c_int, C.c_int, vp]
         L.rgo_solve_block_tridiag.restype = C.c_int
         L.rgo_solve_block_tridiag.argtypes = [C.c_int] * 3 + [vp] * 4
         L.rgo_bins.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dp]
@@ -217,6 +219,15 @@ class Waveguide:
         if st:
             raise OracleError(f"refl_trans status {st}")
         return r, t
+
+    def deposition(self, upot, fpos: int, ipos: int = 1) -> np.ndarray:
+        """depmat (nmu, nmu) of compute_deposition (src/waveguide.f90:712-750), 1-based positions."""
+        u = self._upot(upot)
+        d = np.zeros((self.nmu, self.nmu), complex)
+        st = lib().rgo_deposition(self._h, _dp(u) if u is not None else None, int(fpos), int(ipos), _vp(d))
+        if st:
+            raise OracleError(f"deposition status {st}")
+        return d
 
     def adjoint_grad(self, upot, loss: str = "g"):
         """FP64 adjoint: (loss value, d loss / d upot) with upot in U*dx/k units."""

@@ -13,6 +13,7 @@
  *   - r/t sweep, version 1        src/waveguide.f90:469-519  (rgo_refl_trans_v1)
  *   - r/t by block-tridiag solve  src/waveguide.f90:526-553,
  *                                 src/base_utils.f90:296-335 (rgo_refl_trans_v2)
+ *   - deposition matrix           src/waveguide.f90:712-750  (rgo_deposition)
  *   - zgetrf+zgetri inverse       src/base_utils.f90:235-274 (rgo_inverse)
  *   - zgesvd singular values      src/base_utils.f90:341-380 (rgo_singular_values,
  *                                 one-sided Jacobi instead of LAPACK)
@@ -665,6 +666,34 @@ int rgo_refl_trans_v2(const rgo_wg *w, const double *upot, zc *r, zc *t)
                 double sj = sqrt(creal(w->mumesh[j]));
                 t[(size_t)i * n + j] = 2.0 * I * ((si * green[nn * (nx - 1) + (size_t)i * n + j]) * sj); /* :547 */
                 r[(size_t)i * n + j] = (i == j ? -1.0 : 0.0) + 2.0 * I * ((si * green[(size_t)i * n + j]) * sj); /* :548 */
+            }
+        }
+    free(amat); free(ind); free(green); free(work);
+    return st;
+}
+
+/* Deposition matrix (src/waveguide.f90:712-750): the column block ipos of H^-1 by the block-tridiagonal
+ * solve with the identity at block ipos (:736-739), then dep = 2i s G(fpos; ipos) s with s = sqrt(Re mu)
+ * (:741-742).  fpos, ipos in 1..nx. */
+int rgo_deposition(const rgo_wg *w, const double *upot, int fpos, int ipos, zc *dep)
+{
+    int n = w->nmu, nx = w->nx;
+    if (fpos < 1 || fpos > nx || ipos < 1 || ipos > nx) return RGO_EINVAL;
+    size_t nn = (size_t)n * n;
+    zc *amat = (zc *)malloc(sizeof(zc) * nn * nx), *ind = (zc *)calloc(nn * nx, sizeof(zc));
+    zc *green = (zc *)malloc(sizeof(zc) * nn * nx), *work = (zc *)malloc(sizeof(zc) * nn);
+    int st = (amat && ind && green && work) ? RGO_OK : RGO_ENOMEM;
+    if (!st) {
+        for (int ix = 1; ix <= nx; ix++) rgo_slice_matrix(w, ix, upot, amat + nn * (ix - 1), work);
+        for (int m = 0; m < n; m++) ind[nn * (ipos - 1) + (size_t)m * n + m] = 1.0;
+        st = rgo_solve_block_tridiag(nx, n, n, amat, w->bmat, ind, green);
+    }
+    if (!st)
+        for (int i = 0; i < n; i++) {
+            double si = sqrt(creal(w->mumesh[i]));
+            for (int j = 0; j < n; j++) {
+                double sj = sqrt(creal(w->mumesh[j]));
+                dep[(size_t)i * n + j] = 2.0 * I * ((si * green[nn * (fpos - 1) + (size_t)i * n + j]) * sj);
             }
         }
     free(amat); free(ind); free(green); free(work);

@@ -21,7 +21,7 @@ BOUNDTYPES = {"dirichlet": 0, "periodic": 1}
 # Every symbol include/recurgreen_hip.h declares.
 EXPORTS = (
     "rg_plan_create", "rg_plan_destroy", "rg_plan_dims", "rg_tdistrib", "rg_tdistrib_device",
-    "rg_tdistrib_potential", "rg_transmission_potential", "rg_refl_trans", "rg_refl_trans_potential",
+    "rg_tdistrib_potential", "rg_transmission_potential", "rg_refl_trans", "rg_refl_trans_potential", "rg_deposition", "rg_deposition_potential",
     "rg_tdistrib_uneven",
     "rg_adjoint_reserve", "rg_adjoint_potential", "rg_adjoint_potential_device",
     "rg_generate_potential", "rg_histogram",
@@ -83,6 +83,8 @@ def lib():
         "rg_adjoint_potential": [vp, vp, i32, i32, vp, vp, vp],
         "rg_adjoint_potential_device": [vp, vp, i32, i32, vp, vp, vp],
         "rg_refl_trans_potential": [vp, vp, i32, vp, vp, vp],
+        "rg_deposition": [vp, i64, i32, i32, i32, vp, vp],
+        "rg_deposition_potential": [vp, vp, i32, i32, i32, vp, vp],
         "rg_generate_potential": [vp, i64, i32, vp],
         "rg_histogram": [vp, vp, i64, i32, dp, i32, i64p, i64p, i64p],
         "rg_plan_stream": [vp, C.POINTER(vp)],
@@ -262,6 +264,25 @@ class Plan:
         if code not in (RG_OK, RG_ESINGULAR, RG_ENONFINITE):
             check(code)
         return (np.ascontiguousarray(r.transpose(0, 2, 1)), np.ascontiguousarray(t.transpose(0, 2, 1)), st, code)
+
+    def deposition(self, fpos: int, ipos: int = 1, upot: np.ndarray | None = None, seed0: int = 1, nseed: int = 1):
+        """(dep, status, code) of compute_deposition (src/waveguide.f90:712-750), complex128: dep is
+        (nreal, nmu, nmu), 1-based slice positions; explicit potentials upot(ix, iy[, r]) or seeds."""
+        n = self.nmu
+        if upot is not None:
+            u = np.asfortranarray(upot, dtype=np.float64)
+            nreal = 1 if u.ndim == 2 else u.shape[2]
+        else:
+            u, nreal = None, int(nseed)
+        d = np.zeros((nreal, n, n), complex)  # column-major per realization
+        st = np.zeros(nreal, np.int32)
+        if u is None:
+            code = lib().rg_deposition(self._h, int(seed0), nreal, int(fpos), int(ipos), ptr(d), ptr(st))
+        else:
+            code = lib().rg_deposition_potential(self._h, ptr(u), nreal, int(fpos), int(ipos), ptr(d), ptr(st))
+        if code not in (RG_OK, RG_ESINGULAR, RG_ENONFINITE):
+            check(code)
+        return np.ascontiguousarray(d.transpose(0, 2, 1)), st, code
 
     LOSSES = {"g": 0, "tmax": 1}
 

@@ -230,6 +230,8 @@ void rg_launch_adj_scale(rg_plan *P, int nb, double2 *gbar, const double2 *cbar)
 void rg_launch_adj_grad(rg_plan *P, int nb, const double2 *cbar, int ixd, double *grad);
 // rg_refl.hip
 void rg_launch_refl_trans_out(rg_plan *P, int nb, const double2 *g11, const double2 *gs1, double2 *rmat, double2 *tmat);
+void rg_launch_deposition_out(rg_plan *P, int nb, const double2 *g, double2 *dep);
+void rg_launch_add_c128(rg_plan *P, double2 *a, const double2 *b, size_t count);
 // rg_eig.hip
 void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status);
 void rg_launch_tdt_eig_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status);

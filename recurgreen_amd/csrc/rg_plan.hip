@@ -476,6 +476,60 @@ double2 *run_sweep_refl(rg_plan *P, int nb, int64_t seed0, const double *d_upot,
     return P->d_gs1[cur ^ 1];
 }
 
+// compute_deposition (src/waveguide.f90:712-750) for ipos = 1, by recursion instead of the reference's
+// block-tridiagonal solve: with the right part (slices fpos+1..nx and the right lead) folded into the
+// right-connected Green function h = (A_x - V_x - B h' B)^-1 (a sweep from slice nx down, the same
+// kernels), G(fpos; fpos) = (A_f - V_f - B (g + h) B)^-1 with g = Gss_(fpos-1) of the forward sweep,
+// and G(fpos; 1) = -G(fpos; fpos) B Gs1_(fpos-1) (G(1; 1) itself at fpos = 1).  h is kept in the
+// G11 buffer of the reflection sweep.  Writes dep = 2i s G(fpos; 1) s (column-major nmu x nmu) to
+// dep_out.
+void run_sweep_deposition(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int fpos, double2 *dep_out)
+{
+    const int nx = P->nx;
+    const size_t mat = (size_t)nb * P->np * P->np;
+    auto slot_of = [nx](int ix) { return ix == 1 ? 0 : (ix == nx ? 2 : 1); };
+    auto project = [&](int ix) {
+        const int id = ix - P->nxfree;
+        const bool dis = id >= 1 && id <= P->nxdiso;
+        if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
+        return dis;
+    };
+    if (fpos < nx) {  // right part, from the right lead inwards
+        for (int ix = nx; ix > fpos; ix--) {
+            const bool dis = project(ix);
+            invert_f64(P, nb, slot_of(ix), ix == nx, dis);
+        }
+        (void)hipMemcpyAsync(P->d_g11, P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+    }
+    int cur = 0;
+    for (int ix = 1; ix < fpos; ix++) {  // left part, as the transmission sweep
+        const bool dis = project(ix);
+        invert_f64(P, nb, slot_of(ix), ix == 1, dis);
+        if (ix == 1) {
+            (void)hipMemcpyAsync(P->d_gs1[0], P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+        } else {
+            rg_launch_gemm_f64(P, nb, P->d_gss, P->d_gs1[cur], P->d_gs1[cur ^ 1]);
+            cur ^= 1;
+        }
+    }
+    // slice fpos coupled to both parts: the inversion subtracts B Gss B with Gss = g + h (g alone at
+    // fpos = nx, h alone at fpos = 1; nx >= 2, so one of them is always present)
+    if (fpos < nx) {
+        if (fpos == 1)
+            (void)hipMemcpyAsync(P->d_gss, P->d_g11, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+        else
+            rg_launch_add_c128(P, P->d_gss, P->d_g11, mat);
+    }
+    const bool dis = project(fpos);
+    invert_f64(P, nb, slot_of(fpos), 0, dis);
+    const double2 *gf1 = P->d_gss;
+    if (fpos > 1) {
+        rg_launch_gemm_f64(P, nb, P->d_gss, P->d_gs1[cur], P->d_gs1[cur ^ 1]);
+        gf1 = P->d_gs1[cur ^ 1];
+    }
+    rg_launch_deposition_out(P, nb, gf1, dep_out);
+}
+
 // One complex128 slice step: Gss_x in place in P->d_gss, and s1_out = -Gss_x B s1_in (a copy of
 // Gss_1 at ix == 1).  d_upot: explicit potentials of the pass (nullptr: no disorder).
 void fwd_step_f64(rg_plan *P, int nb, int ix, const double *d_upot, const double2 *s1_in, double2 *s1_out)
@@ -1109,6 +1163,50 @@ rg_status rg_refl_trans_potential(rg_plan *P, const double *upot, int32_t nreal,
 {
     if (!upot && nreal > 0) return RG_EINVAL;
     return refl_trans_pass(P, 0, upot, nreal, rmat, tmat, status);
+}
+
+static rg_status deposition_pass(rg_plan *P, int64_t seed0, const double *upot, int32_t nreal, int32_t fpos,
+                                 int32_t ipos, double *depmat, int32_t *status)
+{
+    rg_status st = check_common(P, nreal, RG_FP64);
+    if (st) return st;
+    if ((!depmat && nreal > 0) || fpos < 1 || fpos > P->nx || ipos < 1 || ipos > P->nx) return RG_EINVAL;
+    if (ipos != 1) return RG_ENOTSUP;
+    DeviceGuard g(P->device);
+    const size_t per = (size_t)P->nxdiso * P->ny, nn2 = (size_t)P->nmu * P->nmu;
+    const int cap = upot ? std::min(P->upot_cap, P->refl_cap) : P->refl_cap;
+    double2 *d_dep = P->d_tmat;  // batch_max x np^2 complex >= nb x nmu^2
+    int32_t worst = 0;
+    for (int32_t off = 0; off < nreal; off += cap) {
+        const int nb = std::min(cap, nreal - off);
+        HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
+        const double *dup = nullptr;
+        if (upot) {
+            for (size_t e = 0; e < per * nb; e++)
+                if (!std::isfinite(upot[per * off + e])) return RG_ENONFINITE;
+            HIPCHK(hipMemcpyAsync(P->d_upot, upot + per * off, sizeof(double) * per * nb, hipMemcpyHostToDevice, P->stream));
+            dup = P->d_upot;
+        }
+        run_sweep_deposition(P, nb, seed0 + off, dup, fpos, d_dep);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(depmat + 2 * nn2 * off, d_dep, sizeof(double2) * nn2 * nb, hipMemcpyDeviceToHost, P->stream));
+        if ((st = harvest_status(P, nb, status ? status + off : nullptr, &worst))) return st;
+    }
+    if (P->profiling) harvest_timers(P);
+    return worst ? (rg_status)worst : RG_OK;
+}
+
+rg_status rg_deposition(rg_plan *P, int64_t seed0, int32_t nseed, int32_t fpos, int32_t ipos, double *depmat,
+                        int32_t *status)
+{
+    return deposition_pass(P, seed0, nullptr, nseed, fpos, ipos, depmat, status);
+}
+
+rg_status rg_deposition_potential(rg_plan *P, const double *upot, int32_t nreal, int32_t fpos, int32_t ipos,
+                                  double *depmat, int32_t *status)
+{
+    if (!upot && nreal > 0) return RG_EINVAL;
+    return deposition_pass(P, 0, upot, nreal, fpos, ipos, depmat, status);
 }
 
 rg_status rg_adjoint_reserve(rg_plan *P, int32_t nreal_max)

@@ -145,12 +145,11 @@ contains
         call wguide%del()
     end subroutine
 
-    !! task_tpattern (src/main.f90:735-856) for probatype 'incident' and 'total': the seed loop of
-    !! compute_refl_trans_v1 (:790-835) runs as batched complex128 device passes of `batch`
-    !! realizations; the per-channel probabilities are summed pass by pass and averaged (:838-840).
-    !! probatype 'deposition' needs the interior Green function of compute_deposition, which the
-    !! transmission sweep does not carry, and stops with an error.  The &gpu group is read as in
-    !! task_tdistrib; its precision and bintype do not apply (reflection needs complex128).
+    !! task_tpattern (src/main.f90:735-856): the seed loop of compute_refl_trans_v1 (probatype
+    !! 'incident', 'total') or compute_deposition (probatype 'deposition', from the first slice to the
+    !! middle one, :764-765) runs as batched complex128 device passes of `batch` realizations; the
+    !! per-channel probabilities are summed pass by pass and averaged (:838-840).  The &gpu group is
+    !! read as in task_tdistrib; its precision and bintype do not apply (these matrices are complex128).
     subroutine task_tpattern(funit)
         integer, intent(in) :: funit
         type(waveguide_gpu_t) :: wguide
@@ -160,6 +159,7 @@ contains
         integer :: nmu, imu, off, nb, k, bsz
         integer(8) :: c0, c1, crate
         complex(wp), allocatable :: rmat(:, :, :), tmat(:, :, :)
+        integer :: ipos, fpos
         real(wp), allocatable :: tprob(:), rprob(:)
         namelist /waveguide/ boundtype, wol, exfac, dscat, dabso, aratio, nxdiso, nxfree, napera, naperb
         namelist /tpattern/ nseed, imode, nthreads, probatype
@@ -183,9 +183,6 @@ contains
             write (error_unit, '(a)') "[ERROR] Invalid probatype, received '" // trim(probatype) // &
                 "', expected incident/total/deposition, aborting..."
             stop errmsg_invalid_arg
-        else if (probatype == "deposition") then
-            write (error_unit, '(a)') "[ERROR] probatype='deposition' is not part of the GPU sweep, aborting..."
-            stop errmsg_invalid_arg
         end if
 
         call wguide%set_options(ny, dos_model, RG_FP64)
@@ -197,7 +194,9 @@ contains
         else
             imu = imode
         end if
-        if (probatype == "incident" .and. (imu < 1 .or. imu > nmu)) then
+        ipos = 1                        !! deposition: from the first slice (src/main.f90:764)
+        fpos = nxfree + nxdiso/2 + 1    !! to the middle of the medium (:765)
+        if (probatype /= "total" .and. (imu < 1 .or. imu > nmu)) then
             write (error_unit, '(2(a,i0),a)') "[ERROR] Invalid incident mode imode=", imode, ", expected in [1, ", nmu, &
                 "], aborting..."
             stop errmsg_invalid_arg
@@ -213,6 +212,13 @@ contains
         call system_clock(c0, crate)
         do off = 0, nseed - 1, bsz
             nb = min(bsz, nseed - off)
+            if (probatype == "deposition") then
+                call wguide%compute_deposition_batch(1 + off, nb, fpos, ipos, tmat(:, :, 1:nb))
+                do k = 1, nb
+                    tprob = tprob + abs(tmat(:, imu, k))**2   !! rprob stays zero (:811)
+                end do
+                cycle
+            end if
             call wguide%compute_refl_trans_batch(1 + off, nb, rmat(:, :, 1:nb), tmat(:, :, 1:nb))
             do k = 1, nb
                 if (probatype == "incident") then

@@ -59,6 +59,14 @@ module recurgreen_hip
             complex(c_double_complex), intent(out) :: rmat(*), tmat(*)
             integer(c_int32_t), intent(out) :: status(*)
         end function
+        integer(c_int) function rg_deposition(plan, seed0, nseed, fpos, ipos, depmat, status) bind(C, name="rg_deposition")
+            import :: c_int, c_int32_t, c_int64_t, c_double_complex, c_ptr
+            type(c_ptr), value :: plan
+            integer(c_int64_t), value :: seed0
+            integer(c_int32_t), value :: nseed, fpos, ipos
+            complex(c_double_complex), intent(out) :: depmat(*)
+            integer(c_int32_t), intent(out) :: status(*)
+        end function
         integer(c_int) function rg_histogram(plan, samples, nsamp, on_device, bins, nbins, counts, missing, dropped) &
                 bind(C, name="rg_histogram")
             import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
@@ -95,6 +103,7 @@ module recurgreen_hip
         procedure :: compute_tdistrib_batch => compute_tdistrib_batch_waveguide_gpu
         procedure :: compute_refl_trans => compute_refl_trans_waveguide_gpu
         procedure :: compute_refl_trans_batch => compute_refl_trans_batch_waveguide_gpu
+        procedure :: compute_deposition_batch => compute_deposition_batch_waveguide_gpu
         procedure :: histogram => histogram_waveguide_gpu
         procedure :: print_parameters => print_parameters_waveguide_gpu
         procedure :: del => del_waveguide_gpu
@@ -252,6 +261,29 @@ contains
         call check_refl_trans_shape(self, size(rmat, 1), size(rmat, 2), size(rmat, 3), shape(tmat), nseed)
         allocate(st(nseed))
         call check_status(rg_refl_trans(self%plan, int(seed0, c_int64_t), int(nseed, c_int32_t), rmat, tmat, st))
+    end subroutine
+
+    !! compute_deposition (src/waveguide.f90:712-750) for realizations seed0 .. seed0+nseed-1:
+    !! depmat(:, :, k) = 2i s G(fpos; ipos) s of realization seed0+k-1, with the position checks of
+    !! :719-727; ipos = 1 (the input lead's side) only.
+    subroutine compute_deposition_batch_waveguide_gpu(self, seed0, nseed, fpos, ipos, depmat)
+        class(waveguide_gpu_t), intent(inout) :: self
+        integer, intent(in) :: seed0, nseed, fpos, ipos
+        complex(wp), intent(inout), contiguous :: depmat(:, :, :)
+        integer(c_int32_t), allocatable :: st(:)
+        if (fpos < 1 .or. self%nx < fpos) then
+            write (error_unit, '(2(a,i0),a)') "[ERROR] Invalid final position index, received fpos=", fpos, &
+                ", expected in 1..", self%nx, ", aborting..."
+            stop errmsg_invalid_arg
+        else if (ipos < 1 .or. self%nx < ipos) then
+            write (error_unit, '(2(a,i0),a)') "[ERROR] Invalid initial position index, received ipos=", ipos, &
+                ", expected in 1..", self%nx, ", aborting..."
+            stop errmsg_invalid_arg
+        end if
+        call check_refl_trans_shape(self, size(depmat, 1), size(depmat, 2), size(depmat, 3), shape(depmat), nseed)
+        allocate(st(nseed))
+        call check_status(rg_deposition(self%plan, int(seed0, c_int64_t), int(nseed, c_int32_t), &
+            int(fpos, c_int32_t), int(ipos, c_int32_t), depmat, st))
     end subroutine
 
     subroutine check_refl_trans_shape(self, n1, n2, n3, tshape, nseed)

@@ -5,9 +5,9 @@ Mirrors task_tpattern and save_tpattern of src/main.f90:735-901 for probatype 'i
 sum_b |t_ba|^2 and sum_b |r_ba|^2), averaged over realizations 1..nseed.  The reference computes
 one compute_refl_trans_v1 per seed inside an OpenMP loop (:790-835); here the seeds run as
 batched complex128 device passes of the plan's batch size, and only the per-channel
-probabilities come back to the host.  probatype 'deposition' needs the Green function between
-an interior slice and the input lead (compute_deposition, src/waveguide.f90:528-606), which the
-transmission sweep does not carry: it is refused with RG_ENOTSUP.
+probabilities come back to the host.  probatype 'deposition' (:808-812) takes |dep(:, imu)|^2 of
+the deposition matrix between the input lead (ipos = 1) and the middle slice fpos = nxfree +
+nxdiso/2 + 1 (:765), from rg_deposition; its reflection column is zero.
 """
 from __future__ import annotations
 
@@ -41,9 +41,6 @@ def check_arguments(nseed: int, probatype: str) -> None:
     if probatype not in PROBATYPES:
         raise RecurGreenError(binding.RG_EINVAL, f"Invalid probatype, received '{probatype}', "
                               "expected incident/total/deposition, aborting...")
-    if probatype == "deposition":
-        raise RecurGreenError(binding.RG_ENOTSUP, "probatype='deposition' needs the interior Green function "
-                              "(compute_deposition), which the transmission sweep does not compute")
 
 
 def channel_probabilities(rmat: np.ndarray, tmat: np.ndarray, probatype: str, imu: int):
@@ -54,6 +51,12 @@ def channel_probabilities(rmat: np.ndarray, tmat: np.ndarray, probatype: str, im
     return np.sum(np.abs(tmat) ** 2, axis=1), np.sum(np.abs(rmat) ** 2, axis=1)
 
 
+def deposition_positions(params: dict) -> tuple[int, int]:
+    """(fpos, ipos) of the deposition pattern (src/main.f90:764-765): the middle of the medium, from
+    the first slice."""
+    return int(params.get("nxfree", 0)) + int(params["nxdiso"]) // 2 + 1, 1
+
+
 def compute_tpattern(wguide, nseed: int, imode: int, probatype: str = "incident", seed0: int = 1):
     """(tprob, rprob, imu): probabilities per outgoing channel averaged over realizations
     seed0 .. seed0+nseed-1 (src/main.f90:838-840).  `wguide` is a recurgreen_amd.Waveguide."""
@@ -61,13 +64,20 @@ def compute_tpattern(wguide, nseed: int, imode: int, probatype: str = "incident"
     plan = wguide.plan
     nmu = plan.nmu
     imu = mode_index(wguide.params["boundtype"], nmu, imode)
-    if probatype == "incident" and not 1 <= imu <= nmu:
+    if probatype in ("incident", "deposition") and not 1 <= imu <= nmu:
         raise RecurGreenError(binding.RG_EINVAL, f"Invalid incident mode imode={imode}, expected in [1, {nmu}]")
     tsum = np.zeros(nmu)
     rsum = np.zeros(nmu)
     chunk = max(1, plan.batch_max)
+    fpos, ipos = deposition_positions(wguide.params)
     for off in range(0, nseed, chunk):
         nb = min(chunk, nseed - off)
+        if probatype == "deposition":
+            d, st, code = plan.deposition(fpos, ipos, None, seed0=seed0 + off, nseed=nb)
+            if code:
+                raise RecurGreenError(code)
+            tsum += (np.abs(d[:, :, imu - 1]) ** 2).sum(axis=0)
+            continue
         r, t, st, code = plan.refl_trans(None, seed0=seed0 + off, nseed=nb)
         if code:
             raise RecurGreenError(code)

@@ -96,6 +96,22 @@ class Waveguide:
             raise RecurGreenError(code)
         return r[0], t[0]
 
+    def compute_deposition(self, iseed: int, fpos: int, ipos: int = 1) -> np.ndarray:
+        """depmat of realization `iseed` as compute_deposition (src/waveguide.f90:712-750) returns it:
+        nmu x nmu complex128, 2i s G(fpos; ipos) s with s = sqrt(Re mu); same position checks
+        (:719-727).  ipos = 1 only (the input lead's side)."""
+        nx = self.plan.nx
+        if fpos < 1 or nx < fpos:
+            raise RecurGreenError(binding.RG_EINVAL,
+                                  f"Invalid final position index, received fpos={fpos}, expected in 1..{nx}, aborting...")
+        if ipos < 1 or nx < ipos:
+            raise RecurGreenError(binding.RG_EINVAL,
+                                  f"Invalid initial position index, received ipos={ipos}, expected in 1..{nx}, aborting...")
+        d, st, code = self.plan.deposition(fpos, ipos, None, seed0=iseed, nseed=1)
+        if code:
+            raise RecurGreenError(code)
+        return d[0]
+
     def generate_potential(self, seed0: int, nreal: int = 1) -> np.ndarray:
         return self.plan.generate_potential(seed0, nreal)
 

@@ -38,7 +38,12 @@ def oracle_tpattern(p, nseed, imode, probatype):
     imu = tpattern.mode_index(p["boundtype"], ow.nmu, imode)
     tsum = np.zeros(ow.nmu)
     rsum = np.zeros(ow.nmu)
+    fpos, ipos = tpattern.deposition_positions(p)
     for s in range(1, nseed + 1):
+        if probatype == "deposition":
+            d = ow.deposition(ow.potential(s), fpos, ipos)
+            tsum += np.abs(d[:, imu - 1]) ** 2
+            continue
         r, t = ow.refl_trans(ow.potential(s), version=1)
         tp, rp = tpattern.channel_probabilities(r[None], t[None], probatype, imu)
         tsum += tp[0]
@@ -46,7 +51,7 @@ def oracle_tpattern(p, nseed, imode, probatype):
     return tsum / nseed, rsum / nseed
 
 
-@pytest.mark.parametrize("probatype", ["incident", "total"])
+@pytest.mark.parametrize("probatype", ["incident", "total", "deposition"])
 @pytest.mark.parametrize("name", list(CASES))
 def test_tpattern_matches_oracle(name, probatype):
     p = CASES[name]
@@ -109,7 +114,7 @@ FORTRAN_NML = """&settings
 """
 
 
-@pytest.mark.parametrize("probatype", ["incident", "total"])
+@pytest.mark.parametrize("probatype", ["incident", "total", "deposition"])
 def test_fortran_tpattern_matches_python_path(tmp_path, probatype):
     say(f"fortran host tpattern {probatype}")
     assert os.path.exists(rgbuild.FEXE), "Fortran host not built"

@@ -280,6 +280,37 @@ def test_green_l0_vs_dense_block_tridiagonal_solve():
     np.testing.assert_allclose(t, ref[:w.imax, :w.jmax], rtol=1e-10, atol=1e-13)
 
 
+@pytest.mark.parametrize("boundtype", ["dirichlet", "periodic"])
+def test_deposition_vs_dense_inverse_and_reciprocity(boundtype):
+    """compute_deposition (src/waveguide.f90:712-750): the (fpos, ipos) block of inv(H) with the
+    sqrt(Re mu) factors, against a dense inverse of H = tridiag(B, A_x, B); fpos = nx gives t and
+    fpos = ipos = 1 gives r + I (src/waveguide.f90:508-512); reciprocity of the complex-symmetric
+    H: G(f; i) = G(i; f)^T for real (Dirichlet) modes, with the channel order reversed (k -> -k)
+    for plane waves."""
+    w = O.Waveguide(boundtype, nxdiso=12, nxfree=2, wol=4.3, dscat=1.5, dabso=0.3)
+    u = w.potential(5)
+    n, nx = w.nmu, w.nx
+    H = np.zeros((n * nx, n * nx), complex)
+    for ix in range(1, nx + 1):
+        s = slice((ix - 1) * n, ix * n)
+        H[s, s] = w.slice_matrix(ix, u)
+        if ix < nx:
+            H[s, ix * n:(ix + 1) * n] = np.diag(w.b)
+            H[ix * n:(ix + 1) * n, s] = np.diag(w.b)
+    G = np.linalg.inv(H)
+    sre = np.sqrt(w.mu.real)
+    for f, i in [(1, 1), (5, 1), (nx, 1), (3, 9), (nx, nx), (8, 8)]:
+        ref = 2j * sre[:, None] * G[(f - 1) * n:f * n, (i - 1) * n:i * n] * sre[None, :]
+        np.testing.assert_allclose(w.deposition(u, f, i), ref, rtol=1e-10, atol=1e-13)
+    r, t = w.refl_trans(u, 1)
+    np.testing.assert_allclose(w.deposition(u, nx, 1), t, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(w.deposition(u, 1, 1), r + np.eye(n), rtol=1e-12, atol=1e-14)
+    P = np.eye(n) if boundtype == "dirichlet" else np.eye(n)[::-1]
+    np.testing.assert_allclose(w.deposition(u, 10, 4), P @ w.deposition(u, 4, 10).T @ P, rtol=1e-10, atol=1e-13)
+    with pytest.raises(O.OracleError):
+        w.deposition(u, nx + 1, 1)
+
+
 def test_mirror_invariance_and_sum_rule():
     w = O.Waveguide("dirichlet", nxdiso=80, wol=6.3, dscat=2.0, dabso=0.3, ny=31)
     u = w.potential(21)

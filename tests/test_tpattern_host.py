@@ -49,9 +49,8 @@ def test_argument_errors():
         tpattern.check_arguments(0, "incident")
     with pytest.raises(RecurGreenError, match="Invalid probatype"):
         tpattern.check_arguments(3, "reflected")
-    with pytest.raises(RecurGreenError) as e:
-        tpattern.check_arguments(3, "deposition")
-    assert e.value.code == binding.RG_ENOTSUP
+    tpattern.check_arguments(3, "deposition")
+    assert tpattern.deposition_positions(dict(nxdiso=101, nxfree=7)) == (7 + 50 + 1, 1)  # src/main.f90:764-765
 
 
 class _StubPlan:
@@ -113,7 +112,6 @@ def _run(tmp_path, text):
 @pytest.mark.parametrize("nseed,probatype,msg", [
     (0, "incident", "Invalid number of disorder realizations (received 0)"),
     (4, "reflected", "Invalid probatype, received 'reflected'"),
-    (4, "deposition", "probatype='deposition' is not part of the GPU sweep"),
 ])
 def test_fortran_tpattern_argument_checks(tmp_path, nseed, probatype, msg):
     """The checks of src/main.f90:753-760 stop before any device call."""
