@@ -122,6 +122,23 @@ rg_status rg_deposition(rg_plan *plan, int64_t seed0, int32_t nseed, int32_t fpo
 /* Same for explicit host potentials upot (nxdiso*ny*nreal doubles). */
 rg_status rg_deposition_potential(rg_plan *plan, const double *upot, int32_t nreal, int32_t fpos, int32_t ipos,
                                   double *depmat, int32_t *status);
+/* Transmission eigenstates inside the medium, the data of compute_tprofile and
+ * compute_tprofile_group (src/waveguide.f90:807-997) for seeds seed0 .. seed0+nseed-1, complex128.
+ * t = 2i s G(L;0) s, s = sqrt(Re mu): the full nmu x nmu matrix (compute_tprofile, filtered = 0) or
+ * rows imin..imax and columns jmin..jmax (compute_tprofile_group, filtered = 1; RG_EINVAL if it has
+ * more columns than rows, as diagonalize_ahca's size check).  With nv its number of columns, tvals
+ * (host, nseed*nv) receives the eigenvalues T_v of t^H t, descending; intensity (host, nseed*nx*nv,
+ * index (r*nx + x)*nv + v) receives |G(x;1) s U(:,v)|^2 summed over the modes, U the eigenvectors
+ * (diagonalize_ahca, src/base_utils.f90:394-443).  The reference's profiles are sums of these over
+ * the eigenvalues in a window (:858-869, :960-985).  rg_tprofile_reserve allocates the workspace
+ * (the Green functions of every slice for nreal_max realizations per device pass; the only
+ * allocating call of this group): rg_tprofile returns RG_EINVAL before it, RG_ENOTSUP above nmu = 256. */
+rg_status rg_tprofile_reserve(rg_plan *plan, int32_t nreal_max);
+rg_status rg_tprofile(rg_plan *plan, int64_t seed0, int32_t nseed, int32_t filtered, double *tvals, double *intensity,
+                      int32_t *status);
+/* Same for explicit host potentials upot (nxdiso*ny*nreal doubles). */
+rg_status rg_tprofile_potential(rg_plan *plan, const double *upot, int32_t nreal, int32_t filtered, double *tvals,
+                                double *intensity, int32_t *status);
 /* Adjoint sweep (no counterpart in the reference: the reverse-mode derivative of the sweep of
  * src/waveguide.f90:427-445).  loss_kind 0: L = Tr(t^H t) = sum of the transmission eigenvalues;
  * loss_kind 1: L = T_max, the largest transmission eigenvalue (t over the filtered ranges, complex

@@ -68,6 +68,8 @@ def lib():
         for name in ("rgo_refl_trans_v1", "rgo_refl_trans_v2"):
             getattr(L, name).restype = C.c_int
             getattr(L, name).argtypes = [vp, dp, vp, vp]
+        L.rgo_green_column.restype = C.c_int
+        L.rgo_green_column.argtypes = [vp, dp, C.c_int, vp]
         L.rgo_deposition.restype = C.c_int
         L.rgo_deposition.argtypes = [vp, dp, C.c_int, C.c_int, vp]
         L.rgo_solve_block_tridiag.restype = C.c_int
@@ -228,6 +230,15 @@ class Waveguide:
         if st:
             raise OracleError(f"deposition status {st}")
         return d
+
+    def green_column(self, upot, ipos: int = 1) -> np.ndarray:
+        """G(x; ipos) for every slice x as (nx, nmu, nmu) (the block-tridiagonal solve)."""
+        u = self._upot(upot)
+        g = np.zeros((self.nx, self.nmu, self.nmu), complex)
+        st = lib().rgo_green_column(self._h, _dp(u) if u is not None else None, int(ipos), _vp(g))
+        if st:
+            raise OracleError(f"green_column status {st}")
+        return g
 
     def adjoint_grad(self, upot, loss: str = "g"):
         """FP64 adjoint: (loss value, d loss / d upot) with upot in U*dx/k units."""

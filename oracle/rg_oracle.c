@@ -14,6 +14,7 @@
  *   - r/t by block-tridiag solve  src/waveguide.f90:526-553,
  *                                 src/base_utils.f90:296-335 (rgo_refl_trans_v2)
  *   - deposition matrix           src/waveguide.f90:712-750  (rgo_deposition)
+ *   - Green function column block src/waveguide.f90:842-848  (rgo_green_column)
  *   - zgetrf+zgetri inverse       src/base_utils.f90:235-274 (rgo_inverse)
  *   - zgesvd singular values      src/base_utils.f90:341-380 (rgo_singular_values,
  *                                 one-sided Jacobi instead of LAPACK)
@@ -697,6 +698,25 @@ int rgo_deposition(const rgo_wg *w, const double *upot, int fpos, int ipos, zc *
             }
         }
     free(amat); free(ind); free(green); free(work);
+    return st;
+}
+
+/* Column block ipos of H^-1 for every slice, green[x] = G(x; ipos) (nx blocks of nmu x nmu, row-major),
+ * by the block-tridiagonal solve of compute_tprofile (src/waveguide.f90:842-848). */
+int rgo_green_column(const rgo_wg *w, const double *upot, int ipos, zc *green)
+{
+    int n = w->nmu, nx = w->nx;
+    if (ipos < 1 || ipos > nx) return RGO_EINVAL;
+    size_t nn = (size_t)n * n;
+    zc *amat = (zc *)malloc(sizeof(zc) * nn * nx), *ind = (zc *)calloc(nn * nx, sizeof(zc));
+    zc *work = (zc *)malloc(sizeof(zc) * nn);
+    int st = (amat && ind && work) ? RGO_OK : RGO_ENOMEM;
+    if (!st) {
+        for (int ix = 1; ix <= nx; ix++) rgo_slice_matrix(w, ix, upot, amat + nn * (ix - 1), work);
+        for (int m = 0; m < n; m++) ind[nn * (ipos - 1) + (size_t)m * n + m] = 1.0;
+        st = rgo_solve_block_tridiag(nx, n, n, amat, w->bmat, ind, green);
+    }
+    free(amat); free(ind); free(work);
     return st;
 }
 

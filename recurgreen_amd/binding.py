@@ -21,7 +21,7 @@ BOUNDTYPES = {"dirichlet": 0, "periodic": 1}
 # Every symbol include/recurgreen_hip.h declares.
 EXPORTS = (
     "rg_plan_create", "rg_plan_destroy", "rg_plan_dims", "rg_tdistrib", "rg_tdistrib_device",
-    "rg_tdistrib_potential", "rg_transmission_potential", "rg_refl_trans", "rg_refl_trans_potential", "rg_deposition", "rg_deposition_potential",
+    "rg_tdistrib_potential", "rg_transmission_potential", "rg_refl_trans", "rg_refl_trans_potential", "rg_deposition", "rg_deposition_potential", "rg_tprofile_reserve", "rg_tprofile", "rg_tprofile_potential",
     "rg_tdistrib_uneven",
     "rg_adjoint_reserve", "rg_adjoint_potential", "rg_adjoint_potential_device",
     "rg_generate_potential", "rg_histogram",
@@ -85,6 +85,9 @@ def lib():
         "rg_refl_trans_potential": [vp, vp, i32, vp, vp, vp],
         "rg_deposition": [vp, i64, i32, i32, i32, vp, vp],
         "rg_deposition_potential": [vp, vp, i32, i32, i32, vp, vp],
+        "rg_tprofile_reserve": [vp, i32],
+        "rg_tprofile": [vp, i64, i32, i32, vp, vp, vp],
+        "rg_tprofile_potential": [vp, vp, i32, i32, vp, vp, vp],
         "rg_generate_potential": [vp, i64, i32, vp],
         "rg_histogram": [vp, vp, i64, i32, dp, i32, i64p, i64p, i64p],
         "rg_plan_stream": [vp, C.POINTER(vp)],
@@ -283,6 +286,35 @@ class Plan:
         if code not in (RG_OK, RG_ESINGULAR, RG_ENONFINITE):
             check(code)
         return np.ascontiguousarray(d.transpose(0, 2, 1)), st, code
+
+    def tprofile_reserve(self, nreal_max: int) -> None:
+        check(lib().rg_tprofile_reserve(self._h, int(nreal_max)))
+
+    def tprofile(self, upot: np.ndarray | None = None, seed0: int = 1, nseed: int = 1, filtered: bool = False,
+                 reserve: int | None = None):
+        """(tvals (nreal, nv) descending, intensity (nreal, nx, nv), status, code): the eigenvalues of
+        t^H t and |G(x;1) s U(:,v)|^2 per slice and eigenvector (compute_tprofile,
+        src/waveguide.f90:807-875, before the window sum; filtered: t over the apertures as
+        compute_tprofile_group, :951-954).  Reserves the workspace for min(nreal, reserve or
+        batch_max) realizations per pass."""
+        if upot is not None:
+            u = np.asfortranarray(upot, dtype=np.float64)
+            nreal = 1 if u.ndim == 2 else u.shape[2]
+        else:
+            u, nreal = None, int(nseed)
+        self.tprofile_reserve(max(1, min(nreal, reserve or self.batch_max)))
+        nv = (self.jmax - self.jmin + 1) if filtered else self.nmu
+        tv = np.zeros((nreal, nv))
+        it = np.zeros((nreal, self.nx, nv))
+        st = np.zeros(nreal, np.int32)
+        f = int(bool(filtered))
+        if u is None:
+            code = lib().rg_tprofile(self._h, int(seed0), nreal, f, ptr(tv), ptr(it), ptr(st))
+        else:
+            code = lib().rg_tprofile_potential(self._h, ptr(u), nreal, f, ptr(tv), ptr(it), ptr(st))
+        if code not in (RG_OK, RG_ESINGULAR, RG_ENOCONV, RG_ENONFINITE):
+            check(code)
+        return tv, it, st, code
 
     LOSSES = {"g": 0, "tmax": 1}
 

@@ -145,6 +145,13 @@ struct rg_plan {
     double2 *d_ck = nullptr, *d_seg = nullptr, *d_adj = nullptr;
     double *d_loss = nullptr, *d_grad = nullptr;
     double2 *d_bconj = nullptr;  // conj(b) (np)
+    // tprofile workspace (rg_tprofile_reserve), tp_cap realizations per pass: Gss and Gs1 of every
+    // slice (nx slabs of tp_cap np^2 each), the SVD work X and V, W = s U, two products Y, the
+    // eigenvalues (tp_cap x nmu) and the intensities (tp_cap x nx x nmu)
+    int tp_cap = 0;
+    double2 *d_tp_g = nullptr, *d_tp_z = nullptr, *d_tp_x = nullptr, *d_tp_v = nullptr, *d_tp_w = nullptr;
+    double2 *d_tp_y[2] = {nullptr, nullptr};
+    double *d_tp_t = nullptr, *d_tp_i = nullptr;
     double *d_cq = nullptr, *d_teig = nullptr, *d_upot = nullptr;
     double *d_us = nullptr;  // disorder of the current slice for the batch (batch_max x ny)
     int32_t *d_status = nullptr;
@@ -231,6 +238,11 @@ void rg_launch_adj_grad(rg_plan *P, int nb, const double2 *cbar, int ixd, double
 // rg_refl.hip
 void rg_launch_refl_trans_out(rg_plan *P, int nb, const double2 *g11, const double2 *gs1, double2 *rmat, double2 *tmat);
 void rg_launch_deposition_out(rg_plan *P, int nb, const double2 *g, double2 *dep);
+// rg_tprofile.hip
+bool rg_tprofile_supported(int n);
+void rg_launch_tprofile_svd(rg_plan *P, int nb, const double2 *gs1, int i0, int m, int j0, int nn, double2 *xwork,
+                            double2 *vwork, double2 *w, double *tvals);
+void rg_launch_colnorms(rg_plan *P, int nb, const double2 *y, int nv, double *out, size_t ostride);
 void rg_launch_add_c128(rg_plan *P, double2 *a, const double2 *b, size_t count);
 // rg_eig.hip
 void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status);

@@ -52,7 +52,8 @@ void free_plan(rg_plan *P)
                     P->d_tmat, P->d_cq, P->d_teig, P->d_upot, P->d_status, P->d_hist, P->d_bins,
                     P->d_gss32, P->d_gs1_32[0], P->d_gs1_32[1], P->d_g11, P->d_g1s[0], P->d_g1s[1],
                     P->d_sqrtremu, P->d_redo, P->d_us, P->d_ck, P->d_seg, P->d_adj, P->d_loss, P->d_grad, P->d_bconj,
-                    P->d_schur, P->d_schur_zero, P->d_schur_i};
+                    P->d_schur, P->d_schur_zero, P->d_schur_i, P->d_tp_g, P->d_tp_z, P->d_tp_x, P->d_tp_v,
+                    P->d_tp_w, P->d_tp_y[0], P->d_tp_y[1], P->d_tp_t, P->d_tp_i};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_graphs(P);
@@ -528,6 +529,45 @@ void run_sweep_deposition(rg_plan *P, int nb, int64_t seed0, const double *d_upo
         gf1 = P->d_gs1[cur ^ 1];
     }
     rg_launch_deposition_out(P, nb, gf1, dep_out);
+}
+
+// Transmission eigenstates inside the medium (compute_tprofile, src/waveguide.f90:807-875), complex128:
+// the transmission sweep keeps Gss_x (g_x) and Gs1_x (z_x) of every slice; t = 2i s z_nx s is
+// decomposed as t^H t = U diag(T) U^H; with W = s U the column block Y_x = G(x;1) W follows from the
+// back substitution of the block-tridiagonal solve (src/base_utils.f90:330-333),
+// Y_nx = z_nx W, Y_x = z_x W - g_x B Y_(x+1), and the intensity of eigenvector v at slice x is
+// |Y_x(:, v)|^2.  filtered: t over the apertures (rows imin..imax, columns jmin..jmax) as in
+// compute_tprofile_group (:951-954), else the full nmu x nmu matrix (:851).  Writes T (descending,
+// nv per realization) to P->d_tp_t and the intensities (nb x nx x nv) to P->d_tp_i.
+void run_sweep_tprofile(rg_plan *P, int nb, int64_t seed0, const double *d_upot, bool filtered)
+{
+    const int nx = P->nx;
+    const int i0 = filtered ? P->imin - 1 : 0, m = filtered ? P->imax - P->imin + 1 : P->nmu;
+    const int j0 = filtered ? P->jmin - 1 : 0, n = filtered ? P->jmax - P->jmin + 1 : P->nmu;
+    const size_t mat = (size_t)nb * P->np * P->np, slab = (size_t)P->tp_cap * P->np * P->np;
+    const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
+    for (int ix = 1; ix <= nx; ix++) {
+        const int id = ix - P->nxfree;
+        const bool dis = id >= 1 && id <= P->nxdiso;
+        if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
+        invert_f64(P, nb, ix == 1 ? 0 : (ix == nx ? 2 : 1), ix == 1, dis);
+        double2 *g = P->d_tp_g + (ix - 1) * slab, *z = P->d_tp_z + (ix - 1) * slab;
+        (void)hipMemcpyAsync(g, P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+        if (ix == 1) (void)hipMemcpyAsync(z, P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+        else rg_launch_gemm_f64(P, nb, P->d_gss, z - slab, z);
+    }
+    rg_launch_tprofile_svd(P, nb, P->d_tp_z + (nx - 1) * slab, i0, m, j0, n, P->d_tp_x, P->d_tp_v, P->d_tp_w,
+                           P->d_tp_t);
+    double2 *ya = P->d_tp_y[0], *yb = P->d_tp_y[1];
+    const size_t ostride = (size_t)nx * n;
+    rg_zgemm(P, nb, one, P->d_tp_z + (nx - 1) * slab, 0, P->d_tp_w, 0, zero, ya, nullptr, nullptr, nullptr);
+    rg_launch_colnorms(P, nb, ya, n, P->d_tp_i + (size_t)(nx - 1) * n, ostride);
+    for (int k = nx - 2; k >= 0; k--) {
+        rg_zgemm(P, nb, one, P->d_tp_z + k * slab, 0, P->d_tp_w, 0, zero, yb, nullptr, nullptr, nullptr);
+        rg_zgemm(P, nb, mone, P->d_tp_g + k * slab, 0, ya, 0, one, yb, nullptr, P->d_b, nullptr);
+        rg_launch_colnorms(P, nb, yb, n, P->d_tp_i + (size_t)k * n, ostride);
+        std::swap(ya, yb);
+    }
 }
 
 // One complex128 slice step: Gss_x in place in P->d_gss, and s1_out = -Gss_x B s1_in (a copy of
@@ -1207,6 +1247,80 @@ rg_status rg_deposition_potential(rg_plan *P, const double *upot, int32_t nreal,
 {
     if (!upot && nreal > 0) return RG_EINVAL;
     return deposition_pass(P, 0, upot, nreal, fpos, ipos, depmat, status);
+}
+
+rg_status rg_tprofile_reserve(rg_plan *P, int32_t nreal_max)
+{
+    if (!P || nreal_max < 1) return RG_EINVAL;
+    if (!rg_tprofile_supported(P->nmu)) return RG_ENOTSUP;
+    DeviceGuard g(P->device);
+    if (P->tp_cap >= nreal_max) return RG_OK;
+    for (void **b : {(void **)&P->d_tp_g, (void **)&P->d_tp_z, (void **)&P->d_tp_x, (void **)&P->d_tp_v,
+                     (void **)&P->d_tp_w, (void **)&P->d_tp_y[0], (void **)&P->d_tp_y[1], (void **)&P->d_tp_t,
+                     (void **)&P->d_tp_i}) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
+    P->tp_cap = 0;
+    const size_t cmat = (size_t)nreal_max * P->np * P->np;
+    rg_status st;
+    if ((st = dalloc(&P->d_tp_g, (size_t)P->nx * cmat)) || (st = dalloc(&P->d_tp_z, (size_t)P->nx * cmat)) ||
+        (st = dalloc(&P->d_tp_x, cmat)) || (st = dalloc(&P->d_tp_v, cmat)) || (st = dalloc(&P->d_tp_w, cmat)) ||
+        (st = dalloc(&P->d_tp_y[0], cmat)) || (st = dalloc(&P->d_tp_y[1], cmat)) ||
+        (st = dalloc(&P->d_tp_t, (size_t)nreal_max * P->nmu)) ||
+        (st = dalloc(&P->d_tp_i, (size_t)nreal_max * P->nx * P->nmu)))
+        return st;
+    P->tp_cap = nreal_max;
+    return RG_OK;
+}
+
+static rg_status tprofile_pass(rg_plan *P, int64_t seed0, const double *upot, int32_t nreal, int32_t filtered,
+                               double *tvals, double *intensity, int32_t *status)
+{
+    rg_status st = check_common(P, nreal, RG_FP64);
+    if (st) return st;
+    if ((!tvals || !intensity) && nreal > 0) return RG_EINVAL;
+    if (P->tp_cap < 1) return RG_EINVAL;
+    // the group variant decomposes the aperture-filtered t; it needs no more columns than rows
+    // (diagonalize_ahca's size check, src/base_utils.f90:411-415)
+    if (filtered && P->jmax - P->jmin > P->imax - P->imin) return RG_EINVAL;
+    DeviceGuard g(P->device);
+    const size_t nvr = filtered ? (size_t)(P->jmax - P->jmin + 1) : (size_t)P->nmu;
+    const size_t per = (size_t)P->nxdiso * P->ny, n = nvr, nxn = (size_t)P->nx * nvr;
+    const int cap = upot ? std::min(P->upot_cap, P->tp_cap) : P->tp_cap;
+    int32_t worst = 0;
+    for (int32_t off = 0; off < nreal; off += cap) {
+        const int nb = std::min(cap, nreal - off);
+        HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
+        const double *dup = nullptr;
+        if (upot) {
+            for (size_t e = 0; e < per * nb; e++)
+                if (!std::isfinite(upot[per * off + e])) return RG_ENONFINITE;
+            HIPCHK(hipMemcpyAsync(P->d_upot, upot + per * off, sizeof(double) * per * nb, hipMemcpyHostToDevice, P->stream));
+            dup = P->d_upot;
+        }
+        run_sweep_tprofile(P, nb, seed0 + off, dup, filtered != 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(tvals + n * off, P->d_tp_t, sizeof(double) * n * nb, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(intensity + nxn * off, P->d_tp_i, sizeof(double) * nxn * nb, hipMemcpyDeviceToHost,
+                              P->stream));
+        if ((st = harvest_status(P, nb, status ? status + off : nullptr, &worst))) return st;
+    }
+    if (P->profiling) harvest_timers(P);
+    return worst ? (rg_status)worst : RG_OK;
+}
+
+rg_status rg_tprofile(rg_plan *P, int64_t seed0, int32_t nseed, int32_t filtered, double *tvals, double *intensity,
+                      int32_t *status)
+{
+    return tprofile_pass(P, seed0, nullptr, nseed, filtered, tvals, intensity, status);
+}
+
+rg_status rg_tprofile_potential(rg_plan *P, const double *upot, int32_t nreal, int32_t filtered, double *tvals,
+                                double *intensity, int32_t *status)
+{
+    if (!upot && nreal > 0) return RG_EINVAL;
+    return tprofile_pass(P, 0, upot, nreal, filtered, tvals, intensity, status);
 }
 
 rg_status rg_adjoint_reserve(rg_plan *P, int32_t nreal_max)

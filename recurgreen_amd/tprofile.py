@@ -1,0 +1,116 @@
+"""Transmission-eigenstate profiles inside the medium (tasks `tprofile` and `tprofile_group`).
+
+compute_tprofile (src/waveguide.f90:807-875) sums, over the transmission eigenvalues T_v in
+[tmin, tmax], the intensity |G(x;1) s U(:,v)|^2 of the eigenstate at every slice x and multiplies
+by 4/nt; compute_tprofile_group (:920-997) does the same for several windows |T - tm| <= dt of the
+aperture-filtered matrix and also returns tavg = sum(T)/neigv.  The device pass rg_tprofile
+returns, for a batch of realizations, the eigenvalues and every per-eigenvector intensity (one
+transmission sweep that keeps the Green functions of all slices, a Jacobi decomposition of t^H t,
+and the back substitution of the block-tridiagonal solve); the window sums below are the
+reference's steps 7 on those arrays.  The task drivers average the profiles of the realizations
+with nt > 0 (src/main.f90:466-483).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import binding
+from .binding import RecurGreenError
+
+
+def check_window(tmin: float, tmax: float) -> None:
+    """src/waveguide.f90:819-828."""
+    if tmin < 0.0 or tmin > 1.0:
+        raise RecurGreenError(binding.RG_EINVAL, f"Invalid Tmin, received Tmin={tmin:.3g}, expected in [0, 1], aborting...")
+    if tmax < 0.0 or tmax > 1.0:
+        raise RecurGreenError(binding.RG_EINVAL, f"Invalid Tmax, received Tmax={tmax:.3g}, expected in [0, 1], aborting...")
+    if tmin >= tmax:
+        raise RecurGreenError(binding.RG_EINVAL, "Invalid range (Tmin >= Tmax), aborting...")
+
+
+def check_group(tm, dt) -> None:
+    """src/waveguide.f90:938-943."""
+    tm, dt = np.asarray(tm, float), np.asarray(dt, float)
+    if tm.shape != dt.shape:
+        raise RecurGreenError(binding.RG_EINVAL, "tm and dt must have the same length")
+    if np.any(tm < 0.0) or np.any(tm > 1.0):
+        raise RecurGreenError(binding.RG_EINVAL, "Invalid transmission value, expected in [0, 1], aborting...")
+    if np.any(dt <= 0.0) or np.any(dt > 0.5):
+        raise RecurGreenError(binding.RG_EINVAL, "Invalid transmission half-width, expected 0 < dt <= 1/2, aborting...")
+
+
+def window_profile(tvals: np.ndarray, intensity: np.ndarray, tmin: float, tmax: float):
+    """(profile[nx], nt) of one realization (src/waveguide.f90:858-872): tvals (nv,), intensity (nx, nv)."""
+    sel = (tmin <= tvals) & (tvals <= tmax)
+    nt = int(sel.sum())
+    prof = intensity[:, sel].sum(axis=1)
+    if nt:
+        prof = prof * (4.0 / nt)
+    return prof, nt
+
+
+def group_profiles(tvals: np.ndarray, intensity: np.ndarray, tm, dt):
+    """(tprofile[nx, nprofile], nt[nprofile], tavg) of one realization (src/waveguide.f90:959-985)."""
+    tm, dt = np.asarray(tm, float), np.asarray(dt, float)
+    prof = np.zeros((intensity.shape[0], tm.size))
+    nt = np.zeros(tm.size, dtype=int)
+    for ip in range(tm.size):
+        sel = np.abs(tvals - tm[ip]) <= dt[ip]
+        nt[ip] = int(sel.sum())
+        prof[:, ip] = intensity[:, sel].sum(axis=1)
+        if nt[ip]:
+            prof[:, ip] *= 4.0 / nt[ip]
+    return prof, nt, float(np.sum(tvals) / tvals.size)
+
+
+def average_profiles(profiles: np.ndarray, nt: np.ndarray):
+    """(average over the realizations with nt != 0, nsample) (src/main.f90:466-483)."""
+    keep = np.asarray(nt) != 0
+    nsample = int(keep.sum())
+    avg = profiles[keep].sum(axis=0) / nsample if nsample else np.full(profiles.shape[1:], np.nan)
+    return avg, nsample
+
+
+def _pass(wguide, seed0: int, nseed: int, filtered: bool):
+    plan = wguide.plan
+    tv, it, st, code = plan.tprofile(None, seed0=seed0, nseed=nseed, filtered=filtered)
+    if code:
+        raise RecurGreenError(code)
+    return tv, it
+
+
+def compute_tprofile_batch(wguide, seed0: int, nseed: int, tmin: float, tmax: float):
+    """(profiles[nseed, nx], nt[nseed]) of realizations seed0 .. seed0+nseed-1 (the OpenMP loop of
+    task_tprofile, src/main.f90:440-461, as device passes)."""
+    check_window(tmin, tmax)
+    tv, it = _pass(wguide, seed0, nseed, False)
+    out = np.zeros((nseed, wguide.plan.nx))
+    nt = np.zeros(nseed, dtype=int)
+    for r in range(nseed):
+        out[r], nt[r] = window_profile(tv[r], it[r], tmin, tmax)
+    return out, nt
+
+
+def compute_tprofile(wguide, iseed: int, tmin: float, tmax: float):
+    """compute_tprofile (src/waveguide.f90:807-875): (tprofile[nx], nt) of realization iseed."""
+    p, nt = compute_tprofile_batch(wguide, iseed, 1, tmin, tmax)
+    return p[0], int(nt[0])
+
+
+def compute_tprofile_group_batch(wguide, seed0: int, nseed: int, tm, dt):
+    """(tprofile[nseed, nx, nprofile], nt[nseed, nprofile], tavg[nseed])."""
+    check_group(tm, dt)
+    tv, it = _pass(wguide, seed0, nseed, True)
+    npf = np.asarray(tm).size
+    out = np.zeros((nseed, wguide.plan.nx, npf))
+    nt = np.zeros((nseed, npf), dtype=int)
+    tavg = np.zeros(nseed)
+    for r in range(nseed):
+        out[r], nt[r], tavg[r] = group_profiles(tv[r], it[r], tm, dt)
+    return out, nt, tavg
+
+
+def compute_tprofile_group(wguide, iseed: int, tm, dt):
+    """compute_tprofile_group (src/waveguide.f90:920-997): (tprofile[nx, nprofile], nt, tavg)."""
+    p, nt, tavg = compute_tprofile_group_batch(wguide, iseed, 1, tm, dt)
+    return p[0], nt[0], float(tavg[0])
