@@ -12,10 +12,17 @@ with nt > 0 (src/main.f90:466-483).
 """
 from __future__ import annotations
 
+import datetime
+import os
+import time
+
 import numpy as np
 
 from . import binding
 from .binding import RecurGreenError
+from .distrib import format_g016
+from .replot import unique_path
+from .tpattern import parameter_lines
 
 
 def check_window(tmin: float, tmax: float) -> None:
@@ -114,3 +121,60 @@ def compute_tprofile_group(wguide, iseed: int, tm, dt):
     """compute_tprofile_group (src/waveguide.f90:920-997): (tprofile[nx, nprofile], nt, tavg)."""
     p, nt, tavg = compute_tprofile_group_batch(wguide, iseed, 1, tm, dt)
     return p[0], nt[0], float(tavg[0])
+
+
+def _g06(x: float) -> str:
+    """Fortran g0.6 for the window bounds of the file headers (values in [0, 1])."""
+    if x == 0.0:
+        return "0.000000"
+    if abs(x) < 0.1:
+        m, e = f"{x:.5E}".split("E")
+        mant = float(m) / 10.0
+        return f"{mant:.6f}E{int(e) + 1:+03d}"
+    digits_before = len(str(int(abs(x)))) if abs(x) >= 1 else 0
+    return f"{x:.{6 - digits_before}f}"
+
+
+def save_tprofile(directory: str, wguide, profiles: np.ndarray, avg: np.ndarray, nsample: int, tmin: float,
+                  tmax: float, nseed: int, elapsed: float):
+    """result_N.raw.csv (one profile per realization) and result_N.avg.csv ('x, tprofile') as
+    save_tprofile (src/main.f90:500-545) writes them, without the TikZ step; returns both paths."""
+    os.makedirs(directory, exist_ok=True)
+    now = datetime.datetime.now().astimezone()
+    head = ["%% Computed on " + now.strftime("%Y-%m-%d at %H:%M:%S %z") + " by RecurGreen-HIP"]
+    head += parameter_lines(wguide, "%%")
+    nx = profiles.shape[1]
+    head.append(f"%% Transmission profile parameters: Found nsample={nsample} / nseed={nseed}, nx={nx}, "
+                f"tmin={_g06(tmin)}, tmax={_g06(tmax)}, time={elapsed:.3f} s.")
+    raw = unique_path(directory, "result_", ".raw.csv")
+    with open(raw, "w") as f:
+        f.write("\n".join(head) + "\n")
+        f.write("%% Here follows, on each line, the function |psi_T(x)|^2 for a given realization of the disorder.\n")
+        for row in profiles:
+            f.write(", ".join(format_g016(v) for v in row) + "\n")
+    avgp = unique_path(directory, "result_", ".avg.csv")
+    xmesh = wguide.get_xmesh()
+    with open(avgp, "w") as f:
+        f.write("\n".join(head) + "\n")
+        f.write("x, tprofile\n")
+        for x, v in zip(xmesh, avg):
+            f.write(f"{format_g016(x)}, {format_g016(v)}\n")
+    return raw, avgp
+
+
+def task_tprofile(wguide, nseed: int, tmin: float, tmax: float, outputdir: str = "out_gpu", log=print):
+    """task_tprofile (src/main.f90:402-493): profiles of realizations 1..nseed, their average over
+    the realizations with nt > 0, and the two files.  Returns (raw path, avg path, avg, nsample)."""
+    if nseed <= 0:
+        raise RecurGreenError(binding.RG_EINVAL, f"Invalid number of disorder realizations (received {nseed}), aborting...")
+    t0 = time.perf_counter()
+    profiles, nt = compute_tprofile_batch(wguide, 1, nseed, tmin, tmax)
+    elapsed = time.perf_counter() - t0
+    avg, nsample = average_profiles(profiles, nt)
+    if nsample == 0:
+        log(f"[WARN] Found no transmission eigenfunction (nsample={nsample}), results will be undetermined...")
+    else:
+        log(f"[INFO] Found {nsample}/{nseed} transmission eigenfunctions, now averaging...")
+    raw, avgp = save_tprofile(os.path.join(outputdir, "tprofile"), wguide, profiles, avg, nsample, tmin, tmax,
+                              nseed, elapsed)
+    return raw, avgp, avg, nsample

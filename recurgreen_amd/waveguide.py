@@ -112,6 +112,18 @@ class Waveguide:
             raise RecurGreenError(code)
         return d[0]
 
+    def compute_tprofile(self, iseed: int, tmin: float, tmax: float):
+        """compute_tprofile (src/waveguide.f90:807-875): (tprofile[nx], nt) of realization iseed."""
+        from . import tprofile
+
+        return tprofile.compute_tprofile(self, iseed, tmin, tmax)
+
+    def compute_tprofile_group(self, iseed: int, tm, dt):
+        """compute_tprofile_group (src/waveguide.f90:920-997): (tprofile[nx, nprofile], nt, tavg)."""
+        from . import tprofile
+
+        return tprofile.compute_tprofile_group(self, iseed, tm, dt)
+
     def generate_potential(self, seed0: int, nreal: int = 1) -> np.ndarray:
         return self.plan.generate_potential(seed0, nreal)
 

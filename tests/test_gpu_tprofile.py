@@ -112,3 +112,26 @@ def test_tprofile_requires_reserve_and_checks_apertures():
         plan.tprofile(None, nseed=1, filtered=True)
     assert e.value.code == binding.RG_EINVAL
     plan.close()
+
+
+def test_task_tprofile_files(tmp_path):
+    """task_tprofile (src/main.f90:402-545): one raw row per realization, the average over the
+    realizations with nt > 0 in 'x, tprofile' rows on the slice abscissas."""
+    p = dict(boundtype="dirichlet", nxdiso=30, wol=10.25, dscat=2.0)
+    say("task tprofile files")
+    wg = Waveguide(**p, batch_max=4)
+    raw, avgp, avg, nsample = tprofile.task_tprofile(wg, 6, 0.6, 1.0, outputdir=str(tmp_path), log=lambda m: None)
+    profiles, nt = tprofile.compute_tprofile_batch(wg, 1, 6, 0.6, 1.0)
+    assert nsample == int((nt > 0).sum()) and nsample > 0
+    rows = [l for l in open(raw).read().splitlines() if not l.startswith("%")]
+    assert len(rows) == 6
+    got = np.array([[float(v) for v in r.split(",")] for r in rows])
+    np.testing.assert_allclose(got, profiles, rtol=1e-15, atol=0)
+    data = [l for l in open(avgp).read().splitlines() if not l.startswith("%")]
+    assert data[0] == "x, tprofile"
+    xa = np.array([[float(v) for v in r.split(",")] for r in data[1:]])
+    np.testing.assert_allclose(xa[:, 0], wg.get_xmesh(), rtol=1e-15)
+    np.testing.assert_allclose(xa[:, 1], profiles[nt > 0].mean(axis=0), rtol=1e-13)
+    pr, n1 = wg.compute_tprofile(3, 0.6, 1.0)
+    np.testing.assert_array_equal(pr, profiles[2])
+    wg.delete()
