@@ -139,6 +139,15 @@ rg_status rg_tprofile(rg_plan *plan, int64_t seed0, int32_t nseed, int32_t filte
 /* Same for explicit host potentials upot (nxdiso*ny*nreal doubles). */
 rg_status rg_tprofile_potential(rg_plan *plan, const double *upot, int32_t nreal, int32_t filtered, double *tvals,
                                 double *intensity, int32_t *status);
+/* Mirrors compute_wavefunction (src/waveguide.f90:636-697) for the realization of `seed`: psi (host,
+ * nx*ny complex, index ix*ny + iy, 0-based) receives psi(x_ix, y_iy) = sum_m chi_m(y_iy) G(x_ix; ipos)[m,
+ * imu] for the incident mode imode (1..nmu; imode = 1 is the frontal mode, :657-661) and the source
+ * slice ipos (1..nx), RG_EINVAL otherwise (:646-654).  The slice Green functions are kept in the
+ * tprofile workspace: call rg_tprofile_reserve(plan, 1) or more first.  Complex128. */
+rg_status rg_wavefunction(rg_plan *plan, int64_t seed, int32_t ipos, int32_t imode, double *psi, int32_t *status);
+/* Same for an explicit host potential upot (nxdiso*ny doubles). */
+rg_status rg_wavefunction_potential(rg_plan *plan, const double *upot, int32_t ipos, int32_t imode, double *psi,
+                                    int32_t *status);
 /* Adjoint sweep (no counterpart in the reference: the reverse-mode derivative of the sweep of
  * src/waveguide.f90:427-445).  loss_kind 0: L = Tr(t^H t) = sum of the transmission eigenvalues;
  * loss_kind 1: L = T_max, the largest transmission eigenvalue (t over the filtered ranges, complex

@@ -21,7 +21,7 @@ BOUNDTYPES = {"dirichlet": 0, "periodic": 1}
 # Every symbol include/recurgreen_hip.h declares.
 EXPORTS = (
     "rg_plan_create", "rg_plan_destroy", "rg_plan_dims", "rg_tdistrib", "rg_tdistrib_device",
-    "rg_tdistrib_potential", "rg_transmission_potential", "rg_refl_trans", "rg_refl_trans_potential", "rg_deposition", "rg_deposition_potential", "rg_tprofile_reserve", "rg_tprofile", "rg_tprofile_potential",
+    "rg_tdistrib_potential", "rg_transmission_potential", "rg_refl_trans", "rg_refl_trans_potential", "rg_deposition", "rg_deposition_potential", "rg_tprofile_reserve", "rg_tprofile", "rg_tprofile_potential", "rg_wavefunction", "rg_wavefunction_potential",
     "rg_tdistrib_uneven",
     "rg_adjoint_reserve", "rg_adjoint_potential", "rg_adjoint_potential_device",
     "rg_generate_potential", "rg_histogram",
@@ -88,6 +88,8 @@ def lib():
         "rg_tprofile_reserve": [vp, i32],
         "rg_tprofile": [vp, i64, i32, i32, vp, vp, vp],
         "rg_tprofile_potential": [vp, vp, i32, i32, vp, vp, vp],
+        "rg_wavefunction": [vp, i64, i32, i32, vp, vp],
+        "rg_wavefunction_potential": [vp, vp, i32, i32, vp, vp],
         "rg_generate_potential": [vp, i64, i32, vp],
         "rg_histogram": [vp, vp, i64, i32, dp, i32, i64p, i64p, i64p],
         "rg_plan_stream": [vp, C.POINTER(vp)],
@@ -315,6 +317,21 @@ class Plan:
         if code not in (RG_OK, RG_ESINGULAR, RG_ENOCONV, RG_ENONFINITE):
             check(code)
         return tv, it, st, code
+
+    def wavefunction(self, ipos: int, imode: int, upot: np.ndarray | None = None, seed: int = 1):
+        """(psi (nx, ny) complex128, status, code): compute_wavefunction's field
+        (src/waveguide.f90:636-697) for one realization, explicit upot(ix, iy) or a seed."""
+        self.tprofile_reserve(1)
+        psi = np.zeros((self.nx, self.ny), complex)
+        st = np.zeros(1, np.int32)
+        if upot is None:
+            code = lib().rg_wavefunction(self._h, int(seed), int(ipos), int(imode), ptr(psi), ptr(st))
+        else:
+            u = np.asfortranarray(upot, dtype=np.float64)
+            code = lib().rg_wavefunction_potential(self._h, ptr(u), int(ipos), int(imode), ptr(psi), ptr(st))
+        if code not in (RG_OK, RG_ESINGULAR, RG_ENONFINITE):
+            check(code)
+        return psi, st, code
 
     LOSSES = {"g": 0, "tmax": 1}
 

@@ -152,6 +152,7 @@ struct rg_plan {
     double2 *d_tp_g = nullptr, *d_tp_z = nullptr, *d_tp_x = nullptr, *d_tp_v = nullptr, *d_tp_w = nullptr;
     double2 *d_tp_y[2] = {nullptr, nullptr};
     double *d_tp_t = nullptr, *d_tp_i = nullptr;
+    double2 *d_tp_psi = nullptr;  // the transverse fields of the wavefunction pass (nx x ny)
     double *d_cq = nullptr, *d_teig = nullptr, *d_upot = nullptr;
     double *d_us = nullptr;  // disorder of the current slice for the batch (batch_max x ny)
     int32_t *d_status = nullptr;
@@ -243,6 +244,8 @@ bool rg_tprofile_supported(int n);
 void rg_launch_tprofile_svd(rg_plan *P, int nb, const double2 *gs1, int i0, int m, int j0, int nn, double2 *xwork,
                             double2 *vwork, double2 *w, double *tvals);
 void rg_launch_colnorms(rg_plan *P, int nb, const double2 *y, int nv, double *out, size_t ostride);
+void rg_launch_unit_column(rg_plan *P, double2 *w, int row);
+void rg_launch_modal_column(rg_plan *P, const double2 *y, double2 *psi);
 void rg_launch_add_c128(rg_plan *P, double2 *a, const double2 *b, size_t count);
 // rg_eig.hip
 void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status);

@@ -53,7 +53,7 @@ void free_plan(rg_plan *P)
                     P->d_gss32, P->d_gs1_32[0], P->d_gs1_32[1], P->d_g11, P->d_g1s[0], P->d_g1s[1],
                     P->d_sqrtremu, P->d_redo, P->d_us, P->d_ck, P->d_seg, P->d_adj, P->d_loss, P->d_grad, P->d_bconj,
                     P->d_schur, P->d_schur_zero, P->d_schur_i, P->d_tp_g, P->d_tp_z, P->d_tp_x, P->d_tp_v,
-                    P->d_tp_w, P->d_tp_y[0], P->d_tp_y[1], P->d_tp_t, P->d_tp_i};
+                    P->d_tp_w, P->d_tp_y[0], P->d_tp_y[1], P->d_tp_t, P->d_tp_i, P->d_tp_psi};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_graphs(P);
@@ -566,6 +566,45 @@ void run_sweep_tprofile(rg_plan *P, int nb, int64_t seed0, const double *d_upot,
         rg_zgemm(P, nb, one, P->d_tp_z + k * slab, 0, P->d_tp_w, 0, zero, yb, nullptr, nullptr, nullptr);
         rg_zgemm(P, nb, mone, P->d_tp_g + k * slab, 0, ya, 0, one, yb, nullptr, P->d_b, nullptr);
         rg_launch_colnorms(P, nb, yb, n, P->d_tp_i + (size_t)k * n, ostride);
+        std::swap(ya, yb);
+    }
+}
+
+// compute_wavefunction (src/waveguide.f90:636-697) for one realization: the column imu of G(x; ipos) for
+// every slice x by the block-tridiagonal solve's two recursions on the stored slice Green functions
+// g_x (src/base_utils.f90:320-333): z_x = 0 (x < ipos), z_ipos = g_ipos e_imu, z_x = -g_x B z_(x-1);
+// y_nx = z_nx, y_x = z_x - g_x B y_(x+1); each y_x goes through the transverse modes into psi[x][:]
+// (ny complex per slice).  Uses the tprofile workspace (one realization).
+void run_sweep_wavefunction(rg_plan *P, int64_t seed0, const double *d_upot, int ipos, int imu, double2 *psi)
+{
+    const int nx = P->nx, nb = 1;
+    const size_t mat = (size_t)P->np * P->np, slab = (size_t)P->tp_cap * mat;
+    const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
+    for (int ix = 1; ix <= nx; ix++) {
+        const int id = ix - P->nxfree;
+        const bool dis = id >= 1 && id <= P->nxdiso;
+        if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
+        invert_f64(P, nb, ix == 1 ? 0 : (ix == nx ? 2 : 1), ix == 1, dis);
+        (void)hipMemcpyAsync(P->d_tp_g + (ix - 1) * slab, P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice,
+                             P->stream);
+    }
+    double2 *g = P->d_tp_g, *z = P->d_tp_z;
+    rg_launch_unit_column(P, P->d_tp_w, imu - 1);
+    rg_zgemm(P, nb, one, g + (ipos - 1) * slab, 0, P->d_tp_w, 0, zero, z + (ipos - 1) * slab, nullptr, nullptr, nullptr);
+    for (int ix = ipos + 1; ix <= nx; ix++)
+        rg_zgemm(P, nb, mone, g + (ix - 1) * slab, 0, z + (ix - 2) * slab, 0, zero, z + (ix - 1) * slab, nullptr, P->d_b,
+                 nullptr);
+    double2 *ya = P->d_tp_y[0], *yb = P->d_tp_y[1];
+    (void)hipMemcpyAsync(ya, z + (nx - 1) * slab, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+    rg_launch_modal_column(P, ya, psi + (size_t)(nx - 1) * P->ny);
+    for (int ix = nx - 1; ix >= 1; ix--) {
+        double2 beta = zero;
+        if (ix >= ipos) {
+            (void)hipMemcpyAsync(yb, z + (ix - 1) * slab, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+            beta = one;
+        }
+        rg_zgemm(P, nb, mone, g + (ix - 1) * slab, 0, ya, 0, beta, yb, nullptr, P->d_b, nullptr);
+        rg_launch_modal_column(P, yb, psi + (size_t)(ix - 1) * P->ny);
         std::swap(ya, yb);
     }
 }
@@ -1257,7 +1296,7 @@ rg_status rg_tprofile_reserve(rg_plan *P, int32_t nreal_max)
     if (P->tp_cap >= nreal_max) return RG_OK;
     for (void **b : {(void **)&P->d_tp_g, (void **)&P->d_tp_z, (void **)&P->d_tp_x, (void **)&P->d_tp_v,
                      (void **)&P->d_tp_w, (void **)&P->d_tp_y[0], (void **)&P->d_tp_y[1], (void **)&P->d_tp_t,
-                     (void **)&P->d_tp_i}) {
+                     (void **)&P->d_tp_i, (void **)&P->d_tp_psi}) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
     }
@@ -1268,7 +1307,8 @@ rg_status rg_tprofile_reserve(rg_plan *P, int32_t nreal_max)
         (st = dalloc(&P->d_tp_x, cmat)) || (st = dalloc(&P->d_tp_v, cmat)) || (st = dalloc(&P->d_tp_w, cmat)) ||
         (st = dalloc(&P->d_tp_y[0], cmat)) || (st = dalloc(&P->d_tp_y[1], cmat)) ||
         (st = dalloc(&P->d_tp_t, (size_t)nreal_max * P->nmu)) ||
-        (st = dalloc(&P->d_tp_i, (size_t)nreal_max * P->nx * P->nmu)))
+        (st = dalloc(&P->d_tp_i, (size_t)nreal_max * P->nx * P->nmu)) ||
+        (st = dalloc(&P->d_tp_psi, (size_t)P->nx * P->ny)))
         return st;
     P->tp_cap = nreal_max;
     return RG_OK;
@@ -1321,6 +1361,47 @@ rg_status rg_tprofile_potential(rg_plan *P, const double *upot, int32_t nreal, i
 {
     if (!upot && nreal > 0) return RG_EINVAL;
     return tprofile_pass(P, 0, upot, nreal, filtered, tvals, intensity, status);
+}
+
+static rg_status wavefunction_pass(rg_plan *P, int64_t seed, const double *upot, int32_t ipos, int32_t imode,
+                                   double *psi, int32_t *status)
+{
+    rg_status st = check_common(P, 1, RG_FP64);
+    if (st) return st;
+    if (!psi || imode < 1 || imode > P->nmu || ipos < 1 || ipos > P->nx) return RG_EINVAL;  // :646-654
+    if (P->tp_cap < 1) return RG_EINVAL;
+    // imode = 1 is the frontal mode: the middle of the plane-wave list (:657-661)
+    const int imu = P->p.boundtype == RG_PERIODIC ? (imode + P->nmu / 2 - 1) % P->nmu + 1 : imode;
+    DeviceGuard g(P->device);
+    const size_t per = (size_t)P->nxdiso * P->ny;
+    HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t), P->stream));
+    const double *dup = nullptr;
+    if (upot) {
+        for (size_t e = 0; e < per; e++)
+            if (!std::isfinite(upot[e])) return RG_ENONFINITE;
+        HIPCHK(hipMemcpyAsync(P->d_upot, upot, sizeof(double) * per, hipMemcpyHostToDevice, P->stream));
+        dup = P->d_upot;
+    }
+    const size_t need = (size_t)P->nx * P->ny;
+    run_sweep_wavefunction(P, seed, dup, ipos, imu, P->d_tp_psi);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(psi, P->d_tp_psi, sizeof(double2) * need, hipMemcpyDeviceToHost, P->stream));
+    int32_t worst = 0;
+    if ((st = harvest_status(P, 1, status, &worst))) return st;
+    if (P->profiling) harvest_timers(P);
+    return worst ? (rg_status)worst : RG_OK;
+}
+
+rg_status rg_wavefunction(rg_plan *P, int64_t seed, int32_t ipos, int32_t imode, double *psi, int32_t *status)
+{
+    return wavefunction_pass(P, seed, nullptr, ipos, imode, psi, status);
+}
+
+rg_status rg_wavefunction_potential(rg_plan *P, const double *upot, int32_t ipos, int32_t imode, double *psi,
+                                    int32_t *status)
+{
+    if (!upot) return RG_EINVAL;
+    return wavefunction_pass(P, 0, upot, ipos, imode, psi, status);
 }
 
 rg_status rg_adjoint_reserve(rg_plan *P, int32_t nreal_max)

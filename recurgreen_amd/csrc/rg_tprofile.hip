@@ -160,9 +160,59 @@ __global__ __launch_bounds__(256) void k_colnorms(const double2 *__restrict__ y,
     }
 }
 
+// W = e_row in column 0 (the active n x n block, zero elsewhere): the right-hand side of one column of
+// the Green function.
+__global__ __launch_bounds__(256) void k_unit_column(double2 *__restrict__ w, int NP, int n, int row)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x) {
+        const int i = e / n, j = e % n;
+        w[(size_t)i * NP + j] = cmk(i == row && j == 0 ? 1.0 : 0.0, 0.0);
+    }
+}
+
+// psi[iy] = sum_m chi_m(y_iy) Y[m][0]: the transverse field of one slice from its modal column, with the
+// modes of src/waveguide.f90:151-229 on y_iy = (iy + 1/2)/ny (0-based iy): sqrt(2/ny) sin(pi m y)
+// (Dirichlet, m = 1..nmu) or exp(2 pi i (m - nmu/2 - 1) y)/sqrt(ny) (periodic).
+__global__ __launch_bounds__(256) void k_modal_column(const double2 *__restrict__ y, int NP, int n, int ny,
+                                                      int boundtype, double2 *__restrict__ psi)
+{
+    const int iy = blockIdx.x * blockDim.x + threadIdx.x;
+    if (iy >= ny) return;
+    const double yv = (iy + 0.5) / ny;
+    double2 acc = cmk(0.0, 0.0);
+    if (boundtype == RG_DIRICHLET) {
+        const double nrm = sqrt(2.0 / ny);
+        for (int m = 0; m < n; m++) {
+            const double c = nrm * sinpi((m + 1) * yv);
+            const double2 v = y[(size_t)m * NP];
+            acc = cmk(fma(c, v.x, acc.x), fma(c, v.y, acc.y));
+        }
+    } else {
+        const double nrm = 1.0 / sqrt((double)ny);
+        for (int m = 0; m < n; m++) {
+            double sn, cs;
+            sincospi(2.0 * (m - n / 2) * yv, &sn, &cs);
+            acc = cadd(acc, cmul(cmk(nrm * cs, nrm * sn), y[(size_t)m * NP]));
+        }
+    }
+    psi[iy] = acc;
+}
+
 }  // namespace
 
 bool rg_tprofile_supported(int n) { return n <= TP_MAXN; }
+
+void rg_launch_unit_column(rg_plan *P, double2 *w, int row)
+{
+    hipLaunchKernelGGL(k_unit_column, dim3((P->nmu * P->nmu + 255) / 256), dim3(256), 0, P->stream, w, P->np, P->nmu,
+                       row);
+}
+
+void rg_launch_modal_column(rg_plan *P, const double2 *y, double2 *psi)
+{
+    hipLaunchKernelGGL(k_modal_column, dim3((P->ny + 255) / 256), dim3(256), 0, P->stream, y, P->np, P->nmu, P->ny,
+                       P->p.boundtype, psi);
+}
 
 void rg_launch_tprofile_svd(rg_plan *P, int nb, const double2 *gs1, int i0, int m, int j0, int nn, double2 *xwork,
                             double2 *vwork, double2 *w, double *tvals)

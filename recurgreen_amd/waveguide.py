@@ -112,6 +112,30 @@ class Waveguide:
             raise RecurGreenError(code)
         return d[0]
 
+    def compute_wavefunction(self, iseed: int, ipos: int, imode: int) -> np.ndarray:
+        """wfundata of compute_wavefunction (src/waveguide.f90:636-697): (nx*ny, 5) rows
+        [x/L, y/W, Re psi, Im psi, Upot] with row (ix-1)*ny + iy, Upot = U dx/k on the disordered
+        slices and 0 on the free ones; same index checks (:646-654)."""
+        nmu, nx, ny = self.plan.nmu, self.plan.nx, self.plan.ny
+        if imode < 1 or nmu < imode:
+            raise RecurGreenError(binding.RG_EINVAL,
+                                  f"Invalid initial mode index, received imode={imode}, expected in 1..{nmu}, aborting...")
+        if ipos < 1 or nx < ipos:
+            raise RecurGreenError(binding.RG_EINVAL,
+                                  f"Invalid initial position index, received ipos={ipos}, expected in 1..{nx}, aborting...")
+        psi, st, code = self.plan.wavefunction(ipos, imode, None, seed=iseed)
+        if code:
+            raise RecurGreenError(code)
+        w = np.zeros((nx * ny, 5))
+        w[:, 0] = np.repeat(self.get_xmesh(), ny)
+        w[:, 1] = np.tile((np.arange(1, ny + 1) - 0.5) / ny, nx)
+        w[:, 2] = psi.real.ravel()
+        w[:, 3] = psi.imag.ravel()
+        u = self.plan.generate_potential(iseed, 1)[:, :, 0]  # (nxdiso, ny)
+        nf = self.params.get("nxfree", 0)
+        w[nf * ny:(nf + u.shape[0]) * ny, 4] = u.ravel()
+        return w
+
     def compute_tprofile(self, iseed: int, tmin: float, tmax: float):
         """compute_tprofile (src/waveguide.f90:807-875): (tprofile[nx], nt) of realization iseed."""
         from . import tprofile
