@@ -1,5 +1,6 @@
 !!**************************************************************************************************
-!! recurgreen_gpu.f90 -- namelist-compatible `tdistrib` and `tpattern` driver whose sweep runs on the GPU.
+!! recurgreen_gpu.f90 -- namelist-compatible task driver (tdistrib, tpattern, tprofile, tprofile_group,
+!! wavefunction) whose sweeps run on the GPU.
 !!
 !! Usage: recurgreen_gpu settings.nml [more.nml ...]
 !! Reads the reference's groups unchanged: &settings task (src/main.f90:45), &waveguide and
@@ -64,9 +65,12 @@ contains
                 call task_tdistrib(funit)
             case ("tpattern")
                 call task_tpattern(funit)
-            case ("wavefunction", "tprofile", "tprofile_group")
-                write (error_unit, '(a)') "[ERROR] Task '" // trim(task) // "' is not part of the GPU sweep, aborting..."
-                stop errmsg_invalid_arg
+            case ("wavefunction")
+                call task_wavefunction(funit)
+            case ("tprofile")
+                call task_tprofile(funit)
+            case ("tprofile_group")
+                call task_tprofile_group(funit)
             case default
                 write (error_unit, '(a)') "[ERROR] Unknown task '" // trim(task) // "', aborting..."
                 stop errmsg_invalid_arg
@@ -236,6 +240,274 @@ contains
         rprob = rprob/nseed
         write (output_unit, '(a,f0.3,a)') "[INFO] Done, computation time is ", time, " s."
         call save_tpattern(wguide, tprob, rprob, probatype, imode, nseed, time)
+        call wguide%del()
+    end subroutine
+
+    !! &waveguide and the optional &gpu group, then the plan for passes of up to min(batch, cap)
+    !! realizations; returns nxdiso, nxfree and the pass size.
+    subroutine open_waveguide(funit, wguide, cap, nxdiso, nxfree, bsz)
+        integer, intent(in) :: funit, cap
+        type(waveguide_gpu_t), intent(inout) :: wguide
+        integer, intent(out) :: nxdiso, nxfree, bsz
+        character(len=50) :: boundtype, precision, bintype
+        real(wp) :: wol, aratio, exfac, dscat, dabso, napera, naperb
+        integer :: istat, device, batch, ny, dos_model
+        namelist /waveguide/ boundtype, wol, exfac, dscat, dabso, aratio, nxdiso, nxfree, napera, naperb
+        namelist /gpu/ device, precision, batch, ny, dos_model, bintype
+        boundtype = "dirichlet"; wol = 4.3_wp; exfac = 1.0_wp; aratio = 1.0_wp; dscat = 1.0_wp; dabso = 0.0_wp
+        nxdiso = 100; nxfree = 0; napera = 1.0_wp; naperb = 1.0_wp
+        device = 0; precision = "fp64"; batch = 64; ny = 0; dos_model = 0; bintype = "chebyshev"
+        read(unit=funit, nml=waveguide, iostat=istat)
+        call rewind_or_stop(funit, istat, .false.)
+        read(unit=funit, nml=gpu, iostat=istat)
+        call rewind_or_stop(funit, istat, .true.)
+        bsz = max(1, min(batch, cap))
+        call wguide%set_options(ny, dos_model, RG_FP64)
+        call wguide%init(boundtype, nxdiso, nxfree, wol, exfac, aratio, dscat, dabso, napera, naperb, &
+                         batch_max=bsz, device=device)
+        call wguide%print_parameters(output_unit, "[INFO]")
+    end subroutine
+
+    !! out_gpu/<sub>/result_N<ext> with the smallest free N (unique_filename, src/base_utils.f90:825-840),
+    !! opened with the timestamp and parameter header (print_timestamp, print_parameters).
+    subroutine open_result(wguide, sub, ext, filename, outfp)
+        type(waveguide_gpu_t), intent(in) :: wguide
+        character(len=*), intent(in) :: sub, ext
+        character(len=*), intent(out) :: filename
+        integer, intent(out) :: outfp
+        character(len=40) :: zone
+        integer :: ifile, dt(8)
+        logical :: fexist
+        call execute_command_line("mkdir -p out_gpu/" // sub)
+        do ifile = 1, 1000000
+            write (filename, '(a,i0,a)') "out_gpu/" // sub // "/result_", ifile, ext
+            inquire(file=filename, exist=fexist)
+            if (.not. fexist) exit
+        end do
+        open(newunit=outfp, file=filename, action='write')
+        call date_and_time(values=dt, zone=zone)
+        write (outfp, '(a,i4,a,5(i0.2,a))') "%% Computed on ", dt(1), "-", dt(2), "-", dt(3), " at ", &
+            dt(5), ":", dt(6), ":", dt(7), " " // trim(zone) // " by RecurGreen-HIP"
+        call wguide%print_parameters(outfp, "%%")
+    end subroutine
+
+    !! task_wavefunction and save_wavefunction (src/main.f90:335-380): the field of one realization
+    !! in out_gpu/wavefun/result_N.wavefun.csv, rows x, y, Re psi, Im psi, Upot.
+    subroutine task_wavefunction(funit)
+        integer, intent(in) :: funit
+        type(waveguide_gpu_t) :: wguide
+        integer :: iseed, ipos, imode, istat, nxdiso, nxfree, bsz, nx, ny, ix, iy, outfp
+        complex(wp), allocatable :: psi(:, :)
+        real(wp), allocatable :: upot(:, :), xmesh(:)
+        real(wp) :: u
+        character(len=200) :: filename
+        namelist /wavefunction/ iseed, ipos, imode
+        iseed = 1; ipos = 1; imode = 1
+        read(unit=funit, nml=wavefunction, iostat=istat)
+        call rewind_or_stop(funit, istat, .false.)
+        call open_waveguide(funit, wguide, 1, nxdiso, nxfree, bsz)
+        write (output_unit, '(3(a,i0))') "[INFO] Wavefunction parameters: iseed=", iseed, ", ipos=", ipos, ", imode=", imode
+        nx = wguide%get_nx()
+        ny = wguide%get_ny()
+        allocate(psi(ny, nx), upot(nxdiso, ny), xmesh(nx))
+        call wguide%compute_wavefunction_field(iseed, ipos, imode, psi)
+        call wguide%generate_potential(iseed, upot)
+        call wguide%get_xmesh(xmesh)
+        call open_result(wguide, "wavefun", ".wavefun.csv", filename, outfp)
+        write (outfp, '(3(a,i0))') "%% Wavefunction parameters: iseed=", iseed, ", ipos=", ipos, ", imode=", imode
+        write (outfp, '(a)') "x, y, repsi, impsi, upot"
+        do ix = 1, nx
+            do iy = 1, ny
+                u = 0.0_wp
+                if (ix - nxfree >= 1 .and. ix - nxfree <= nxdiso) u = upot(ix - nxfree, iy)
+                write (outfp, '(g0.16,4(", ",g0.16))') xmesh(ix), (iy - 0.5_wp)/ny, real(psi(iy, ix)), &
+                    aimag(psi(iy, ix)), u
+            end do
+        end do
+        close(outfp)
+        write (output_unit, '(a)') "[INFO] Result written in file: " // trim(filename)
+        call wguide%del()
+    end subroutine
+
+    !! task_tprofile and save_tprofile (src/main.f90:402-545): the profiles of realizations 1..nseed
+    !! in batched device passes, their average over the realizations with nt > 0, and the raw and
+    !! average files under out_gpu/tprofile.
+    subroutine task_tprofile(funit)
+        integer, intent(in) :: funit
+        type(waveguide_gpu_t) :: wguide
+        real(wp) :: tmin, tmax, time
+        integer :: nseed, nthreads, istat, nxdiso, nxfree, bsz, nx, nmu, off, nb, k, v, nsample, outfp, ix
+        integer(8) :: c0, c1, crate
+        real(wp), allocatable :: tvals(:, :), intens(:, :, :), prof(:, :), avg(:), xmesh(:)
+        integer, allocatable :: nt(:)
+        character(len=200) :: filename
+        character(len=40) :: sfmt
+        namelist /tprofile/ tmin, tmax, nseed, nthreads
+        tmin = 0.9_wp; tmax = 1.0_wp; nseed = 1; nthreads = 1
+        read(unit=funit, nml=tprofile, iostat=istat)
+        call rewind_or_stop(funit, istat, .false.)
+        if (nseed <= 0) then
+            write (error_unit, '(a,i0,a)') "[ERROR] Invalid number of disorder realizations (received ", nseed, &
+                "), aborting..."
+            stop errmsg_invalid_arg
+        else if (tmin < 0.0_wp .or. tmin > 1.0_wp) then
+            write (error_unit, '(a,g0.3,a)') "[ERROR] Invalid Tmin, received Tmin=", tmin, ", expected in [0, 1], aborting..."
+            stop errmsg_invalid_arg
+        else if (tmax < 0.0_wp .or. tmax > 1.0_wp) then
+            write (error_unit, '(a,g0.3,a)') "[ERROR] Invalid Tmax, received Tmax=", tmax, ", expected in [0, 1], aborting..."
+            stop errmsg_invalid_arg
+        else if (tmin >= tmax) then
+            write (error_unit, '(a)') "[ERROR] Invalid range (Tmin >= Tmax), aborting..."
+            stop errmsg_invalid_arg
+        end if
+        call open_waveguide(funit, wguide, nseed, nxdiso, nxfree, bsz)
+        nx = wguide%get_nx()
+        nmu = wguide%get_nmu()
+        call wguide%tprofile_reserve(bsz)
+        allocate(tvals(nmu, bsz), intens(nmu, nx, bsz), prof(nseed, nx), nt(nseed), avg(nx), xmesh(nx))
+        prof = 0.0_wp
+        nt = 0
+        call system_clock(c0, crate)
+        do off = 0, nseed - 1, bsz
+            nb = min(bsz, nseed - off)
+            call wguide%compute_tprofile_data(1 + off, nb, .false., tvals(:, 1:nb), intens(:, :, 1:nb))
+            do k = 1, nb   !! compute_tprofile, step 7 (src/waveguide.f90:858-872)
+                do v = 1, nmu
+                    if (tmin <= tvals(v, k) .and. tvals(v, k) <= tmax) then
+                        prof(off + k, :) = prof(off + k, :) + intens(v, :, k)
+                        nt(off + k) = nt(off + k) + 1
+                    end if
+                end do
+                if (nt(off + k) /= 0) prof(off + k, :) = prof(off + k, :)*(4.0_wp/nt(off + k))
+            end do
+        end do
+        call system_clock(c1)
+        time = real(c1 - c0, wp)/real(crate, wp)
+        nsample = count(nt /= 0)
+        avg = 0.0_wp
+        if (nsample == 0) then
+            write (output_unit, '(a,i0,a)') "[WARN] Found no transmission eigenfunction (nsample=", nsample, &
+                "), results will be undetermined..."
+        else
+            write (output_unit, '(2(a,i0),a)') "[INFO] Found ", nsample, "/", nseed, &
+                " transmission eigenfunctions, now averaging..."
+            do k = 1, nseed
+                if (nt(k) /= 0) avg = avg + prof(k, :)
+            end do
+            avg = avg/nsample
+        end if
+        call wguide%get_xmesh(xmesh)
+        call open_result(wguide, "tprofile", ".raw.csv", filename, outfp)
+        write (outfp, '(3(a,i0),2(a,g0.6),a,f0.3,a)') "%% Transmission profile parameters: Found nsample=", &
+            nsample, " / nseed=", nseed, ", nx=", nx, ", tmin=", tmin, ", tmax=", tmax, ", time=", time, " s."
+        write (outfp, '(a)') "%% Here follows, on each line, the function |psi_T(x)|^2 for a given realization of the disorder."
+        write (sfmt, '(a,i0,a)') "(g0.16,", max(1, nx - 1), "(', ',g0.16))"
+        do k = 1, nseed
+            write (outfp, sfmt) prof(k, :)
+        end do
+        close(outfp)
+        write (output_unit, '(a)') "[INFO] Raw profile data written in file: " // trim(filename)
+        call open_result(wguide, "tprofile", ".avg.csv", filename, outfp)
+        write (outfp, '(3(a,i0),2(a,g0.6),a,f0.3,a)') "%% Transmission profile parameters: Found nsample=", &
+            nsample, " / nseed=", nseed, ", nx=", nx, ", tmin=", tmin, ", tmax=", tmax, ", time=", time, " s."
+        write (outfp, '(a)') "x, tprofile"
+        do ix = 1, nx
+            write (outfp, '(g0.16,", ",g0.16)') xmesh(ix), avg(ix)
+        end do
+        close(outfp)
+        write (output_unit, '(a)') "[INFO] Average profile written in file: " // trim(filename)
+        call wguide%del()
+    end subroutine
+
+    !! task_tprofile_group and save_tprofile_group (src/main.f90:564-735): four windows |T - tm| <= dt
+    !! of the aperture-filtered matrix, averaged over the realizations with nt > 0, the mean
+    !! transmission tavg with its uncertainty, in out_gpu/tprofile_group/result_N.csv.
+    subroutine task_tprofile_group(funit)
+        integer, intent(in) :: funit
+        integer, parameter :: nprofile = 4
+        type(waveguide_gpu_t) :: wguide
+        real(wp) :: tm(nprofile), dt(nprofile), tavg, uctavg, time
+        integer :: nseed, nthreads, istat, nxdiso, nxfree, bsz, nx, nv, off, nb, k, v, ip, outfp, ix
+        integer :: nsample(nprofile)
+        integer(8) :: c0, c1, crate
+        real(wp), allocatable :: tvals(:, :), intens(:, :, :), prof(:, :), sumprof(:, :), tavgdata(:), xmesh(:)
+        integer :: ntk(nprofile)
+        character(len=200) :: filename
+        character(len=120) :: sfmt
+        namelist /tprofile_group/ tm, dt, nseed, nthreads
+        tm = [0.1_wp, 0.4_wp, 0.7_wp, 0.95_wp]; dt = 0.05_wp; nseed = 1; nthreads = 1
+        read(unit=funit, nml=tprofile_group, iostat=istat)
+        call rewind_or_stop(funit, istat, .false.)
+        if (nseed <= 0) then
+            write (error_unit, '(a,i0,a)') "[ERROR] Invalid number of disorder realizations (received ", nseed, &
+                "), aborting..."
+            stop errmsg_invalid_arg
+        else if (any(tm < 0.0_wp) .or. any(tm > 1.0_wp)) then
+            write (error_unit, '(a)') "[ERROR] Invalid transmission value, expected in [0, 1], aborting..."
+            stop errmsg_invalid_arg
+        else if (any(dt <= 0.0_wp) .or. any(dt > 0.5_wp)) then
+            write (error_unit, '(a)') "[ERROR] Invalid transmission half-width, expected 0 < dt <= 1/2, aborting..."
+            stop errmsg_invalid_arg
+        end if
+        call open_waveguide(funit, wguide, nseed, nxdiso, nxfree, bsz)
+        nx = wguide%get_nx()
+        nv = wguide%get_neigv_filtered()
+        call wguide%tprofile_reserve(bsz)
+        allocate(tvals(nv, bsz), intens(nv, nx, bsz), prof(nx, nprofile), sumprof(nx, nprofile), tavgdata(nseed))
+        allocate(xmesh(nx))
+        sumprof = 0.0_wp
+        nsample = 0
+        call system_clock(c0, crate)
+        do off = 0, nseed - 1, bsz
+            nb = min(bsz, nseed - off)
+            call wguide%compute_tprofile_data(1 + off, nb, .true., tvals(:, 1:nb), intens(:, :, 1:nb))
+            do k = 1, nb   !! compute_tprofile_group, steps 6-7 (src/waveguide.f90:958-985)
+                tavgdata(off + k) = sum(tvals(:, k))/nv
+                prof = 0.0_wp
+                ntk = 0
+                do ip = 1, nprofile
+                    do v = 1, nv
+                        if (abs(tvals(v, k) - tm(ip)) <= dt(ip)) then
+                            prof(:, ip) = prof(:, ip) + intens(v, :, k)
+                            ntk(ip) = ntk(ip) + 1
+                        end if
+                    end do
+                    if (ntk(ip) /= 0) then
+                        sumprof(:, ip) = sumprof(:, ip) + prof(:, ip)*(4.0_wp/ntk(ip))
+                        nsample(ip) = nsample(ip) + 1
+                    end if
+                end do
+            end do
+        end do
+        call system_clock(c1)
+        time = real(c1 - c0, wp)/real(crate, wp)
+        do ip = 1, nprofile
+            if (nsample(ip) /= 0) then
+                write (output_unit, '(4(a,i0),a)') "[INFO] Profile ", ip, "/", nprofile, " : Found ", nsample(ip), "/", &
+                    nseed, " transmission eigenfunctions, now averaging..."
+                sumprof(:, ip) = sumprof(:, ip)/nsample(ip)
+            else
+                write (output_unit, '(3(a,i0),a)') "[WARN] Profile ", ip, "/", nprofile, &
+                    " : Found no transmission eigenstate (nsample=", nsample(ip), "), results will be undetermined..."
+            end if
+        end do
+        tavg = sum(tavgdata)/nseed
+        uctavg = sqrt(sum((tavgdata - tavg)**2))/nseed
+        call wguide%get_xmesh(xmesh)
+        call open_result(wguide, "tprofile_group", ".csv", filename, outfp)
+        write (sfmt, '(a,i0,a)') "(a,i0,2(a,g0.6,", nprofile-1, "(', ',g0.6)),a)"
+        write (outfp, sfmt) "%% Transmission profile parameters: nx=", nx, ", tm=[", tm, "], dt=[", dt, "]."
+        write (sfmt, '(a,i0,a)') "(a,i0,", nprofile-1, "(', ',i0),a,i0,2(a,g0.6),a,f0.3,a)"
+        write (outfp, sfmt) "%% Results: Found nsample=[", nsample, "] over nseed=", nseed, &
+            ", tavg=", tavg, "+-", uctavg, ", time=", time, " s."
+        write (sfmt, '(a,i0,a)') "('x',", nprofile, "(', tprofile',f0.4))"
+        write (outfp, sfmt) tm
+        write (sfmt, '(a,i0,a)') "(g0.16,", nprofile, "(', ',g0.16))"
+        do ix = 1, nx
+            write (outfp, sfmt) xmesh(ix), sumprof(ix, :)
+        end do
+        close(outfp)
+        write (output_unit, '(a)') "[INFO] Average profiles written in file: " // trim(filename)
         call wguide%del()
     end subroutine
 

@@ -67,6 +67,35 @@ module recurgreen_hip
             complex(c_double_complex), intent(out) :: depmat(*)
             integer(c_int32_t), intent(out) :: status(*)
         end function
+        integer(c_int) function rg_tprofile_reserve(plan, nreal_max) bind(C, name="rg_tprofile_reserve")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: plan
+            integer(c_int32_t), value :: nreal_max
+        end function
+        integer(c_int) function rg_tprofile(plan, seed0, nseed, filtered, tvals, intensity, status) &
+                bind(C, name="rg_tprofile")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            type(c_ptr), value :: plan
+            integer(c_int64_t), value :: seed0
+            integer(c_int32_t), value :: nseed, filtered
+            real(c_double), intent(out) :: tvals(*), intensity(*)
+            integer(c_int32_t), intent(out) :: status(*)
+        end function
+        integer(c_int) function rg_wavefunction(plan, seed, ipos, imode, psi, status) bind(C, name="rg_wavefunction")
+            import :: c_int, c_int32_t, c_int64_t, c_double_complex, c_ptr
+            type(c_ptr), value :: plan
+            integer(c_int64_t), value :: seed
+            integer(c_int32_t), value :: ipos, imode
+            complex(c_double_complex), intent(out) :: psi(*)
+            integer(c_int32_t), intent(out) :: status(*)
+        end function
+        integer(c_int) function rg_generate_potential(plan, seed0, nreal, upot) bind(C, name="rg_generate_potential")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            type(c_ptr), value :: plan
+            integer(c_int64_t), value :: seed0
+            integer(c_int32_t), value :: nreal
+            real(c_double), intent(out) :: upot(*)
+        end function
         integer(c_int) function rg_histogram(plan, samples, nsamp, on_device, bins, nbins, counts, missing, dropped) &
                 bind(C, name="rg_histogram")
             import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
@@ -104,6 +133,13 @@ module recurgreen_hip
         procedure :: compute_refl_trans => compute_refl_trans_waveguide_gpu
         procedure :: compute_refl_trans_batch => compute_refl_trans_batch_waveguide_gpu
         procedure :: compute_deposition_batch => compute_deposition_batch_waveguide_gpu
+        procedure :: tprofile_reserve => tprofile_reserve_waveguide_gpu
+        procedure :: compute_tprofile_data => compute_tprofile_data_waveguide_gpu
+        procedure :: compute_wavefunction_field => compute_wavefunction_field_waveguide_gpu
+        procedure :: generate_potential => generate_potential_waveguide_gpu
+        procedure :: get_xmesh => get_xmesh_waveguide_gpu
+        procedure :: get_ny => get_ny_waveguide_gpu
+        procedure :: get_neigv_filtered => get_neigv_filtered_waveguide_gpu
         procedure :: histogram => histogram_waveguide_gpu
         procedure :: print_parameters => print_parameters_waveguide_gpu
         procedure :: del => del_waveguide_gpu
@@ -284,6 +320,79 @@ contains
         allocate(st(nseed))
         call check_status(rg_deposition(self%plan, int(seed0, c_int64_t), int(nseed, c_int32_t), &
             int(fpos, c_int32_t), int(ipos, c_int32_t), depmat, st))
+    end subroutine
+
+    !! Workspace of the tprofile and wavefunction passes (the Green functions of every slice for
+    !! nreal_max realizations per device pass).
+    subroutine tprofile_reserve_waveguide_gpu(self, nreal_max)
+        class(waveguide_gpu_t), intent(inout) :: self
+        integer, intent(in) :: nreal_max
+        call check_status(rg_tprofile_reserve(self%plan, int(nreal_max, c_int32_t)))
+    end subroutine
+
+    !! Number of transmission eigenvalues of the aperture-filtered matrix (jmax-jmin+1).
+    integer function get_neigv_filtered_waveguide_gpu(self) result(nv)
+        class(waveguide_gpu_t), intent(in) :: self
+        nv = self%jmax - self%jmin + 1
+    end function
+
+    !! The data of compute_tprofile / compute_tprofile_group (src/waveguide.f90:807-997) for
+    !! realizations seed0 .. seed0+nseed-1: tvals(:, k) the eigenvalues of t^H t (descending),
+    !! intensity(:, ix, k) = |G(x_ix;1) s U(:,v)|^2 per eigenvector v; filtered: t over the apertures.
+    subroutine compute_tprofile_data_waveguide_gpu(self, seed0, nseed, filtered, tvals, intensity)
+        class(waveguide_gpu_t), intent(inout) :: self
+        integer, intent(in) :: seed0, nseed
+        logical, intent(in) :: filtered
+        real(wp), intent(inout), contiguous :: tvals(:, :), intensity(:, :, :)
+        integer(c_int32_t), allocatable :: st(:)
+        integer(c_int32_t) :: f
+        f = 0
+        if (filtered) f = 1
+        allocate(st(nseed))
+        call check_status(rg_tprofile(self%plan, int(seed0, c_int64_t), int(nseed, c_int32_t), f, tvals, intensity, st))
+    end subroutine
+
+    !! The field of compute_wavefunction (src/waveguide.f90:636-697): psi(iy, ix) = psi(x_ix, y_iy).
+    subroutine compute_wavefunction_field_waveguide_gpu(self, iseed, ipos, imode, psi)
+        class(waveguide_gpu_t), intent(inout) :: self
+        integer, intent(in) :: iseed, ipos, imode
+        complex(wp), intent(inout), contiguous :: psi(:, :)
+        integer(c_int32_t) :: st(1)
+        if (imode < 1 .or. self%nmu < imode) then
+            write (error_unit, '(3(a,i0),a)') "[ERROR] Invalid initial mode index, received imode=", imode, &
+                ", expected in 1..", self%nmu, ", aborting..."
+            stop errmsg_invalid_arg
+        else if (ipos < 1 .or. self%nx < ipos) then
+            write (error_unit, '(2(a,i0),a)') "[ERROR] Invalid initial position index, received ipos=", ipos, &
+                ", expected in 1..", self%nx, ", aborting..."
+            stop errmsg_invalid_arg
+        end if
+        call check_status(rg_tprofile_reserve(self%plan, 1_c_int32_t))
+        call check_status(rg_wavefunction(self%plan, int(iseed, c_int64_t), int(ipos, c_int32_t), int(imode, c_int32_t), &
+            psi, st))
+    end subroutine
+
+    !! The disorder upot(ix, iy) = U dx/k of realization `seed` (src/waveguide.f90:366-372 layout).
+    subroutine generate_potential_waveguide_gpu(self, seed, upot)
+        class(waveguide_gpu_t), intent(inout) :: self
+        integer, intent(in) :: seed
+        real(wp), intent(inout), contiguous :: upot(:, :)
+        call check_status(rg_generate_potential(self%plan, int(seed, c_int64_t), 1_c_int32_t, upot))
+    end subroutine
+
+    integer function get_ny_waveguide_gpu(self) result(ny)
+        class(waveguide_gpu_t), intent(in) :: self
+        ny = self%ny
+    end function
+
+    !! get_xmesh (src/waveguide.f90:319-323): x_i/L = aratio (i - 1/2 - nxfree)/nxdiso.
+    subroutine get_xmesh_waveguide_gpu(self, xmesh)
+        class(waveguide_gpu_t), intent(in) :: self
+        real(wp), intent(out) :: xmesh(:)
+        integer :: i
+        do i = 1, self%nx
+            xmesh(i) = self%params%aratio*(i - 0.5_wp - self%params%nxfree)/self%params%nxdiso
+        end do
     end subroutine
 
     subroutine check_refl_trans_shape(self, n1, n2, n3, tshape, nseed)
