@@ -233,6 +233,32 @@ def adjoint_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def tprofile_bench(args):
+    """Transmission eigenstates inside the medium (rg_tprofile, the data of compute_tprofile) for the
+    c4 waveguide by default: one complex128 pass over a batch of seeds, timed after a warmup pass,
+    host copies of T and of the per-slice intensities included."""
+    from recurgreen_amd import binding
+
+    params = workload_params(args)
+    B = args.batch
+    plan = binding.Plan(params, batch_max=B)
+    plan.tprofile_reserve(B)
+    plan.tprofile(None, seed0=1, nseed=B)
+    plan.synchronize()
+    t0 = time.perf_counter()
+    tv, it, st, code = plan.tprofile(None, seed0=1, nseed=B)
+    dt = time.perf_counter() - t0
+    n, nx = plan.nmu, plan.nx
+    flops = (32.0 * n ** 3 * nx) * B  # sweep (inversion + Dyson) and back substitution (two products) per slice
+    line = {"metric": "tprofile realizations/s (eigenvalues of t^H t and |G(x;1) s U(:,v)|^2 at every slice), complex128",
+            "value": B / dt, "unit": "realizations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+            "data": "synthetic", "failed_realizations": int((st != 0).sum()),
+            "config": {"workload": f"{args.config}: W={params['wol']} N={n} nx={nx} ny={params['ny']} batch {B}"},
+            "algorithmic_tflops": flops / dt / 1e12}
+    plan.close()
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(mib):
     """Plain device-to-device copies of `mib` MiB (torch's copy kernel): the reference kernel whose
     FETCH_SIZE / WRITE_SIZE counters scale the sweep kernels' counters to bytes."""
@@ -264,6 +290,8 @@ def main():
     ap.add_argument("--uneven", type=int, default=0, metavar="K",
                     help="measure an uneven batch of K realizations (rg_tdistrib_uneven) instead")
     ap.add_argument("--adjoint", action="store_true", help="time the adjoint (gradient) sweep instead (c4 by default)")
+    ap.add_argument("--tprofile", action="store_true",
+                    help="time the transmission-eigenstate pass (rg_tprofile) instead (c4 by default)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch options instead of the autotuner's (recurgreen_amd.autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -273,12 +301,15 @@ def main():
         print(f"bench.py measures one GPU only (realizations are independent and would shard by seed "
               f"range across GPUs); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
-    if args.adjoint and args.config == "n128":
+    if (args.adjoint or args.tprofile) and args.config == "n128":
         args.config = "c4"
     if args.batch is None:
         args.batch = CONFIGS[args.config][3]
     if args.adjoint:
         adjoint_bench(args)
+        return
+    if args.tprofile:
+        tprofile_bench(args)
         return
     if args.uneven:
         uneven_bench(args)
