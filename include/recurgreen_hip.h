@@ -151,7 +151,8 @@ rg_status rg_wavefunction_potential(rg_plan *plan, const double *upot, int32_t i
 /* Adjoint sweep (no counterpart in the reference: the reverse-mode derivative of the sweep of
  * src/waveguide.f90:427-445).  loss_kind 0: L = Tr(t^H t) = sum of the transmission eigenvalues;
  * loss_kind 1: L = T_max, the largest transmission eigenvalue (t over the filtered ranges, complex
- * sqrt(mu) as in :443; the derivative is that of a simple eigenvalue).  For each realization: loss[r] = L and
+ * sqrt(mu) as in :443; the derivative is that of a simple eigenvalue; its eigenvector comes from the
+ * eigensolver up to order 128 and from a one-sided Jacobi decomposition up to 256).  For each realization: loss[r] = L and
  * grad = dL/d upot with the potential layout (nxdiso, ny, nreal), units of upot (U dx / k; dL/dU =
  * grad * dx / k).  Complex128; slice Green functions are recomputed from checkpoints kept every
  * ceil(sqrt(nx)) slices.  rg_adjoint_reserve allocates the workspace for nreal_max realizations

@@ -145,6 +145,7 @@ struct rg_plan {
     double2 *d_ck = nullptr, *d_seg = nullptr, *d_adj = nullptr;
     double *d_loss = nullptr, *d_grad = nullptr;
     double2 *d_bconj = nullptr;  // conj(b) (np)
+    double2 *d_adj_work = nullptr;  // T_max above order 128: Jacobi work X and V (2 adj_cap np^2)
     // tprofile workspace (rg_tprofile_reserve), tp_cap realizations per pass: Gss and Gs1 of every
     // slice (nx slabs of tp_cap np^2 each), the SVD work X and V, W = s U, two products Y, the
     // eigenvalues (tp_cap x nmu) and the intensities (tp_cap x nx x nmu)
@@ -245,6 +246,11 @@ void rg_launch_tprofile_svd(rg_plan *P, int nb, const double2 *gs1, int i0, int 
                             double2 *vwork, double2 *w, double *tvals);
 void rg_launch_colnorms(rg_plan *P, int nb, const double2 *y, int nv, double *out, size_t ostride);
 void rg_launch_unit_column(rg_plan *P, double2 *w, int row);
+// top eigenvector of the T_max Gram matrix for orders up to 256 (work X, V: np^2 complex128 per realization)
+void rg_launch_topvec_jacobi(rg_plan *P, int nb, const double2 *gs1, double2 *xwork, double2 *vwork, double2 *xvec,
+                             int xstride);
+void rg_launch_topvec_jacobi(rg_plan *P, int nb, const float2 *gs1, double2 *xwork, double2 *vwork, double2 *xvec,
+                             int xstride);
 void rg_launch_modal_column(rg_plan *P, const double2 *y, double2 *psi);
 void rg_launch_add_c128(rg_plan *P, double2 *a, const double2 *b, size_t count);
 // rg_eig.hip

@@ -53,7 +53,7 @@ void free_plan(rg_plan *P)
                     P->d_gss32, P->d_gs1_32[0], P->d_gs1_32[1], P->d_g11, P->d_g1s[0], P->d_g1s[1],
                     P->d_sqrtremu, P->d_redo, P->d_us, P->d_ck, P->d_seg, P->d_adj, P->d_loss, P->d_grad, P->d_bconj,
                     P->d_schur, P->d_schur_zero, P->d_schur_i, P->d_tp_g, P->d_tp_z, P->d_tp_x, P->d_tp_v,
-                    P->d_tp_w, P->d_tp_y[0], P->d_tp_y[1], P->d_tp_t, P->d_tp_i, P->d_tp_psi};
+                    P->d_tp_w, P->d_tp_y[0], P->d_tp_y[1], P->d_tp_t, P->d_tp_i, P->d_tp_psi, P->d_adj_work};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_graphs(P);
@@ -707,10 +707,16 @@ void run_adjoint(rg_plan *P, int nb, int loss_kind, const double *d_upot, double
     }
     if (loss_kind == 0) {
         rg_launch_adj_terminal(P, nb, sa, d_loss, sbar);
-    } else {  // T_max: top eigenvector of the Gram matrix (128 complex128 slots per realization in T1)
+    } else {  // T_max: top eigenvector of the Gram matrix (128 or 256 complex128 slots per realization in T1)
         double2 *xv = reinterpret_cast<double2 *>(T1);
-        topvec(P, nb, sa, xv);
-        rg_launch_adj_terminal_tmax(P, nb, sa, xv, 128, d_loss, sbar);
+        if (rg_topvec_supported(P)) {
+            topvec(P, nb, sa, xv);
+            rg_launch_adj_terminal_tmax(P, nb, sa, xv, 128, d_loss, sbar);
+        } else {  // order 129..256: Jacobi with the eigenvectors
+            const size_t wmat = (size_t)P->adj_cap * P->np * P->np;
+            rg_launch_topvec_jacobi(P, nb, sa, P->d_adj_work, P->d_adj_work + wmat, xv, 256);
+            rg_launch_adj_terminal_tmax(P, nb, sa, xv, 256, d_loss, sbar);
+        }
     }
     (void)hipMemsetAsync(gbar, 0, sizeof(E) * mat, P->stream);
     (void)hipMemsetAsync(d_grad, 0, sizeof(double) * (size_t)nb * P->nxdiso * P->ny, P->stream);
@@ -1410,9 +1416,10 @@ rg_status rg_adjoint_reserve(rg_plan *P, int32_t nreal_max)
     if (P->np > 256) return RG_ENOTSUP;
     DeviceGuard g(P->device);
     if (P->adj_cap >= nreal_max) return RG_OK;
-    for (void *b : {(void *)P->d_ck, (void *)P->d_seg, (void *)P->d_adj, (void *)P->d_loss, (void *)P->d_grad})
+    for (void *b : {(void *)P->d_ck, (void *)P->d_seg, (void *)P->d_adj, (void *)P->d_loss, (void *)P->d_grad,
+                    (void *)P->d_adj_work})
         if (b) (void)hipFree(b);
-    P->d_ck = P->d_seg = P->d_adj = nullptr;
+    P->d_ck = P->d_seg = P->d_adj = P->d_adj_work = nullptr;
     P->d_loss = P->d_grad = nullptr;
     P->adj_cap = 0;
     const int nx = P->nx;
@@ -1426,6 +1433,7 @@ rg_status rg_adjoint_reserve(rg_plan *P, int32_t nreal_max)
         (st = dalloc(&P->d_loss, (size_t)nreal_max)) ||
         (st = dalloc(&P->d_grad, (size_t)nreal_max * P->nxdiso * P->ny)))
         return st;
+    if (!rg_topvec_supported(P) && (st = dalloc(&P->d_adj_work, 2 * cmat))) return st;  // T_max above order 128
     P->adj_cap = nreal_max;
     P->adj_seg = seg;
     P->adj_nck = nck;
@@ -1437,7 +1445,7 @@ static rg_status adjoint_check(rg_plan *P, int32_t nreal, int32_t loss_kind)
     rg_status st = check_common(P, nreal, RG_FP64);
     if (st) return st;
     if (loss_kind != 0 && loss_kind != 1) return RG_EINVAL;
-    if (loss_kind == 1 && !rg_topvec_supported(P)) return RG_ENOTSUP;
+    if (loss_kind == 1 && !rg_eig_supported(P)) return RG_ENOTSUP;  // Gram order above 256
     if (P->adj_cap < 1) return RG_EINVAL;  // rg_adjoint_reserve first
     return RG_OK;
 }

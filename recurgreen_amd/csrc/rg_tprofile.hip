@@ -198,9 +198,132 @@ __global__ __launch_bounds__(256) void k_modal_column(const double2 *__restrict_
     psi[iy] = acc;
 }
 
+// Top eigenvector of the Gram matrix of t = 2i s G s over the filtered ranges with the complex
+// sqrt(mu) of src/waveguide.f90:443 (the adjoint of T_max): t^H t when t has no more columns than rows,
+// else t t^H, as the eigenvalue kernel orients it.  One-sided Jacobi on the columns of t (or of t^H)
+// with the rotations accumulated into V; the column of largest norm gives the eigenvector, written
+// (unit norm, n complex) to xvec[b * xstride ..].  For orders the LDS-resident eigensolver does not hold.
+template <class E>
+__global__ __launch_bounds__(256) void k_topvec_jacobi(const E *__restrict__ gs1, int NP, int imin, int imax, int jmin,
+                                                       int jmax, const double2 *__restrict__ sqrtmu,
+                                                       double2 *__restrict__ xwork, double2 *__restrict__ vwork,
+                                                       double2 *__restrict__ xvec, int xstride,
+                                                       int32_t *__restrict__ status, int max_sweeps)
+{
+    __shared__ int s_rot;
+    __shared__ double s_t[TP_MAXN];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int mr = imax - imin + 1, mc = jmax - jmin + 1;
+    const bool gram_cols = mc <= mr;
+    const int n = gram_cols ? mc : mr, rows = gram_cols ? mr : mc;  // columns of the decomposed matrix
+    const int cols_e = n + (n & 1);
+    const E *G = gs1 + (size_t)b * NP * NP;
+    double2 *X = xwork + (size_t)b * NP * NP;
+    double2 *V = vwork + (size_t)b * NP * NP;
+    for (int e = tid; e < cols_e * NP; e += blockDim.x) {
+        const int j = e / NP, i = e % NP;
+        double2 t = cmk(0.0, 0.0);
+        if (i < rows && j < n) {
+            const int a = gram_cols ? i : j, c = gram_cols ? j : i;  // element (a, c) of t
+            const int ii = imin - 1 + a, jj = jmin - 1 + c;
+            const double2 v = cmul(cmul(sqrtmu[ii], rg_mt<E>::ld(G[(size_t)ii * NP + jj])), sqrtmu[jj]);
+            t = cmk(-2.0 * v.y, 2.0 * v.x);
+            if (!gram_cols) t = cconj(t);  // columns of t^H
+        }
+        X[(size_t)j * NP + i] = t;
+        if (i < cols_e) V[(size_t)j * NP + i] = cmk(i == j ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+    const double tol = 2.220446049250313e-16 * sqrt((double)rows);
+    const int grp = tid >> 4, lane = tid & 15, ngrp = blockDim.x >> 4;
+    const int last = cols_e - 1;
+    int sweep = 0;
+    for (; sweep < max_sweeps; sweep++) {
+        if (tid == 0) s_rot = 0;
+        __syncthreads();
+        for (int round = 0; round < last; round++) {
+            for (int pr = grp; pr < cols_e / 2; pr += ngrp) {
+                int p, q;
+                tp_pair(round, pr, last, p, q);
+                double2 *ap = X + (size_t)p * NP, *aq = X + (size_t)q * NP;
+                double al = 0.0, be = 0.0, gx = 0.0, gy = 0.0;
+                for (int i = lane; i < rows; i += 16) {
+                    const double2 x = ap[i], y = aq[i];
+                    al = fma(x.x, x.x, fma(x.y, x.y, al));
+                    be = fma(y.x, y.x, fma(y.y, y.y, be));
+                    gx = fma(x.x, y.x, fma(x.y, y.y, gx));
+                    gy = fma(x.x, y.y, fma(-x.y, y.x, gy));
+                }
+#pragma unroll
+                for (int off = 8; off >= 1; off >>= 1) {
+                    al += __shfl_xor(al, off, 16);
+                    be += __shfl_xor(be, off, 16);
+                    gx += __shfl_xor(gx, off, 16);
+                    gy += __shfl_xor(gy, off, 16);
+                }
+                const double g = sqrt(gx * gx + gy * gy);
+                if (g == 0.0 || g <= tol * sqrt(al * be)) continue;
+                if (lane == 0) s_rot = 1;
+                const double zeta = (be - al) / (2.0 * g);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+                const double2 e = cmk(gx / g, -gy / g);
+                for (int i = lane; i < rows; i += 16) {
+                    const double2 x = ap[i], y = cmul(e, aq[i]);
+                    ap[i] = cmk(c * x.x - sn * y.x, c * x.y - sn * y.y);
+                    aq[i] = cmk(sn * x.x + c * y.x, sn * x.y + c * y.y);
+                }
+                double2 *vp = V + (size_t)p * NP, *vq = V + (size_t)q * NP;
+                for (int i = lane; i < cols_e; i += 16) {
+                    const double2 x = vp[i], y = cmul(e, vq[i]);
+                    vp[i] = cmk(c * x.x - sn * y.x, c * x.y - sn * y.y);
+                    vq[i] = cmk(sn * x.x + c * y.x, sn * x.y + c * y.y);
+                }
+            }
+            __syncthreads();
+        }
+        if (!s_rot) break;
+        __syncthreads();
+    }
+    for (int j = grp; j < n; j += ngrp) {
+        double s2 = 0.0;
+        const double2 *aj = X + (size_t)j * NP;
+        for (int i = lane; i < rows; i += 16) s2 = fma(aj[i].x, aj[i].x, fma(aj[i].y, aj[i].y, s2));
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1) s2 += __shfl_xor(s2, off, 16);
+        if (lane == 0) s_t[j] = s2;
+    }
+    __syncthreads();
+    int top = 0;  // every thread finds the same largest column (lowest index on ties)
+    for (int j = 1; j < n; j++)
+        if (s_t[j] > s_t[top]) top = j;
+    for (int i = tid; i < n; i += blockDim.x) xvec[(size_t)b * xstride + i] = V[(size_t)top * NP + i];
+    if (tid == 0 && (sweep == max_sweeps || !(s_t[top] == s_t[top]))) status[b] = RG_ENOCONV;
+}
+
 }  // namespace
 
 bool rg_tprofile_supported(int n) { return n <= TP_MAXN; }
+
+template <class E>
+static void launch_topvec_jacobi(rg_plan *P, int nb, const E *gs1, double2 *xwork, double2 *vwork, double2 *xvec,
+                                 int xstride)
+{
+    hipLaunchKernelGGL(k_topvec_jacobi<E>, dim3(nb), dim3(256), 0, P->stream, gs1, P->np, P->imin, P->imax, P->jmin,
+                       P->jmax, P->d_sqrtmu, xwork, vwork, xvec, xstride, P->d_status, 60);
+}
+
+void rg_launch_topvec_jacobi(rg_plan *P, int nb, const double2 *gs1, double2 *xwork, double2 *vwork, double2 *xvec,
+                             int xstride)
+{
+    launch_topvec_jacobi(P, nb, gs1, xwork, vwork, xvec, xstride);
+}
+
+void rg_launch_topvec_jacobi(rg_plan *P, int nb, const float2 *gs1, double2 *xwork, double2 *vwork, double2 *xvec,
+                             int xstride)
+{
+    launch_topvec_jacobi(P, nb, gs1, xwork, vwork, xvec, xstride);
+}
 
 void rg_launch_unit_column(rg_plan *P, double2 *w, int row)
 {
