@@ -190,6 +190,7 @@ class Plan:
         check(L.rg_plan_create(C.byref(p), int(batch_max), int(device), C.byref(h)))
         self._h = h
         self.batch_max = int(batch_max)
+        self.device = int(device)
         d = (C.c_int32 * 11)()
         v = (C.c_double * 4)()
         check(L.rg_plan_dims(self._h, d, v))
@@ -292,6 +293,22 @@ class Plan:
     def tprofile_reserve(self, nreal_max: int) -> None:
         check(lib().rg_tprofile_reserve(self._h, int(nreal_max)))
 
+    def tprofile_pass_size(self, nreal: int, fraction: float = 0.6) -> int:
+        """Realizations per tprofile pass: at most nreal and batch_max, and as many as fit in `fraction`
+        of the device's free memory (every slice's Gss and Gs1 plus five work matrices, complex128,
+        and the intensities per realization)."""
+        np_ = (self.nmu + 15) // 16 * 16
+        per = (2 * self.nx + 5) * np_ * np_ * 16 + self.nx * self.nmu * 8
+        cap = max(1, min(int(nreal), self.batch_max))
+        try:
+            import torch
+
+            free = torch.cuda.mem_get_info(self.device)[0]
+            cap = max(1, min(cap, int(fraction * free) // per))
+        except Exception:
+            pass
+        return cap
+
     def tprofile(self, upot: np.ndarray | None = None, seed0: int = 1, nseed: int = 1, filtered: bool = False,
                  reserve: int | None = None):
         """(tvals (nreal, nv) descending, intensity (nreal, nx, nv), status, code): the eigenvalues of
@@ -304,7 +321,7 @@ class Plan:
             nreal = 1 if u.ndim == 2 else u.shape[2]
         else:
             u, nreal = None, int(nseed)
-        self.tprofile_reserve(max(1, min(nreal, reserve or self.batch_max)))
+        self.tprofile_reserve(max(1, reserve or self.tprofile_pass_size(nreal)))
         nv = (self.jmax - self.jmin + 1) if filtered else self.nmu
         tv = np.zeros((nreal, nv))
         it = np.zeros((nreal, self.nx, nv))
