@@ -190,7 +190,9 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  * recurgreen_amd.autotune).  Keys and values:
  *   "inverse_fp64"  0 register-resident Gauss-Jordan, 1 look-ahead Gauss-Jordan, 2 blocked LU;
  *   "inverse_mixed" 1 look-ahead, 2 blocked, 3 verified diagonal pivots (default);
- *   "dyson"         0 64x64 output tiles, 1 one workgroup per realization (default, NP >= 80);
+ *   "dyson"         complex64 Dyson product: 0 64x64 output tiles, 1 one workgroup per realization
+ *                   (default, NP 80..128);
+ *   "dyson_fp64"    the same choice for the complex128 product (default 1);
  *   "project_parts" 1..8 partial sums of the projection over transverse cells;
  *   "lanes"         1 one stream, 2 the batch split in halves on two streams;
  *   "eigen"         0 Hermitian eigensolver of t^H t (default), 1 one-sided Jacobi SVD of t;

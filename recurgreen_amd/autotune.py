@@ -27,7 +27,7 @@ from .binding import Plan
 CACHE = os.environ.get("RG_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_cache.json"))
 
 CANDIDATES = {
-    "fp64": [("inverse_fp64", (0, 1, 2)), ("project_parts", (1, 2, 4, 8))],
+    "fp64": [("inverse_fp64", (0, 1, 2)), ("dyson_fp64", (1, 0)), ("project_parts", (1, 2, 4, 8))],
     "mixed_fp32": [("inverse_mixed", (3, 1, 2)), ("dyson", (1, 0)), ("project_parts", (1, 2, 4, 8))],
     "mixed_bf16x3": [("inverse_mixed", (3, 1, 2)), ("project_parts", (1, 2, 4, 8))],
 }
@@ -83,7 +83,7 @@ def tune(params: dict, batch: int, precision: str, probe_slices: int = 20, reps:
     try:
         chosen, times = {}, {}
         for key, values in CANDIDATES[precision]:
-            if key == "dyson" and probe.nmu and (probe.nmu + 15) // 16 * 16 < 80:
+            if key in ("dyson", "dyson_fp64") and probe.nmu and (probe.nmu + 15) // 16 * 16 < 80:
                 continue  # only the 64 x 64 kernel exists below NP = 80
             res = {}
             for v in values:

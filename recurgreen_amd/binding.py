@@ -418,7 +418,7 @@ class Plan:
         check(lib().rg_plan_synchronize(self._h))
 
     # -- launch options (the autotuner's choices) -------------------------------------------
-    OPTIONS = ("inverse_fp64", "inverse_mixed", "dyson", "project_parts", "lanes", "eigen", "graph",
+    OPTIONS = ("inverse_fp64", "inverse_mixed", "dyson", "dyson_fp64", "project_parts", "lanes", "eigen", "graph",
                "adjoint_precision")
 
     def set_option(self, key: str, value: int) -> None:

@@ -1,0 +1,152 @@
+// rg_dyson64.hip -- the Dyson product of the complex128 sweep, one realization per workgroup (gfx950).
+//
+// Gs1_new = -Gss diag(b) Gs1_old (src/waveguide.f90:438) for NP = 80..128 on v_mfma_f64_16x16x4_f64.
+// One workgroup of 8 wavefronts computes the whole NP x NP product of one realization, so Gss and Gs1
+// are each read from HBM once (the 64 x 64 tiling reads them twice).  Wavefront w owns row tiles
+// {w/2, w/2 + 4} and column tiles {w%2, w%2 + 2, w%2 + 4, w%2 + 6}.  k runs in chunks of 16: the next
+// chunk is loaded into registers (one complex per 16-byte load) while the current one is multiplied
+// out of LDS, then stored to the other LDS buffer -- one barrier per chunk.  b scales the rows of
+// Gs1_old in FP64 on the way into LDS.  133 KiB of LDS: one workgroup per CU, 256 registers a lane.
+//
+// LDS planes are [k][row] (A, transposed on the way in) and [k][col] (B), row length NP + 2 doubles,
+// so that the 16 k values of one row written by 16 lanes fall in distinct bank pairs.
+// MFMA lane layout (v_mfma_f64_16x16x4_f64): A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// result register v of lane l holds D[4v + l/16][l%16].
+#include "rg_internal.h"
+
+namespace {
+
+typedef double dy64_d4 __attribute__((ext_vector_type(4)));
+
+template <int NP>
+struct DysonShared64 {
+    static constexpr int S = NP + 2;
+    double a[2][2][16][S];  // [buffer][re, im][k][row]
+    double b[2][2][16][S];  // [buffer][re, im][k][col]
+};
+
+template <int NP>
+__global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__restrict__ gss,
+                                                            const double2 *__restrict__ gs1, double2 *__restrict__ out,
+                                                            const double2 *__restrict__ bvec, int n, int cs)
+{
+    bvec += (size_t)blockIdx.x * cs;  // cs = np: constants of each realization, else shared
+    constexpr int T = NP / 16, NK = NP / 16, NQ = NP * 16;  // tiles, k chunks, complex per operand chunk
+    constexpr int LOADS = (NQ + 511) / 512;
+    __shared__ DysonShared64<NP> sh;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1, li = lane & 15, lg = lane >> 4;
+    const size_t base = (size_t)blockIdx.x * NP * NP;
+    const double2 *A = gss + base, *B = gs1 + base;
+
+    double2 ra[LOADS], rb[LOADS];
+    // chunk kc: A row idx >> 4, k idx & 15 (16 k of a row contiguous); B k row idx / NP, column idx % NP
+    auto load = [&](int kc) __attribute__((always_inline)) {
+#pragma unroll
+        for (int e = 0; e < LOADS; e++) {
+            const int idx = tid + 512 * e;
+            if (idx < NQ) {
+                ra[e] = A[(size_t)(idx >> 4) * NP + kc * 16 + (idx & 15)];
+                rb[e] = B[(size_t)(kc * 16 + idx / NP) * NP + idx % NP];
+            }
+        }
+    };
+    auto store = [&](int kc, int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int e = 0; e < LOADS; e++) {
+            const int idx = tid + 512 * e;
+            if (idx < NQ) {
+                const int row = idx >> 4, k = idx & 15;
+                sh.a[buf][0][k][row] = ra[e].x;
+                sh.a[buf][1][k][row] = ra[e].y;
+                const int kk = idx / NP, col = idx % NP, gk = kc * 16 + kk;
+                const double2 bk = gk < n ? bvec[gk] : cmk(0.0, 0.0);
+                sh.b[buf][0][kk][col] = bk.x * rb[e].x - bk.y * rb[e].y;
+                sh.b[buf][1][kk][col] = bk.x * rb[e].y + bk.y * rb[e].x;
+            }
+        }
+    };
+
+    dy64_d4 cr[2][4], ci[2][4];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            cr[x][y] = dy64_d4{0.0, 0.0, 0.0, 0.0};
+            ci[x][y] = dy64_d4{0.0, 0.0, 0.0, 0.0};
+        }
+
+    load(0);
+    store(0, 0);
+    __syncthreads();
+    for (int kc = 0; kc < NK; kc++) {
+        const int buf = kc & 1;
+        if (kc + 1 < NK) load(kc + 1);
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            const int kr = 4 * ks + lg;
+            double ar[2], ai[2], nai[2], br[4], bi[4];
+#pragma unroll
+            for (int x = 0; x < 2; x++) {
+                const int rt = wr + 4 * x;
+                ar[x] = rt < T ? sh.a[buf][0][kr][16 * rt + li] : 0.0;
+                ai[x] = rt < T ? sh.a[buf][1][kr][16 * rt + li] : 0.0;
+                nai[x] = -ai[x];
+            }
+#pragma unroll
+            for (int y = 0; y < 4; y++) {
+                const int ct = wc + 2 * y;
+                br[y] = ct < T ? sh.b[buf][0][kr][16 * ct + li] : 0.0;
+                bi[y] = ct < T ? sh.b[buf][1][kr][16 * ct + li] : 0.0;
+            }
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++) {
+                    if (wr + 4 * x >= T || wc + 2 * y >= T) continue;  // uniform per wavefront
+                    cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], cr[x][y], 0, 0, 0);
+                    cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[x], bi[y], cr[x][y], 0, 0, 0);
+                    ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], ci[x][y], 0, 0, 0);
+                    ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], ci[x][y], 0, 0, 0);
+                }
+        }
+        if (kc + 1 < NK) store(kc + 1, buf ^ 1);
+        __syncthreads();
+    }
+    double2 *C = out + base;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const int rt = wr + 4 * x, ct = wc + 2 * y;
+            if (rt >= T || ct >= T) continue;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                C[(size_t)(16 * rt + 4 * v + lg) * NP + 16 * ct + li] = cmk(-cr[x][y][v], -ci[x][y][v]);
+        }
+}
+
+}  // namespace
+
+bool rg_launch_gemm_wg64(rg_plan *P, int nb, const double2 *gss, const double2 *gs1_old, double2 *gs1_new)
+{
+    switch (P->np) {
+        case 80:
+            hipLaunchKernelGGL(k_dyson_gemm_wg64<80>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b,
+                               P->nmu, P->cstride);
+            return true;
+        case 96:
+            hipLaunchKernelGGL(k_dyson_gemm_wg64<96>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b,
+                               P->nmu, P->cstride);
+            return true;
+        case 112:
+            hipLaunchKernelGGL(k_dyson_gemm_wg64<112>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b,
+                               P->nmu, P->cstride);
+            return true;
+        case 128:
+            hipLaunchKernelGGL(k_dyson_gemm_wg64<128>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b,
+                               P->nmu, P->cstride);
+            return true;
+        default: return false;
+    }
+}

@@ -197,6 +197,9 @@ struct rg_plan {
     // Concurrent lanes: the batch split in halves, each swept on a stream of its own by a view of
     // this plan (same buffers, re-based pointers).  lanes = 1: one stream.
     int lanes = 1;
+    // complex128 Dyson product: 1 one workgroup per realization at NP 80..128 (default), 0 the
+    // 64 x 64-tiled kernel (option dyson_fp64)
+    int dyson64 = 1;
     rg_plan *lane[2] = {nullptr, nullptr};
     hipEvent_t lane_ev[3] = {nullptr, nullptr, nullptr};
 };
@@ -276,6 +279,8 @@ size_t rg_schur_elems(int np);
 // rg_dyson32.hip: the FP32 Dyson product with one realization per workgroup; false if NP is not
 // one of 80, 96, 112, 128
 bool rg_launch_gemm_wg(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new);
+// rg_dyson64.hip: the complex128 product, one workgroup per realization (NP 80..128; false otherwise)
+bool rg_launch_gemm_wg64(rg_plan *P, int nb, const double2 *gss, const double2 *gs1_old, double2 *gs1_new);
 void rg_launch_copy_f2(rg_plan *P, const float2 *src, float2 *dst, size_t n);
 void rg_launch_histogram(rg_plan *P, const double *d_samples, int64_t n, const double *d_bins, int nbins,
                          unsigned long long *d_counts);

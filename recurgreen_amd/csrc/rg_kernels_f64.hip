@@ -596,6 +596,10 @@ void rg_launch_gemm_f64(rg_plan *P, int nb, const double2 *gss, const double2 *g
 {
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_GEMM, &stop);
+    if (P->dyson64 && rg_launch_gemm_wg64(P, nb, gss, gs1_old, gs1_new)) {
+        rg_timer_end(P, RGK_GEMM, stop);
+        return;
+    }
     const int np = P->np;
     const int tt = (np + 63) / 64;
 #define RG_GEMM_CASE(T)                                                                                     \

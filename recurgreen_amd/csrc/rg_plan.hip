@@ -328,6 +328,8 @@ static void lane_view(const rg_plan *P, rg_plan *L, int off, int nb)
     L->d_stdu_r = P->d_stdu_r ? P->d_stdu_r + off : nullptr;
     L->inv_fp64 = P->inv_fp64;
     L->inv_mixed = P->inv_mixed;
+    L->gemm_wg = P->gemm_wg;
+    L->dyson64 = P->dyson64;
     L->eig_jacobi = P->eig_jacobi;
     L->profiling = P->profiling;
     L->profile_mask = P->profile_mask;
@@ -341,7 +343,8 @@ __global__ void k_set_seed(int64_t *__restrict__ dst, int64_t v) { *dst = v; }
 // Options that change the kernels a sweep launches (part of a captured sweep's key).
 static int graph_opts(const rg_plan *P)
 {
-    return P->inv_fp64 | P->inv_mixed << 4 | (P->gemm_wg ? 1 : 0) << 8 | P->cq_parts << 9 | (P->eig_jacobi ? 1 : 0) << 13;
+    return P->inv_fp64 | P->inv_mixed << 4 | (P->gemm_wg ? 1 : 0) << 8 | P->cq_parts << 9 | (P->eig_jacobi ? 1 : 0) << 13 |
+           P->dyson64 << 14;
 }
 
 void free_graphs(rg_plan *P)
@@ -1585,6 +1588,9 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
     } else if (std::strcmp(key, "dyson") == 0) {
         if (value != 0 && value != 1) return RG_EINVAL;
         P->gemm_wg = value == 1;
+    } else if (std::strcmp(key, "dyson_fp64") == 0) {
+        if (value != 0 && value != 1) return RG_EINVAL;
+        P->dyson64 = value;
     } else if (std::strcmp(key, "project_parts") == 0) {
         if (value < 1 || value > kMaxCqParts) return RG_EINVAL;
         P->cq_parts = value;
@@ -1617,6 +1623,7 @@ rg_status rg_plan_get_option(const rg_plan *P, const char *key, int32_t *value)
     if (!P || !key || !value) return RG_EINVAL;
     if (std::strcmp(key, "inverse_fp64") == 0) *value = P->inv_fp64;
     else if (std::strcmp(key, "inverse_mixed") == 0) *value = P->inv_mixed;
+    else if (std::strcmp(key, "dyson_fp64") == 0) *value = P->dyson64;
     else if (std::strcmp(key, "dyson") == 0) *value = P->gemm_wg ? 1 : 0;
     else if (std::strcmp(key, "project_parts") == 0) *value = P->cq_parts;
     else if (std::strcmp(key, "lanes") == 0) *value = P->lanes;
