@@ -111,7 +111,7 @@ rg_status rg_refl_trans(rg_plan *plan, int64_t seed0, int32_t nseed, double *rma
 /* Same for explicit host potentials upot (nxdiso*ny*nreal doubles). */
 rg_status rg_refl_trans_potential(rg_plan *plan, const double *upot, int32_t nreal, double *rmat, double *tmat,
                                   int32_t *status);
-/* Mirrors compute_deposition (src/waveguide.f90:712-750) for seeds seed0 .. seed0+nseed-1: the
+/* Mirrors compute_deposition (src/waveguide.f90:710-745) for seeds seed0 .. seed0+nseed-1: the
  * deposition matrix dep = 2i s G(fpos; ipos) s, s = sqrt(Re mu) (:741-742), full nmu x nmu,
  * column-major complex, realization-major in depmat (host, nseed*nmu*nmu complex).  fpos, ipos in
  * 1..nx (RG_EINVAL otherwise); ipos = 1 only, the input lead's side (RG_ENOTSUP otherwise): G(fpos; 1)
@@ -123,7 +123,7 @@ rg_status rg_deposition(rg_plan *plan, int64_t seed0, int32_t nseed, int32_t fpo
 rg_status rg_deposition_potential(rg_plan *plan, const double *upot, int32_t nreal, int32_t fpos, int32_t ipos,
                                   double *depmat, int32_t *status);
 /* Transmission eigenstates inside the medium, the data of compute_tprofile and
- * compute_tprofile_group (src/waveguide.f90:807-997) for seeds seed0 .. seed0+nseed-1, complex128.
+ * compute_tprofile_group (src/waveguide.f90:843-997) for seeds seed0 .. seed0+nseed-1, complex128.
  * t = 2i s G(L;0) s, s = sqrt(Re mu): the full nmu x nmu matrix (compute_tprofile, filtered = 0) or
  * rows imin..imax and columns jmin..jmax (compute_tprofile_group, filtered = 1; RG_EINVAL if it has
  * more columns than rows, as diagonalize_ahca's size check).  With nv its number of columns, tvals

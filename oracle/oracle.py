@@ -223,7 +223,7 @@ class Waveguide:
         return r, t
 
     def deposition(self, upot, fpos: int, ipos: int = 1) -> np.ndarray:
-        """depmat (nmu, nmu) of compute_deposition (src/waveguide.f90:712-750), 1-based positions."""
+        """depmat (nmu, nmu) of compute_deposition (src/waveguide.f90:710-745), 1-based positions."""
         u = self._upot(upot)
         d = np.zeros((self.nmu, self.nmu), complex)
         st = lib().rgo_deposition(self._h, _dp(u) if u is not None else None, int(fpos), int(ipos), _vp(d))

@@ -13,8 +13,8 @@
  *   - r/t sweep, version 1        src/waveguide.f90:469-519  (rgo_refl_trans_v1)
  *   - r/t by block-tridiag solve  src/waveguide.f90:526-553,
  *                                 src/base_utils.f90:296-335 (rgo_refl_trans_v2)
- *   - deposition matrix           src/waveguide.f90:712-750  (rgo_deposition)
- *   - Green function column block src/waveguide.f90:842-848  (rgo_green_column)
+ *   - deposition matrix           src/waveguide.f90:710-745  (rgo_deposition)
+ *   - Green function column block src/waveguide.f90:873-878  (rgo_green_column)
  *   - zgetrf+zgetri inverse       src/base_utils.f90:235-274 (rgo_inverse)
  *   - zgesvd singular values      src/base_utils.f90:341-380 (rgo_singular_values,
  *                                 one-sided Jacobi instead of LAPACK)
@@ -673,7 +673,7 @@ int rgo_refl_trans_v2(const rgo_wg *w, const double *upot, zc *r, zc *t)
     return st;
 }
 
-/* Deposition matrix (src/waveguide.f90:712-750): the column block ipos of H^-1 by the block-tridiagonal
+/* Deposition matrix (src/waveguide.f90:710-745): the column block ipos of H^-1 by the block-tridiagonal
  * solve with the identity at block ipos (:736-739), then dep = 2i s G(fpos; ipos) s with s = sqrt(Re mu)
  * (:741-742).  fpos, ipos in 1..nx. */
 int rgo_deposition(const rgo_wg *w, const double *upot, int fpos, int ipos, zc *dep)
@@ -702,7 +702,7 @@ int rgo_deposition(const rgo_wg *w, const double *upot, int fpos, int ipos, zc *
 }
 
 /* Column block ipos of H^-1 for every slice, green[x] = G(x; ipos) (nx blocks of nmu x nmu, row-major),
- * by the block-tridiagonal solve of compute_tprofile (src/waveguide.f90:842-848). */
+ * by the block-tridiagonal solve of compute_tprofile (src/waveguide.f90:873-878). */
 int rgo_green_column(const rgo_wg *w, const double *upot, int ipos, zc *green)
 {
     int n = w->nmu, nx = w->nx;

@@ -272,7 +272,7 @@ class Plan:
         return (np.ascontiguousarray(r.transpose(0, 2, 1)), np.ascontiguousarray(t.transpose(0, 2, 1)), st, code)
 
     def deposition(self, fpos: int, ipos: int = 1, upot: np.ndarray | None = None, seed0: int = 1, nseed: int = 1):
-        """(dep, status, code) of compute_deposition (src/waveguide.f90:712-750), complex128: dep is
+        """(dep, status, code) of compute_deposition (src/waveguide.f90:710-745), complex128: dep is
         (nreal, nmu, nmu), 1-based slice positions; explicit potentials upot(ix, iy[, r]) or seeds."""
         n = self.nmu
         if upot is not None:
@@ -313,7 +313,7 @@ class Plan:
                  reserve: int | None = None):
         """(tvals (nreal, nv) descending, intensity (nreal, nx, nv), status, code): the eigenvalues of
         t^H t and |G(x;1) s U(:,v)|^2 per slice and eigenvector (compute_tprofile,
-        src/waveguide.f90:807-875, before the window sum; filtered: t over the apertures as
+        src/waveguide.f90:843-907, before the window sum; filtered: t over the apertures as
         compute_tprofile_group, :951-954).  Reserves the workspace for min(nreal, reserve or
         batch_max) realizations per pass."""
         if upot is not None:

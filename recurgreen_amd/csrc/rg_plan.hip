@@ -480,7 +480,7 @@ double2 *run_sweep_refl(rg_plan *P, int nb, int64_t seed0, const double *d_upot,
     return P->d_gs1[cur ^ 1];
 }
 
-// compute_deposition (src/waveguide.f90:712-750) for ipos = 1, by recursion instead of the reference's
+// compute_deposition (src/waveguide.f90:710-745) for ipos = 1, by recursion instead of the reference's
 // block-tridiagonal solve: with the right part (slices fpos+1..nx and the right lead) folded into the
 // right-connected Green function h = (A_x - V_x - B h' B)^-1 (a sweep from slice nx down, the same
 // kernels), G(fpos; fpos) = (A_f - V_f - B (g + h) B)^-1 with g = Gss_(fpos-1) of the forward sweep,
@@ -534,7 +534,7 @@ void run_sweep_deposition(rg_plan *P, int nb, int64_t seed0, const double *d_upo
     rg_launch_deposition_out(P, nb, gf1, dep_out);
 }
 
-// Transmission eigenstates inside the medium (compute_tprofile, src/waveguide.f90:807-875), complex128:
+// Transmission eigenstates inside the medium (compute_tprofile, src/waveguide.f90:843-907), complex128:
 // the transmission sweep keeps Gss_x (g_x) and Gs1_x (z_x) of every slice; t = 2i s z_nx s is
 // decomposed as t^H t = U diag(T) U^H; with W = s U the column block Y_x = G(x;1) W follows from the
 // back substitution of the block-tridiagonal solve (src/base_utils.f90:330-333),

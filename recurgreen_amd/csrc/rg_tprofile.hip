@@ -1,6 +1,6 @@
 // rg_tprofile.hip -- transmission eigenvectors and eigenstate intensities for the tprofile tasks (gfx950).
 //
-// compute_tprofile / compute_tprofile_group (src/waveguide.f90:807-997) need, per realization, the
+// compute_tprofile / compute_tprofile_group (src/waveguide.f90:843-997) need, per realization, the
 // eigen-decomposition t^H t = U diag(T) U^H of the transmission matrix t = 2i s G(L;0) s,
 // s = sqrt(Re mu), over the rows imin.. and columns jmin.. of a range (the full nmu x nmu matrix for
 // compute_tprofile, the apertures for the group variant; diagonalize_ahca, src/base_utils.f90:394-443:

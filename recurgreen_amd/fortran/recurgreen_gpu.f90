@@ -193,12 +193,12 @@ contains
         call wguide%init(boundtype, nxdiso, nxfree, wol, exfac, aratio, dscat, dabso, napera, naperb, &
                          batch_max=max(1, min(batch, nseed)), device=device)
         nmu = wguide%get_nmu()
-        if (boundtype == "periodic") then   !! imode=1 is the frontal mode (src/main.f90:768-772)
+        if (boundtype == "periodic") then   !! imode=1 is the frontal mode (src/main.f90:772-776)
             imu = mod(imode + nmu/2 - 1, nmu) + 1
         else
             imu = imode
         end if
-        ipos = 1                        !! deposition: from the first slice (src/main.f90:764)
+        ipos = 1                        !! deposition: from the first slice (src/main.f90:769)
         fpos = nxfree + nxdiso/2 + 1    !! to the middle of the medium (:765)
         if (probatype /= "total" .and. (imu < 1 .or. imu > nmu)) then
             write (error_unit, '(2(a,i0),a)') "[ERROR] Invalid incident mode imode=", imode, ", expected in [1, ", nmu, &
@@ -371,7 +371,7 @@ contains
         do off = 0, nseed - 1, bsz
             nb = min(bsz, nseed - off)
             call wguide%compute_tprofile_data(1 + off, nb, .false., tvals(:, 1:nb), intens(:, :, 1:nb))
-            do k = 1, nb   !! compute_tprofile, step 7 (src/waveguide.f90:858-872)
+            do k = 1, nb   !! compute_tprofile, step 7 (src/waveguide.f90:887-903)
                 do v = 1, nmu
                     if (tmin <= tvals(v, k) .and. tvals(v, k) <= tmax) then
                         prof(off + k, :) = prof(off + k, :) + intens(v, :, k)
@@ -461,7 +461,7 @@ contains
         do off = 0, nseed - 1, bsz
             nb = min(bsz, nseed - off)
             call wguide%compute_tprofile_data(1 + off, nb, .true., tvals(:, 1:nb), intens(:, :, 1:nb))
-            do k = 1, nb   !! compute_tprofile_group, steps 6-7 (src/waveguide.f90:958-985)
+            do k = 1, nb   !! compute_tprofile_group, steps 6-7 (src/waveguide.f90:967-990)
                 tavgdata(off + k) = sum(tvals(:, k))/nv
                 prof = 0.0_wp
                 ntk = 0

@@ -299,7 +299,7 @@ contains
         call check_status(rg_refl_trans(self%plan, int(seed0, c_int64_t), int(nseed, c_int32_t), rmat, tmat, st))
     end subroutine
 
-    !! compute_deposition (src/waveguide.f90:712-750) for realizations seed0 .. seed0+nseed-1:
+    !! compute_deposition (src/waveguide.f90:710-745) for realizations seed0 .. seed0+nseed-1:
     !! depmat(:, :, k) = 2i s G(fpos; ipos) s of realization seed0+k-1, with the position checks of
     !! :719-727; ipos = 1 (the input lead's side) only.
     subroutine compute_deposition_batch_waveguide_gpu(self, seed0, nseed, fpos, ipos, depmat)
@@ -336,7 +336,7 @@ contains
         nv = self%jmax - self%jmin + 1
     end function
 
-    !! The data of compute_tprofile / compute_tprofile_group (src/waveguide.f90:807-997) for
+    !! The data of compute_tprofile / compute_tprofile_group (src/waveguide.f90:843-997) for
     !! realizations seed0 .. seed0+nseed-1: tvals(:, k) the eigenvalues of t^H t (descending),
     !! intensity(:, ix, k) = |G(x_ix;1) s U(:,v)|^2 per eigenvector v; filtered: t over the apertures.
     subroutine compute_tprofile_data_waveguide_gpu(self, seed0, nseed, filtered, tvals, intensity)

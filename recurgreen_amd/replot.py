@@ -67,7 +67,7 @@ def parse_namelist(text: str, group: str) -> dict:
 
 
 def check_settings(bintype: str, nbins: int, xmin: float, xmax: float) -> None:
-    """src/plot_histogram.f90:64-71 and :84-85."""
+    """src/plot_histogram.f90:65-72 and :84-85."""
     if nbins < 1:
         raise RecurGreenError(binding.RG_EINVAL, f"Invalid number of bins (nbins={nbins} must be positive), aborting...")
     if xmax <= xmin or math.isnan(xmin) or math.isnan(xmax):

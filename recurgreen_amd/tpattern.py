@@ -25,7 +25,7 @@ PROBATYPES = ("incident", "total", "deposition")
 
 
 def mode_index(boundtype: str, nmu: int, imode: int) -> int:
-    """Channel index imu (1-based) of incident mode `imode` (src/main.f90:768-772): in the
+    """Channel index imu (1-based) of incident mode `imode` (src/main.f90:772-776): in the
     periodic case the channels are ordered by transverse momentum, so imode=1 (the frontal mode)
     sits at the middle of the list."""
     if boundtype == "periodic":
@@ -34,7 +34,7 @@ def mode_index(boundtype: str, nmu: int, imode: int) -> int:
 
 
 def check_arguments(nseed: int, probatype: str) -> None:
-    """Argument checks of src/main.f90:753-760."""
+    """Argument checks of src/main.f90:754-761."""
     if nseed <= 0:
         raise RecurGreenError(binding.RG_EINVAL,
                               f"Invalid number of disorder realizations (received {nseed}), aborting...")
@@ -45,21 +45,21 @@ def check_arguments(nseed: int, probatype: str) -> None:
 
 def channel_probabilities(rmat: np.ndarray, tmat: np.ndarray, probatype: str, imu: int):
     """Per-realization (tprob, rprob), each (nreal, nmu), from r and t as (nreal, nmu, nmu) with
-    [r, b, a] = matrix element (b, a) (src/main.f90:816-822)."""
+    [r, b, a] = matrix element (b, a) (src/main.f90:815-821)."""
     if probatype == "incident":
         return np.abs(tmat[:, :, imu - 1]) ** 2, np.abs(rmat[:, :, imu - 1]) ** 2
     return np.sum(np.abs(tmat) ** 2, axis=1), np.sum(np.abs(rmat) ** 2, axis=1)
 
 
 def deposition_positions(params: dict) -> tuple[int, int]:
-    """(fpos, ipos) of the deposition pattern (src/main.f90:764-765): the middle of the medium, from
+    """(fpos, ipos) of the deposition pattern (src/main.f90:769-770): the middle of the medium, from
     the first slice."""
     return int(params.get("nxfree", 0)) + int(params["nxdiso"]) // 2 + 1, 1
 
 
 def compute_tpattern(wguide, nseed: int, imode: int, probatype: str = "incident", seed0: int = 1):
     """(tprob, rprob, imu): probabilities per outgoing channel averaged over realizations
-    seed0 .. seed0+nseed-1 (src/main.f90:838-840).  `wguide` is a recurgreen_amd.Waveguide."""
+    seed0 .. seed0+nseed-1 (src/main.f90:802-845).  `wguide` is a recurgreen_amd.Waveguide."""
     check_arguments(nseed, probatype)
     plan = wguide.plan
     nmu = plan.nmu

@@ -1,6 +1,6 @@
 """Transmission-eigenstate profiles inside the medium (tasks `tprofile` and `tprofile_group`).
 
-compute_tprofile (src/waveguide.f90:807-875) sums, over the transmission eigenvalues T_v in
+compute_tprofile (src/waveguide.f90:843-907) sums, over the transmission eigenvalues T_v in
 [tmin, tmax], the intensity |G(x;1) s U(:,v)|^2 of the eigenstate at every slice x and multiplies
 by 4/nt; compute_tprofile_group (:920-997) does the same for several windows |T - tm| <= dt of the
 aperture-filtered matrix and also returns tavg = sum(T)/neigv.  The device pass rg_tprofile
@@ -26,7 +26,7 @@ from .tpattern import parameter_lines
 
 
 def check_window(tmin: float, tmax: float) -> None:
-    """src/waveguide.f90:819-828."""
+    """src/waveguide.f90:855-868."""
     if tmin < 0.0 or tmin > 1.0:
         raise RecurGreenError(binding.RG_EINVAL, f"Invalid Tmin, received Tmin={tmin:.3g}, expected in [0, 1], aborting...")
     if tmax < 0.0 or tmax > 1.0:
@@ -36,7 +36,7 @@ def check_window(tmin: float, tmax: float) -> None:
 
 
 def check_group(tm, dt) -> None:
-    """src/waveguide.f90:938-943."""
+    """src/waveguide.f90:938-949."""
     tm, dt = np.asarray(tm, float), np.asarray(dt, float)
     if tm.shape != dt.shape:
         raise RecurGreenError(binding.RG_EINVAL, "tm and dt must have the same length")
@@ -47,7 +47,7 @@ def check_group(tm, dt) -> None:
 
 
 def window_profile(tvals: np.ndarray, intensity: np.ndarray, tmin: float, tmax: float):
-    """(profile[nx], nt) of one realization (src/waveguide.f90:858-872): tvals (nv,), intensity (nx, nv)."""
+    """(profile[nx], nt) of one realization (src/waveguide.f90:887-903): tvals (nv,), intensity (nx, nv)."""
     sel = (tmin <= tvals) & (tvals <= tmax)
     nt = int(sel.sum())
     prof = intensity[:, sel].sum(axis=1)
@@ -57,7 +57,7 @@ def window_profile(tvals: np.ndarray, intensity: np.ndarray, tmin: float, tmax: 
 
 
 def group_profiles(tvals: np.ndarray, intensity: np.ndarray, tm, dt):
-    """(tprofile[nx, nprofile], nt[nprofile], tavg) of one realization (src/waveguide.f90:959-985)."""
+    """(tprofile[nx, nprofile], nt[nprofile], tavg) of one realization (src/waveguide.f90:973-990)."""
     tm, dt = np.asarray(tm, float), np.asarray(dt, float)
     prof = np.zeros((intensity.shape[0], tm.size))
     nt = np.zeros(tm.size, dtype=int)
@@ -99,7 +99,7 @@ def compute_tprofile_batch(wguide, seed0: int, nseed: int, tmin: float, tmax: fl
 
 
 def compute_tprofile(wguide, iseed: int, tmin: float, tmax: float):
-    """compute_tprofile (src/waveguide.f90:807-875): (tprofile[nx], nt) of realization iseed."""
+    """compute_tprofile (src/waveguide.f90:843-907): (tprofile[nx], nt) of realization iseed."""
     p, nt = compute_tprofile_batch(wguide, iseed, 1, tmin, tmax)
     return p[0], int(nt[0])
 
@@ -118,7 +118,7 @@ def compute_tprofile_group_batch(wguide, seed0: int, nseed: int, tm, dt):
 
 
 def compute_tprofile_group(wguide, iseed: int, tm, dt):
-    """compute_tprofile_group (src/waveguide.f90:920-997): (tprofile[nx, nprofile], nt, tavg)."""
+    """compute_tprofile_group (src/waveguide.f90:924-997): (tprofile[nx, nprofile], nt, tavg)."""
     p, nt, tavg = compute_tprofile_group_batch(wguide, iseed, 1, tm, dt)
     return p[0], nt[0], float(tavg[0])
 

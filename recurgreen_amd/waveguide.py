@@ -97,7 +97,7 @@ class Waveguide:
         return r[0], t[0]
 
     def compute_deposition(self, iseed: int, fpos: int, ipos: int = 1) -> np.ndarray:
-        """depmat of realization `iseed` as compute_deposition (src/waveguide.f90:712-750) returns it:
+        """depmat of realization `iseed` as compute_deposition (src/waveguide.f90:710-745) returns it:
         nmu x nmu complex128, 2i s G(fpos; ipos) s with s = sqrt(Re mu); same position checks
         (:719-727).  ipos = 1 only (the input lead's side)."""
         nx = self.plan.nx
@@ -137,13 +137,13 @@ class Waveguide:
         return w
 
     def compute_tprofile(self, iseed: int, tmin: float, tmax: float):
-        """compute_tprofile (src/waveguide.f90:807-875): (tprofile[nx], nt) of realization iseed."""
+        """compute_tprofile (src/waveguide.f90:843-907): (tprofile[nx], nt) of realization iseed."""
         from . import tprofile
 
         return tprofile.compute_tprofile(self, iseed, tmin, tmax)
 
     def compute_tprofile_group(self, iseed: int, tm, dt):
-        """compute_tprofile_group (src/waveguide.f90:920-997): (tprofile[nx, nprofile], nt, tavg)."""
+        """compute_tprofile_group (src/waveguide.f90:924-997): (tprofile[nx, nprofile], nt, tavg)."""
         from . import tprofile
 
         return tprofile.compute_tprofile_group(self, iseed, tm, dt)

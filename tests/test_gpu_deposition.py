@@ -1,4 +1,4 @@
-"""Deposition matrix (compute_deposition, src/waveguide.f90:712-750) through the C ABI against the
+"""Deposition matrix (compute_deposition, src/waveguide.f90:710-745) through the C ABI against the
 oracle's block-tridiagonal solve on identical explicit potentials.
 
 Tolerances (complex128): 1e-10 relative to max|dep|.  At fpos = nx the deposition matrix is t and

@@ -1,5 +1,5 @@
 """Transmission eigenstates inside the medium (rg_tprofile; compute_tprofile and
-compute_tprofile_group, src/waveguide.f90:807-997) against the oracle: G(x;1) for every slice from
+compute_tprofile_group, src/waveguide.f90:843-997) against the oracle: G(x;1) for every slice from
 the reference's block-tridiagonal solve, t^H t decomposed by LAPACK (numpy eigh) on the host.
 
 Tolerances (complex128): eigenvalues to 1e-10 relative (1e-13 absolute); per-eigenvector

@@ -282,7 +282,7 @@ def test_green_l0_vs_dense_block_tridiagonal_solve():
 
 @pytest.mark.parametrize("boundtype", ["dirichlet", "periodic"])
 def test_deposition_vs_dense_inverse_and_reciprocity(boundtype):
-    """compute_deposition (src/waveguide.f90:712-750): the (fpos, ipos) block of inv(H) with the
+    """compute_deposition (src/waveguide.f90:710-745): the (fpos, ipos) block of inv(H) with the
     sqrt(Re mu) factors, against a dense inverse of H = tridiag(B, A_x, B); fpos = nx gives t and
     fpos = ipos = 1 gives r + I (src/waveguide.f90:508-512); reciprocity of the complex-symmetric
     H: G(f; i) = G(i; f)^T for real (Dirichlet) modes, with the channel order reversed (k -> -k)

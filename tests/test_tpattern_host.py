@@ -50,7 +50,7 @@ def test_argument_errors():
     with pytest.raises(RecurGreenError, match="Invalid probatype"):
         tpattern.check_arguments(3, "reflected")
     tpattern.check_arguments(3, "deposition")
-    assert tpattern.deposition_positions(dict(nxdiso=101, nxfree=7)) == (7 + 50 + 1, 1)  # src/main.f90:764-765
+    assert tpattern.deposition_positions(dict(nxdiso=101, nxfree=7)) == (7 + 50 + 1, 1)  # src/main.f90:769-770
 
 
 class _StubPlan:
@@ -74,7 +74,7 @@ def test_save_load_round_trip(tmp_path):
     assert text[5] == "imu, tprob, rprob"
     data = tpattern.load_tpattern(path)
     np.testing.assert_array_equal(data[:, 0], np.arange(1, 6))
-    # g0.16 keeps 16 significant digits (src/main.f90:890)
+    # g0.16 keeps 16 significant digits (src/main.f90:889)
     np.testing.assert_allclose(data[:, 1], tp, rtol=1e-15, atol=0)
     np.testing.assert_allclose(data[:, 2], rp, rtol=1e-15, atol=0)
 
@@ -114,7 +114,7 @@ def _run(tmp_path, text):
     (4, "reflected", "Invalid probatype, received 'reflected'"),
 ])
 def test_fortran_tpattern_argument_checks(tmp_path, nseed, probatype, msg):
-    """The checks of src/main.f90:753-760 stop before any device call."""
+    """The checks of src/main.f90:754-761 stop before any device call."""
     assert os.path.exists(rgbuild.FEXE), "Fortran host not built"
     r = _run(tmp_path, NML.format(nseed=nseed, probatype=probatype))
     assert msg in r.stderr, r.stderr

@@ -1,4 +1,4 @@
-"""Host logic of the tprofile tasks (src/waveguide.f90:807-997, src/main.f90:466-483), no GPU."""
+"""Host logic of the tprofile tasks (src/waveguide.f90:843-997, src/main.f90:466-483), no GPU."""
 import numpy as np
 import pytest
 
