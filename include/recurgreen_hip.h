@@ -114,9 +114,9 @@ rg_status rg_refl_trans_potential(rg_plan *plan, const double *upot, int32_t nre
 /* Mirrors compute_deposition (src/waveguide.f90:710-745) for seeds seed0 .. seed0+nseed-1: the
  * deposition matrix dep = 2i s G(fpos; ipos) s, s = sqrt(Re mu) (:741-742), full nmu x nmu,
  * column-major complex, realization-major in depmat (host, nseed*nmu*nmu complex).  fpos, ipos in
- * 1..nx (RG_EINVAL otherwise); ipos = 1 only, the input lead's side (RG_ENOTSUP otherwise): G(fpos; 1)
- * comes from a sweep from the right lead down to fpos+1 and the transmission sweep up to fpos-1,
- * joined at slice fpos.  Complex128 only. */
+ * 1..nx (RG_EINVAL otherwise).  G(fpos; ipos) comes from a sweep from the left lead up to fpos-1 and
+ * one from the right lead down to fpos+1, joined at slice fpos; the sweep on the side of ipos also
+ * carries the propagator from slice ipos.  Complex128 only. */
 rg_status rg_deposition(rg_plan *plan, int64_t seed0, int32_t nseed, int32_t fpos, int32_t ipos, double *depmat,
                         int32_t *status);
 /* Same for explicit host potentials upot (nxdiso*ny*nreal doubles). */

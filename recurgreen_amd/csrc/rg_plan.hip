@@ -480,14 +480,18 @@ double2 *run_sweep_refl(rg_plan *P, int nb, int64_t seed0, const double *d_upot,
     return P->d_gs1[cur ^ 1];
 }
 
-// compute_deposition (src/waveguide.f90:710-745) for ipos = 1, by recursion instead of the reference's
-// block-tridiagonal solve: with the right part (slices fpos+1..nx and the right lead) folded into the
-// right-connected Green function h = (A_x - V_x - B h' B)^-1 (a sweep from slice nx down, the same
-// kernels), G(fpos; fpos) = (A_f - V_f - B (g + h) B)^-1 with g = Gss_(fpos-1) of the forward sweep,
-// and G(fpos; 1) = -G(fpos; fpos) B Gs1_(fpos-1) (G(1; 1) itself at fpos = 1).  h is kept in the
-// G11 buffer of the reflection sweep.  Writes dep = 2i s G(fpos; 1) s (column-major nmu x nmu) to
-// dep_out.
-void run_sweep_deposition(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int fpos, double2 *dep_out)
+// compute_deposition (src/waveguide.f90:710-745), by recursion instead of the reference's
+// block-tridiagonal solve.  The slices on either side of fpos fold into the left-connected Green
+// function g = (A_x - V_x - B g' B)^-1 (a sweep from slice 1 up to fpos-1) and the right-connected h
+// (the same kernels from slice nx down to fpos+1); slice fpos coupled to both gives
+// G(fpos; fpos) = (A_f - V_f - B (g + h) B)^-1.  The sweep on the side of ipos also carries the
+// propagator of its part from ipos, P_ipos = g_ipos (or h_ipos), P_x = -g_x B P_(x-1) (or
+// -h_x B P_(x+1)), as Gs1 of the transmission sweep (src/waveguide.f90:438), and
+// G(fpos; ipos) = -G(fpos; fpos) B P at the slice next to fpos (G(fpos; fpos) itself at ipos = fpos).
+// The part without ipos is kept in the G11 buffer of the reflection sweep.  Writes
+// dep = 2i s G(fpos; ipos) s (column-major nmu x nmu) to dep_out.
+void run_sweep_deposition(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int fpos, int ipos,
+                          double2 *dep_out)
 {
     const int nx = P->nx;
     const size_t mat = (size_t)nb * P->np * P->np;
@@ -498,40 +502,51 @@ void run_sweep_deposition(rg_plan *P, int nb, int64_t seed0, const double *d_upo
         if (dis) rg_launch_project(P, nb, id - 1, seed0, d_upot, nb);
         return dis;
     };
-    if (fpos < nx) {  // right part, from the right lead inwards
-        for (int ix = nx; ix > fpos; ix--) {
-            const bool dis = project(ix);
-            invert_f64(P, nb, slot_of(ix), ix == nx, dis);
-        }
-        (void)hipMemcpyAsync(P->d_g11, P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
-    }
     int cur = 0;
-    for (int ix = 1; ix < fpos; ix++) {  // left part, as the transmission sweep
+    // one slice of a sweep started at slice `first`: Gss, and the propagator from ipos once reached
+    auto step = [&](int ix, int first, bool carry) {
         const bool dis = project(ix);
-        invert_f64(P, nb, slot_of(ix), ix == 1, dis);
-        if (ix == 1) {
+        invert_f64(P, nb, slot_of(ix), ix == first, dis);
+        if (!carry) return;
+        if (ix == ipos) {
             (void)hipMemcpyAsync(P->d_gs1[0], P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+            cur = 0;
         } else {
             rg_launch_gemm_f64(P, nb, P->d_gss, P->d_gs1[cur], P->d_gs1[cur ^ 1]);
             cur ^= 1;
         }
-    }
-    // slice fpos coupled to both parts: the inversion subtracts B Gss B with Gss = g + h (g alone at
-    // fpos = nx, h alone at fpos = 1; nx >= 2, so one of them is always present)
-    if (fpos < nx) {
-        if (fpos == 1)
-            (void)hipMemcpyAsync(P->d_gss, P->d_g11, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
-        else
-            rg_launch_add_c128(P, P->d_gss, P->d_g11, mat);
+    };
+    // slice fpos: the inversion subtracts B Gss B with Gss = g + h (h alone at fpos = 1, g alone at
+    // fpos = nx; nx >= 2, so fpos always has a neighbour).  The part without ipos goes first and waits
+    // in the G11 buffer.
+    auto keep = [&]() {
+        (void)hipMemcpyAsync(P->d_g11, P->d_gss, sizeof(double2) * mat, hipMemcpyDeviceToDevice, P->stream);
+    };
+    if (ipos <= fpos) {
+        if (fpos < nx) {
+            for (int ix = nx; ix > fpos; ix--) step(ix, nx, false);
+            if (fpos > 1) keep();
+        }
+        if (fpos > 1) {
+            for (int ix = 1; ix < fpos; ix++) step(ix, 1, ipos < fpos && ix >= ipos);
+            if (fpos < nx) rg_launch_add_c128(P, P->d_gss, P->d_g11, mat);
+        }
+    } else {
+        if (fpos > 1) {
+            for (int ix = 1; ix < fpos; ix++) step(ix, 1, false);
+            keep();
+        }
+        for (int ix = nx; ix > fpos; ix--) step(ix, nx, ix <= ipos);
+        if (fpos > 1) rg_launch_add_c128(P, P->d_gss, P->d_g11, mat);
     }
     const bool dis = project(fpos);
     invert_f64(P, nb, slot_of(fpos), 0, dis);
-    const double2 *gf1 = P->d_gss;
-    if (fpos > 1) {
+    const double2 *gfi = P->d_gss;
+    if (ipos != fpos) {
         rg_launch_gemm_f64(P, nb, P->d_gss, P->d_gs1[cur], P->d_gs1[cur ^ 1]);
-        gf1 = P->d_gs1[cur ^ 1];
+        gfi = P->d_gs1[cur ^ 1];
     }
-    rg_launch_deposition_out(P, nb, gf1, dep_out);
+    rg_launch_deposition_out(P, nb, gfi, dep_out);
 }
 
 // Transmission eigenstates inside the medium (compute_tprofile, src/waveguide.f90:843-907), complex128:
@@ -1259,7 +1274,6 @@ static rg_status deposition_pass(rg_plan *P, int64_t seed0, const double *upot, 
     rg_status st = check_common(P, nreal, RG_FP64);
     if (st) return st;
     if ((!depmat && nreal > 0) || fpos < 1 || fpos > P->nx || ipos < 1 || ipos > P->nx) return RG_EINVAL;
-    if (ipos != 1) return RG_ENOTSUP;
     DeviceGuard g(P->device);
     const size_t per = (size_t)P->nxdiso * P->ny, nn2 = (size_t)P->nmu * P->nmu;
     const int cap = upot ? std::min(P->upot_cap, P->refl_cap) : P->refl_cap;
@@ -1275,7 +1289,7 @@ static rg_status deposition_pass(rg_plan *P, int64_t seed0, const double *upot, 
             HIPCHK(hipMemcpyAsync(P->d_upot, upot + per * off, sizeof(double) * per * nb, hipMemcpyHostToDevice, P->stream));
             dup = P->d_upot;
         }
-        run_sweep_deposition(P, nb, seed0 + off, dup, fpos, d_dep);
+        run_sweep_deposition(P, nb, seed0 + off, dup, fpos, ipos, d_dep);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(depmat + 2 * nn2 * off, d_dep, sizeof(double2) * nn2 * nb, hipMemcpyDeviceToHost, P->stream));
         if ((st = harvest_status(P, nb, status ? status + off : nullptr, &worst))) return st;

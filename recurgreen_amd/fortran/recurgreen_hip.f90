@@ -301,7 +301,7 @@ contains
 
     !! compute_deposition (src/waveguide.f90:710-745) for realizations seed0 .. seed0+nseed-1:
     !! depmat(:, :, k) = 2i s G(fpos; ipos) s of realization seed0+k-1, with the position checks of
-    !! :719-727; ipos = 1 (the input lead's side) only.
+    !! :719-727.
     subroutine compute_deposition_batch_waveguide_gpu(self, seed0, nseed, fpos, ipos, depmat)
         class(waveguide_gpu_t), intent(inout) :: self
         integer, intent(in) :: seed0, nseed, fpos, ipos

@@ -56,6 +56,14 @@ def window_profile(tvals: np.ndarray, intensity: np.ndarray, tmin: float, tmax: 
     return prof, nt
 
 
+def itx_window(tvals: np.ndarray, intensity: np.ndarray, tmin: float, tmax: float, wol: float):
+    """itx[nx] of one realization (src/waveguide.f90:808-821): the same window sum as
+    window_profile, normalized by 2/(pi wol |tmax - tmin|) instead of 4/nt (the reference's older
+    convention, I_T(x) = sum_n |psi_n|^2 delta(T - T_n) with a rectangular delta)."""
+    sel = (tmin <= tvals) & (tvals <= tmax)
+    return intensity[:, sel].sum(axis=1) * (2.0 / (np.pi * wol * abs(tmax - tmin)))
+
+
 def group_profiles(tvals: np.ndarray, intensity: np.ndarray, tm, dt):
     """(tprofile[nx, nprofile], nt[nprofile], tavg) of one realization (src/waveguide.f90:973-990)."""
     tm, dt = np.asarray(tm, float), np.asarray(dt, float)
@@ -102,6 +110,19 @@ def compute_tprofile(wguide, iseed: int, tmin: float, tmax: float):
     """compute_tprofile (src/waveguide.f90:843-907): (tprofile[nx], nt) of realization iseed."""
     p, nt = compute_tprofile_batch(wguide, iseed, 1, tmin, tmax)
     return p[0], int(nt[0])
+
+
+def compute_itx_profile_batch(wguide, seed0: int, nseed: int, tmin: float, tmax: float):
+    """itx[nseed, nx] of realizations seed0 .. seed0+nseed-1."""
+    check_window(tmin, tmax)
+    tv, it = _pass(wguide, seed0, nseed, False)
+    wol = float(wguide.params["wol"])
+    return np.stack([itx_window(tv[r], it[r], tmin, tmax, wol) for r in range(nseed)])
+
+
+def compute_itx_profile(wguide, iseed: int, tmin: float, tmax: float):
+    """compute_itx_profile (src/waveguide.f90:764-825): itx[nx] of realization iseed."""
+    return compute_itx_profile_batch(wguide, iseed, 1, tmin, tmax)[0]
 
 
 def compute_tprofile_group_batch(wguide, seed0: int, nseed: int, tm, dt):

@@ -99,7 +99,7 @@ class Waveguide:
     def compute_deposition(self, iseed: int, fpos: int, ipos: int = 1) -> np.ndarray:
         """depmat of realization `iseed` as compute_deposition (src/waveguide.f90:710-745) returns it:
         nmu x nmu complex128, 2i s G(fpos; ipos) s with s = sqrt(Re mu); same position checks
-        (:719-727).  ipos = 1 only (the input lead's side)."""
+        (:719-727)."""
         nx = self.plan.nx
         if fpos < 1 or nx < fpos:
             raise RecurGreenError(binding.RG_EINVAL,
@@ -141,6 +141,12 @@ class Waveguide:
         from . import tprofile
 
         return tprofile.compute_tprofile(self, iseed, tmin, tmax)
+
+    def compute_itx_profile(self, iseed: int, tmin: float, tmax: float):
+        """compute_itx_profile (src/waveguide.f90:764-825): itx[nx] of realization iseed."""
+        from . import tprofile
+
+        return tprofile.compute_itx_profile(self, iseed, tmin, tmax)
 
     def compute_tprofile_group(self, iseed: int, tm, dt):
         """compute_tprofile_group (src/waveguide.f90:924-997): (tprofile[nx, nprofile], nt, tavg)."""

@@ -40,12 +40,39 @@ def test_deposition_matches_oracle(name):
     plan = binding.Plan(p, batch_max=3)
     U = upot_batch(ow, [2, 3, 4])
     nx = plan.nx
-    for fpos in sorted({1, 2, p.get("nxfree", 0) + p["nxdiso"] // 2 + 1, nx - 1, nx}):
-        d, st, code = plan.deposition(fpos, 1, U)
+    mid = p.get("nxfree", 0) + p["nxdiso"] // 2 + 1
+    pairs = {(f, 1) for f in (1, 2, mid, nx - 1, nx)}
+    # interior and right-hand sources: ipos below fpos, above it, equal to it, and at both ends
+    pairs |= {(mid, 3), (3, mid), (mid, mid), (nx, nx), (1, nx), (nx, nx - 1), (1, 2), (2, 2), (nx - 1, nx)}
+    for fpos, ipos in sorted(pairs):
+        d, st, code = plan.deposition(fpos, ipos, U)
         assert code == 0 and not st.any()
         for k in range(3):
-            ref = ow.deposition(U[:, :, k].T, fpos, 1)
-            assert np.max(np.abs(d[k] - ref)) <= 1e-10 * max(1.0, np.max(np.abs(ref))), (fpos, k)
+            ref = ow.deposition(U[:, :, k].T, fpos, ipos)
+            assert np.max(np.abs(d[k] - ref)) <= 1e-10 * max(1.0, np.max(np.abs(ref))), (fpos, ipos, k)
+    plan.close()
+
+
+@pytest.mark.parametrize("boundtype", ["dirichlet", "periodic"])
+def test_deposition_reciprocity_n128(boundtype):
+    """H is complex symmetric, so G(f; i) = G(i; f)^T: dep(f, i) = dep(i, f)^T for real (Dirichlet)
+    modes, with the channel order reversed (k -> -k) for plane waves; on the N = 128 shape, where
+    the oracle is slow, for a source left of, right of and at the observation slice."""
+    if boundtype == "dirichlet":
+        p = dict(boundtype="dirichlet", nxdiso=200, wol=64.25, aratio=10.0 / 64.25, dscat=1.0, ny=1285, dos_model=1)
+    else:
+        p = dict(boundtype="periodic", nxdiso=200, wol=64.3, aratio=0.15, dscat=1.0, dabso=0.1)
+    say(f"deposition reciprocity N128 {boundtype}")
+    plan = binding.Plan(p, batch_max=4)
+    n = plan.nmu
+    perm = np.arange(n) if boundtype == "dirichlet" else np.arange(n)[::-1]
+    for f, i in [(150, 40), (7, 180), (plan.nx, 1)]:
+        a, st, code = plan.deposition(f, i, None, seed0=11, nseed=4)
+        b, st2, code2 = plan.deposition(i, f, None, seed0=11, nseed=4)
+        assert code == 0 and code2 == 0 and not st.any() and not st2.any()
+        bt = np.swapaxes(b, 1, 2)[:, perm][:, :, perm]
+        scale = np.max(np.abs(a))
+        assert np.max(np.abs(a - bt)) <= 1e-10 * scale, (f, i)
     plan.close()
 
 
@@ -84,9 +111,10 @@ def test_deposition_argument_checks():
     with pytest.raises(binding.RecurGreenError) as e:
         plan.deposition(plan.nx + 1, 1, None, nseed=1)
     assert e.value.code == binding.RG_EINVAL
-    with pytest.raises(binding.RecurGreenError) as e:
-        plan.deposition(3, 2, None, nseed=1)
-    assert e.value.code == binding.RG_ENOTSUP
+    for ipos in (0, plan.nx + 1):
+        with pytest.raises(binding.RecurGreenError) as e:
+            plan.deposition(3, ipos, None, nseed=1)
+        assert e.value.code == binding.RG_EINVAL
     wg = Waveguide(**p, batch_max=2)
     with pytest.raises(binding.RecurGreenError, match="Invalid final position index"):
         wg.compute_deposition(1, 0)

@@ -74,6 +74,11 @@ def test_tprofile_eigenstates_match_oracle(name):
         ref = inten[:, T >= lo].sum(axis=1) * 4.0 / max(1, int((T >= lo).sum()))
         assert nt == int((T >= lo).sum())
         np.testing.assert_allclose(prof, ref, rtol=1e-9, atol=1e-12 * ref.max())
+        # compute_itx_profile: the same window, normalized by 2/(pi wol (tmax - tmin))
+        itx = tprofile.itx_window(tv[k], it[k], lo, 1.0, p["wol"])
+        sums = inten[:, T >= lo].sum(axis=1)
+        np.testing.assert_allclose(itx * (np.pi * p["wol"] * (1.0 - lo) / 2.0), sums, rtol=1e-9,
+                                   atol=1e-12 * max(sums.max(), 1.0))
     plan.close()
 
 
@@ -94,6 +99,8 @@ def test_tprofile_passes_and_host_calls():
     ref, nref = tprofile.window_profile(tv[2], it[2], 0.5, 1.0)
     assert nt == nref
     np.testing.assert_array_equal(prof, ref)
+    itx = wg.compute_itx_profile(5, 0.5, 1.0)
+    np.testing.assert_array_equal(itx, tprofile.itx_window(tv[2], it[2], 0.5, 1.0, p["wol"]))
     gp, gnt, tavg = tprofile.compute_tprofile_group(wg, 5, [0.25, 0.75], [0.25, 0.25])
     assert tavg == pytest.approx(tv[2].sum() / tv.shape[1], rel=1e-14)
     assert gnt.sum() == int(((tv[2] >= 0.0) & (tv[2] <= 1.0)).sum()) + int((tv[2] == 0.5).sum())

@@ -24,6 +24,13 @@ def test_window_and_group_sums_brute_force():
         r = np.array([sum(it[x, v] for v in s) for x in range(nx)])
         np.testing.assert_allclose(gp[:, ip], r * (4.0 / len(s)) if s else r, rtol=1e-15)
     assert tavg == pytest.approx(tv.sum() / nv, rel=1e-15)
+    # compute_itx_profile's normalization: 2/(pi wol (tmax - tmin)), whatever nt
+    wol = 7.3
+    itx = tprofile.itx_window(tv, it, 0.3, 0.8, wol)
+    ref_itx = np.array([sum(it[x, v] for v in sel) for x in range(nx)]) * 2.0 / (np.pi * wol * 0.5)
+    np.testing.assert_allclose(itx, ref_itx, rtol=1e-15)
+    np.testing.assert_allclose(itx * np.pi * wol * 0.5 / 2.0, prof * nt / 4.0, rtol=1e-14)
+    assert not tprofile.itx_window(tv, it, 0.0, 1e-9, wol).any()
     # empty window: zero profile, nt = 0 (the realization is then left out of the average)
     p0, n0 = tprofile.window_profile(tv, it, 0.0, 1e-9)
     assert n0 == 0 and not p0.any()
