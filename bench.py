@@ -21,6 +21,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "realizations/s at N=128 modes, mixed vs FP64 sweep; max rho(T) bin error"
 PEAK_TFLOPS = {"fp64": 78.6, "mixed_fp32": 157.3, "mixed_bf16x3": 2500.0}
 DTYPE = {"fp64": "f64", "mixed_fp32": "f32+f64", "mixed_bf16x3": "bf16x3+f64"}
 
@@ -66,7 +67,7 @@ def cpu_baseline(params, seconds):
     ow.tdistrib_batch(1, nreal, nthreads=threads)
     dt = O.wtime() - t0
     return {"value": nreal / dt, "unit": "realizations/s", "cores": threads, "kind": "port",
-            "sample": f"{nreal} realizations of the same N=128, nx={params['nxdiso']}, ny={params['ny']} workload "
+            "sample": f"{nreal} realizations of the same N={ow.nmu}, nx={params['nxdiso']}, ny={params['ny']} workload "
                       f"(FP64 oracle, OpenMP over realizations, {dt:.1f} s wall)"}
 
 
@@ -375,7 +376,7 @@ def main():
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get(f"{r['dominant']}:{main_prec}")
     line = {
-        "metric": "realizations/s at N=128 modes, mixed vs FP64 sweep; max rho(T) bin error",
+        "metric": METRIC if args.config == "n128" else f"realizations/s at N={n} modes ({args.config})",
         "value": r["value"], "unit": "realizations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": DTYPE[main_prec], "data": "synthetic",
