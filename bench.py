@@ -379,7 +379,9 @@ def main():
         "metric": METRIC if args.config == "n128" else f"realizations/s at N={n} modes ({args.config})",
         "value": r["value"], "unit": "realizations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": DTYPE[main_prec], "data": "synthetic",
+        "vs_baseline": None, "data": "synthetic",
+        # option dyson = 2: the complex64 Dyson product on BF16 matrix cores with three-way split FP32 operands
+        "dtype": "f32+bf16x6+f64" if (options or {}).get(main_prec, {}).get("dyson") == 2 else DTYPE[main_prec],
         "config": {"workload": f"{args.config}: dirichlet W={params['wol']} N={n} L={nx * 0.05:g} nx={nx} ny={plan.ny} "
                                f"ls={CONFIGS[args.config][2]:g} eps=0, batch {B} realizations, precision {main_prec}",
                    "global_batch": B, "parallelism": "realizations"},

@@ -191,7 +191,10 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *   "inverse_fp64"  0 register-resident Gauss-Jordan, 1 look-ahead Gauss-Jordan, 2 blocked LU;
  *   "inverse_mixed" 1 look-ahead, 2 blocked, 3 verified diagonal pivots (default);
  *   "dyson"         complex64 Dyson product: 0 64x64 output tiles, 1 one workgroup per realization
- *                   (default, NP 80..128);
+ *                   (default, NP 80..128), 2 the same on the BF16 matrix cores with every FP32
+ *                   operand split into three BF16 values (six products, FP32 accuracy per
+ *                   product; NP 96 and 128, else 1; faster, but the N = 128 sweep's mean T moved
+ *                   4.7e-7 from complex128 against 2.0e-8 with 1, so not the default);
  *   "dyson_fp64"    the same choice for the complex128 product (default 1);
  *   "project_parts" 1..8 partial sums of the projection over transverse cells;
  *   "lanes"         1 one stream, 2 the batch split in halves on two streams;

@@ -602,8 +602,9 @@ void rg_launch_gemm_sp(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_GEMM, &stop);
     if (precision == RG_MIXED_FP32) {
-        if (!P->gemm_wg || !rg_launch_gemm_wg(P, nb, gss, gs1_old, gs1_new))
-            launch_gemm_sp<RG_MIXED_FP32>(P, nb, gss, gs1_old, gs1_new);
+        const bool done = (P->gemm_wg == 2 && rg_launch_gemm_bf3(P, nb, gss, gs1_old, gs1_new)) ||
+                          (P->gemm_wg >= 1 && rg_launch_gemm_wg(P, nb, gss, gs1_old, gs1_new));
+        if (!done) launch_gemm_sp<RG_MIXED_FP32>(P, nb, gss, gs1_old, gs1_new);
     }
     else launch_gemm_sp<RG_MIXED_BF16X3>(P, nb, gss, gs1_old, gs1_new);
     rg_timer_end(P, RGK_GEMM, stop);

@@ -167,7 +167,10 @@ struct rg_plan {
     // slice-inversion kernel per state precision (RG_INV_*): FP64 state, complex64 state
     int inv_fp64 = 0, inv_mixed = 3;
     bool eig_jacobi = false;  // transmission eigenvalues by one-sided Jacobi SVD of t instead of eig(t^H t)
-    bool gemm_wg = true;      // FP32 Dyson product: one realization per workgroup (NP >= 80) or 64 x 64 tiles
+    // complex64 Dyson product (option dyson): 0 64 x 64 tiles, 1 one realization per workgroup on FP32
+    // MFMA (NP >= 80), 2 the same on BF16 MFMA with three-way split operands (NP 96 and 128, else 1)
+    int gemm_wg = 1;
+    int ncu = 256;  // compute units of the device: the grid of the persistent kernels
     rg_timer timers[RGK_COUNT];
     // Captured sweeps (option "graph"): one executable graph per shape/options/output, replayed with
     // the seed read from the device word d_seed (seed_dev points at it while a sweep is captured).
@@ -281,6 +284,7 @@ size_t rg_schur_elems(int np);
 bool rg_launch_gemm_wg(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new);
 // rg_dyson64.hip: the complex128 product, one workgroup per realization (NP 80..128; false otherwise)
 bool rg_launch_gemm_wg64(rg_plan *P, int nb, const double2 *gss, const double2 *gs1_old, double2 *gs1_new);
+bool rg_launch_gemm_bf3(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new);
 void rg_launch_copy_f2(rg_plan *P, const float2 *src, float2 *dst, size_t n);
 void rg_launch_histogram(rg_plan *P, const double *d_samples, int64_t n, const double *d_bins, int nbins,
                          unsigned long long *d_counts);

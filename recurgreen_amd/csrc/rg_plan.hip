@@ -343,8 +343,8 @@ __global__ void k_set_seed(int64_t *__restrict__ dst, int64_t v) { *dst = v; }
 // Options that change the kernels a sweep launches (part of a captured sweep's key).
 static int graph_opts(const rg_plan *P)
 {
-    return P->inv_fp64 | P->inv_mixed << 4 | (P->gemm_wg ? 1 : 0) << 8 | P->cq_parts << 9 | (P->eig_jacobi ? 1 : 0) << 13 |
-           P->dyson64 << 14;
+    return P->inv_fp64 | P->inv_mixed << 4 | P->cq_parts << 9 | (P->eig_jacobi ? 1 : 0) << 13 | P->dyson64 << 14 |
+           P->gemm_wg << 16;
 }
 
 void free_graphs(rg_plan *P)
@@ -1038,16 +1038,20 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
     // Transmission eigenvalues: Hermitian eigensolver of t^H t (default) or RG_EIGEN=jacobi.
     const char *eg = std::getenv("RG_EIGEN");
     P->eig_jacobi = eg && std::strcmp(eg, "jacobi") == 0;
-    // FP32 Dyson product: RG_DYSON=tile64 for the 64 x 64 tiled kernel instead of one workgroup
-    // per realization.
+    // complex64 Dyson product: RG_DYSON=tile64 for the 64 x 64 tiled kernel, RG_DYSON=bf16x6 for the
+    // BF16 matrix-core kernel, instead of one workgroup per realization on FP32 MFMA.
     const char *dy = std::getenv("RG_DYSON");
-    P->gemm_wg = !(dy && std::strcmp(dy, "tile64") == 0);
+    P->gemm_wg = dy && std::strcmp(dy, "tile64") == 0 ? 0 : (dy && std::strcmp(dy, "bf16x6") == 0 ? 2 : P->gemm_wg);
     rg_status st = init_modes(P);  // argument checks first, like init_waveguide
     if (st) { delete P; return st; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) { delete P; return RG_EDEVICE; }
     DeviceGuard g(device);
     if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess) { free_plan(P); return RG_EDEVICE; }
+    if (hipDeviceGetAttribute(&P->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || P->ncu < 1) {
+        free_plan(P);
+        return RG_EDEVICE;
+    }
     if ((st = upload_constants(P)) || (st = alloc_batch(P))) { free_plan(P); return st; }
     // Concurrent lanes (RG_LANES=1 or 2): views of this plan on streams of their own.
     const char *ln = std::getenv("RG_LANES");
@@ -1600,8 +1604,8 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
         if (value != RG_INV_LOOKAHEAD && value != RG_INV_BLOCKED && value != RG_INV_NOPIV) return RG_EINVAL;
         P->inv_mixed = value;
     } else if (std::strcmp(key, "dyson") == 0) {
-        if (value != 0 && value != 1) return RG_EINVAL;
-        P->gemm_wg = value == 1;
+        if (value < 0 || value > 2) return RG_EINVAL;
+        P->gemm_wg = value;
     } else if (std::strcmp(key, "dyson_fp64") == 0) {
         if (value != 0 && value != 1) return RG_EINVAL;
         P->dyson64 = value;
@@ -1638,7 +1642,7 @@ rg_status rg_plan_get_option(const rg_plan *P, const char *key, int32_t *value)
     if (std::strcmp(key, "inverse_fp64") == 0) *value = P->inv_fp64;
     else if (std::strcmp(key, "inverse_mixed") == 0) *value = P->inv_mixed;
     else if (std::strcmp(key, "dyson_fp64") == 0) *value = P->dyson64;
-    else if (std::strcmp(key, "dyson") == 0) *value = P->gemm_wg ? 1 : 0;
+    else if (std::strcmp(key, "dyson") == 0) *value = P->gemm_wg;
     else if (std::strcmp(key, "project_parts") == 0) *value = P->cq_parts;
     else if (std::strcmp(key, "lanes") == 0) *value = P->lanes;
     else if (std::strcmp(key, "eigen") == 0) *value = P->eig_jacobi ? 1 : 0;

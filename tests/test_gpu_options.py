@@ -19,12 +19,13 @@ pytestmark = pytest.mark.gpu
 def test_option_round_trip_and_rejection():
     say("options round trip")
     plan = binding.Plan(CASES["c1_N20_ny205"], batch_max=2)
-    for key, values in (("inverse_fp64", (0, 1, 2)), ("inverse_mixed", (1, 2, 3)), ("dyson", (0, 1)),
+    for key, values in (("inverse_fp64", (0, 1, 2)), ("inverse_mixed", (1, 2, 3)), ("dyson", (0, 1, 2)),
                         ("project_parts", (1, 3, 8)), ("lanes", (2, 1)), ("eigen", (1, 0))):
         for v in values:
             plan.set_option(key, v)
             assert plan.get_option(key) == v
-    for key, v in (("inverse_fp64", 3), ("inverse_mixed", 0), ("project_parts", 9), ("lanes", 3), ("nonsense", 0)):
+    for key, v in (("inverse_fp64", 3), ("inverse_mixed", 0), ("dyson", 3), ("project_parts", 9), ("lanes", 3),
+                   ("nonsense", 0)):
         with pytest.raises(binding.RecurGreenError):
             plan.set_option(key, v)
     plan.close()
@@ -45,7 +46,7 @@ def test_every_option_value_matches_oracle(name):
             Tg, st, code = plan.tdistrib_potential(U, "fp64")
             assert code == 0 and not st.any(), (key, v)
             assert_T_close(Tg.reshape(2, -1).ravel(), To.ravel())
-    for key, values in (("inverse_mixed", (1, 2, 3)), ("dyson", (0, 1))):
+    for key, values in (("inverse_mixed", (1, 2, 3)), ("dyson", (0, 1, 2))):
         for v in values:
             plan.set_option(key, v)
             Tg, st, code = plan.tdistrib_potential(U, "mixed_fp32")
@@ -129,4 +130,26 @@ def test_graph_replays_match_direct_launches(precision):
     for _ in range(3):
         T1, st1, c1 = plan.tdistrib_potential(U, precision)
         assert c1 == 0 and np.array_equal(T1, T0)
+    plan.close()
+
+
+@pytest.mark.parametrize("wol", [48.25, 64.25])
+def test_bf16x6_dyson_matches_oracle(wol):
+    """Option dyson = 2 (the complex64 Dyson product on the BF16 matrix cores with three-way split
+    FP32 operands) at both orders it is built for, NP = 96 and 128, within the complex64 bound."""
+    p = dict(boundtype="dirichlet", nxdiso=40, wol=wol, aratio=2.0 / wol, dscat=1.0, ny=int(20 * wol), dos_model=1)
+    say(f"bf16x6 Dyson product N={int(2 * wol)}")
+    ow = O.Waveguide(**p)
+    U = upot_batch(ow, [3, 8, 9])
+    To = np.stack([ow.tdistrib_potential(U[:, :, k].T) for k in range(3)])
+    plan = binding.Plan(p, batch_max=3)
+    plan.set_option("dyson", 2)
+    Tg, st, code = plan.tdistrib_potential(U, "mixed_fp32")
+    assert code == 0 and not st.any()
+    err = np.max(np.abs(Tg.reshape(3, -1) - To))
+    print(f"[gpu-case] bf16x6 N={int(2 * wol)}: max |dT| = {err:.3e}", flush=True)
+    assert err <= MIXED_TOL["mixed_fp32"]
+    plan.set_option("dyson", 1)
+    T1, st1, code1 = plan.tdistrib_potential(U, "mixed_fp32")
+    assert code1 == 0 and np.max(np.abs(T1 - Tg)) <= MIXED_TOL["mixed_fp32"]
     plan.close()
