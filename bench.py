@@ -117,6 +117,7 @@ def accuracy(plan, t_ref, t_mix):
     from recurgreen_amd import distrib
 
     out = {"mean_T_fp64": float(t_ref.mean().item()), "mean_T_err": float((t_mix.mean() - t_ref.mean()).abs().item()),
+           "mean_T_shift": float((t_mix.mean() - t_ref.mean()).item()),
            "max_abs_dT": float((t_mix - t_ref).abs().max().item())}
     out["fp64_T_above_1"] = int((t_ref > 1.0).sum().item())
     out["mixed_T_at_1"] = int((t_mix == 1.0).sum().item())
