@@ -1,6 +1,6 @@
 """Benchmark of the recursive Green function sweep on one MI355X.
 
-Workload (BASELINE metric at N=128): Dirichlet waveguide, wavelength 1, W=64.25 (N=128 open
+Workload (the headline metric, at N=128): Dirichlet waveguide, wavelength 1, W=64.25 (N=128 open
 modes), L=50 with dx=dy=0.05 (nx=1000 slices, ny=1285 transverse cells), white-noise disorder
 with l_s=10 (L/l_s=5), epsilon=0, a batch of 4096 realizations per step.  The disorder is
 generated on the device from the seeds, so every input is resident in HBM when the timed
