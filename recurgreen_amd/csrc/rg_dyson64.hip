@@ -36,6 +36,13 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
     __shared__ DysonShared64<NP> sh;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1, li = lane & 15, lg = lane >> 4;
+    // ownership of the T x T output tiles: 2 row x 4 column tiles per wavefront, or at T = 7 (49
+    // tiles) one row tile and all 7 columns for wavefronts 0..6, which evens the work out (7 tiles at
+    // most instead of 8)
+    constexpr bool ROW1 = T == 7;
+    constexpr int XR = ROW1 ? 1 : 2, YC = ROW1 ? 7 : 4;
+    auto rtile = [&](int x) { return ROW1 ? w : wr + 4 * x; };
+    auto ctile = [&](int y) { return ROW1 ? y : wc + 2 * y; };
     const size_t base = (size_t)blockIdx.x * NP * NP;
     const double2 *A = gss + base, *B = gs1 + base;
 
@@ -67,11 +74,11 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
         }
     };
 
-    dy64_d4 cr[2][4], ci[2][4];
+    dy64_d4 cr[XR][YC], ci[XR][YC];
 #pragma unroll
-    for (int x = 0; x < 2; x++)
+    for (int x = 0; x < XR; x++)
 #pragma unroll
-        for (int y = 0; y < 4; y++) {
+        for (int y = 0; y < YC; y++) {
             cr[x][y] = dy64_d4{0.0, 0.0, 0.0, 0.0};
             ci[x][y] = dy64_d4{0.0, 0.0, 0.0, 0.0};
         }
@@ -85,25 +92,25 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
 #pragma unroll
         for (int ks = 0; ks < 4; ks++) {
             const int kr = 4 * ks + lg;
-            double ar[2], ai[2], nai[2], br[4], bi[4];
+            double ar[XR], ai[XR], nai[XR], br[YC], bi[YC];
 #pragma unroll
-            for (int x = 0; x < 2; x++) {
-                const int rt = wr + 4 * x;
+            for (int x = 0; x < XR; x++) {
+                const int rt = rtile(x);
                 ar[x] = rt < T ? sh.a[buf][0][kr][16 * rt + li] : 0.0;
                 ai[x] = rt < T ? sh.a[buf][1][kr][16 * rt + li] : 0.0;
                 nai[x] = -ai[x];
             }
 #pragma unroll
-            for (int y = 0; y < 4; y++) {
-                const int ct = wc + 2 * y;
+            for (int y = 0; y < YC; y++) {
+                const int ct = ctile(y);
                 br[y] = ct < T ? sh.b[buf][0][kr][16 * ct + li] : 0.0;
                 bi[y] = ct < T ? sh.b[buf][1][kr][16 * ct + li] : 0.0;
             }
 #pragma unroll
-            for (int x = 0; x < 2; x++)
+            for (int x = 0; x < XR; x++)
 #pragma unroll
-                for (int y = 0; y < 4; y++) {
-                    if (wr + 4 * x >= T || wc + 2 * y >= T) continue;  // uniform per wavefront
+                for (int y = 0; y < YC; y++) {
+                    if (rtile(x) >= T || ctile(y) >= T) continue;  // uniform per wavefront
                     cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], cr[x][y], 0, 0, 0);
                     cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[x], bi[y], cr[x][y], 0, 0, 0);
                     ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], ci[x][y], 0, 0, 0);
@@ -115,10 +122,10 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
     }
     double2 *C = out + base;
 #pragma unroll
-    for (int x = 0; x < 2; x++)
+    for (int x = 0; x < XR; x++)
 #pragma unroll
-        for (int y = 0; y < 4; y++) {
-            const int rt = wr + 4 * x, ct = wc + 2 * y;
+        for (int y = 0; y < YC; y++) {
+            const int rt = rtile(x), ct = ctile(y);
             if (rt >= T || ct >= T) continue;
 #pragma unroll
             for (int v = 0; v < 4; v++)
