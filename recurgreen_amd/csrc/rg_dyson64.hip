@@ -3,7 +3,8 @@
 // Gs1_new = -Gss diag(b) Gs1_old (src/waveguide.f90:438) for NP = 80..128 on v_mfma_f64_16x16x4_f64.
 // One workgroup of 8 wavefronts computes the whole NP x NP product of one realization, so Gss and Gs1
 // are each read from HBM once (the 64 x 64 tiling reads them twice).  Wavefront w owns row tiles
-// {w/2, w/2 + 4} and column tiles {w%2, w%2 + 2, w%2 + 4, w%2 + 6}.  k runs in chunks of 16: the next
+// {w/2, w/2 + 4} and column tiles {w%2, w%2 + 2, w%2 + 4, w%2 + 6}; at NP = 112 (7 x 7 tiles)
+// wavefronts 0..6 own one row tile each with all 7 columns.  k runs in chunks of 16: the next
 // chunk is loaded into registers (one complex per 16-byte load) while the current one is multiplied
 // out of LDS, then stored to the other LDS buffer -- one barrier per chunk.  b scales the rows of
 // Gs1_old in FP64 on the way into LDS.  133 KiB of LDS: one workgroup per CU, 256 registers a lane.
