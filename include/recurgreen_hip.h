@@ -91,7 +91,7 @@ rg_status rg_tdistrib_device(rg_plan *plan, int64_t seed0, int32_t nseed, int32_
  * finished ones dropping out); other widths form further groups.  Output in the caller's order:
  * the eigenvalues of realization r (descending) at tallvals[offsets[r] .. offsets[r+1]), offsets
  * has nreal+1 entries; status (may be NULL) one rg_status per realization (RG_ENOTSUP for mode
- * counts above 128).  Allocates one plan per group of at most batch_max realizations. */
+ * counts above 256).  Allocates one plan per group of at most batch_max realizations. */
 rg_status rg_tdistrib_uneven(const rg_params *params, const int64_t *seeds, int32_t nreal, int32_t precision,
                              int32_t device, int32_t batch_max, double *tallvals, int64_t *offsets, int32_t *status);
 /* Explicit potentials (host upot, nxdiso*ny*nreal doubles): the path used for parity against
@@ -198,7 +198,12 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *   "dyson_fp64"    the same choice for the complex128 product (default 1);
  *   "project_parts" 1..8 partial sums of the projection over transverse cells;
  *   "lanes"         1 one stream, 2 the batch split in halves on two streams;
- *   "eigen"         0 Hermitian eigensolver of t^H t (default), 1 one-sided Jacobi SVD of t;
+ *   "eigen"         transmission eigenvalues: 0 eigenvalues of t^H t (Householder tridiagonalisation
+ *                   and Sturm bisection); 1 one-sided Jacobi SVD of t; 2 Householder bidiagonalisation
+ *                   of t and Sturm bisection of its singular values (orders <= 128, Jacobi above);
+ *                   3 (default) 2 for complex128 sweeps, 0 for complex64 sweeps.  The SVD forms keep
+ *                   each sigma within a few n eps sigma_max, as zgesvd (src/base_utils.f90:341-380);
+ *                   t^H t loses closed channels (T ~ eps T_max absolute, src/base_utils.f90:385);
  *   "profile_mask"  bit k set: rg_plan_profile times the launches of entry k of
  *                   rg_plan_kernel_times (default: all bits);
  *   "graph"         1: seeded and explicit-potential sweeps run as captured HIP graphs (captured

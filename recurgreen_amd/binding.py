@@ -213,6 +213,16 @@ class Plan:
         except Exception:
             pass
 
+    def _upot(self, upot: np.ndarray) -> tuple:
+        """upot(ix, iy[, r]) as a Fortran-ordered float64 array and its realization count; raises
+        RecurGreenError(RG_EINVAL) unless the leading dimensions are (nxdiso, ny), so that the
+        library never reads past the caller's buffer."""
+        u = np.asfortranarray(upot, dtype=np.float64)
+        if u.ndim not in (2, 3) or u.shape[:2] != (self.nxdiso, self.ny) or (u.ndim == 3 and u.shape[2] < 1):
+            raise RecurGreenError(RG_EINVAL, f"potential of shape {np.shape(upot)}: expected (nxdiso, ny[, nreal]) = "
+                                             f"({self.nxdiso}, {self.ny}[, nreal])")
+        return u, (1 if u.ndim == 2 else u.shape[2])
+
     # -- sweeps ---------------------------------------------------------------------------
     def tdistrib(self, seed0: int, nseed: int, precision: str = "fp64"):
         T = np.zeros(nseed * self.neigv)
@@ -227,9 +237,7 @@ class Plan:
                                        ptr(status)))
 
     def tdistrib_potential(self, upot: np.ndarray, precision: str = "fp64"):
-        u = np.asfortranarray(upot, dtype=np.float64)
-        nreal = 1 if u.ndim == 2 else u.shape[2]
-        assert u.shape[:2] == (self.nxdiso, self.ny), u.shape
+        u, nreal = self._upot(upot)
         T = np.zeros(nreal * self.neigv)
         st = np.zeros(nreal, np.int32)
         code = lib().rg_tdistrib_potential(self._h, ptr(u), nreal, PRECISIONS[precision], ptr(T), ptr(st))
@@ -240,8 +248,7 @@ class Plan:
     def transmission_potential(self, upot: np.ndarray | None, nreal: int = 1, precision: str = "fp64"):
         m, n = self.imax - self.imin + 1, self.jmax - self.jmin + 1
         if upot is not None:
-            u = np.asfortranarray(upot, dtype=np.float64)
-            nreal = 1 if u.ndim == 2 else u.shape[2]
+            u, nreal = self._upot(upot)
         else:
             u = None
         t = np.zeros((nreal, n, m), complex)  # column-major m x n per realization
@@ -256,8 +263,7 @@ class Plan:
         r and t are (nreal, nmu, nmu); explicit potentials upot(ix, iy[, r]) or seeds seed0...."""
         n = self.nmu
         if upot is not None:
-            u = np.asfortranarray(upot, dtype=np.float64)
-            nreal = 1 if u.ndim == 2 else u.shape[2]
+            u, nreal = self._upot(upot)
         else:
             u, nreal = None, int(nseed)
         r = np.zeros((nreal, n, n), complex)  # column-major per realization
@@ -276,8 +282,7 @@ class Plan:
         (nreal, nmu, nmu), 1-based slice positions; explicit potentials upot(ix, iy[, r]) or seeds."""
         n = self.nmu
         if upot is not None:
-            u = np.asfortranarray(upot, dtype=np.float64)
-            nreal = 1 if u.ndim == 2 else u.shape[2]
+            u, nreal = self._upot(upot)
         else:
             u, nreal = None, int(nseed)
         d = np.zeros((nreal, n, n), complex)  # column-major per realization
@@ -317,8 +322,7 @@ class Plan:
         compute_tprofile_group, :951-954).  Reserves the workspace for min(nreal, reserve or
         batch_max) realizations per pass."""
         if upot is not None:
-            u = np.asfortranarray(upot, dtype=np.float64)
-            nreal = 1 if u.ndim == 2 else u.shape[2]
+            u, nreal = self._upot(upot)
         else:
             u, nreal = None, int(nseed)
         self.tprofile_reserve(max(1, reserve or self.tprofile_pass_size(nreal)))
@@ -344,7 +348,9 @@ class Plan:
         if upot is None:
             code = lib().rg_wavefunction(self._h, int(seed), int(ipos), int(imode), ptr(psi), ptr(st))
         else:
-            u = np.asfortranarray(upot, dtype=np.float64)
+            u, nreal = self._upot(upot)
+            if nreal != 1:
+                raise RecurGreenError(RG_EINVAL, "one potential (nxdiso, ny) per wavefunction")
             code = lib().rg_wavefunction_potential(self._h, ptr(u), int(ipos), int(imode), ptr(psi), ptr(st))
         if code not in (RG_OK, RG_ESINGULAR, RG_ENONFINITE):
             check(code)
@@ -358,10 +364,9 @@ class Plan:
     def adjoint(self, upot: np.ndarray, loss: str = "g"):
         """(loss[r], dL/du as (nxdiso, ny, nreal) Fortran-ordered, status, code) for explicit host
         potentials; complex128 adjoint sweep."""
-        u = np.asfortranarray(upot, dtype=np.float64)
+        u, nreal = self._upot(upot)
         if u.ndim == 2:
             u = u[:, :, None]
-        nreal = u.shape[2]
         self.adjoint_reserve(min(nreal, self.batch_max))
         val = np.zeros(nreal)
         grad = np.zeros((nreal, self.ny, self.nxdiso))
@@ -376,6 +381,10 @@ class Plan:
         (nxdiso, ny, nreal) Fortran layout).  Returns (loss, grad) tensors; asynchronous on stream()."""
         import torch
 
+        if (not isinstance(upot, torch.Tensor) or upot.dtype != torch.float64 or not upot.is_cuda or upot.dim() != 3
+                or tuple(upot.shape[1:]) != (self.ny, self.nxdiso) or upot.shape[0] < 1 or not upot.is_contiguous()):
+            raise RecurGreenError(RG_EINVAL, f"adjoint_device expects a contiguous float64 CUDA tensor of shape "
+                                             f"(nreal, {self.ny}, {self.nxdiso})")
         nreal = upot.shape[0]
         self.adjoint_reserve(min(nreal, self.batch_max))
         val = torch.empty(nreal, dtype=torch.float64, device=upot.device)

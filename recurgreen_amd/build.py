@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librecurgreen_hip.so")
-SOURCES = ["rg_plan.hip", "rg_kernels_f64.hip", "rg_blocked.hip", "rg_gj.hip", "rg_project.hip", "rg_eig.hip", "rg_zgemm.hip", "rg_refl.hip", "rg_adjoint.hip", "rg_nopiv.hip", "rg_dyson32.hip", "rg_schur.hip", "rg_tprofile.hip", "rg_dyson64.hip", "rg_dyson_bf3.hip"]
+SOURCES = ["rg_plan.hip", "rg_svd.hip", "rg_kernels_f64.hip", "rg_blocked.hip", "rg_gj.hip", "rg_project.hip", "rg_eig.hip", "rg_zgemm.hip", "rg_refl.hip", "rg_adjoint.hip", "rg_nopiv.hip", "rg_dyson32.hip", "rg_schur.hip", "rg_tprofile.hip", "rg_dyson64.hip", "rg_dyson_bf3.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops is switched off for every HIP object: the v_pk_*_f32 forms are avoided.
@@ -24,16 +24,27 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
+def build(verbose=False, only_changed=False):
     """Compile every HIP source for gfx950 (one hipcc per file, in parallel) and link the
-    shared library in-tree; raises on failure."""
+    shared library in-tree; raises on failure.  only_changed: recompile only the sources newer than
+    their object (or than rg_internal.h / the ABI header)."""
     from concurrent.futures import ThreadPoolExecutor
 
     odir = os.path.join(HERE, "_build")
     os.makedirs(odir, exist_ok=True)
     objs = [os.path.join(odir, os.path.splitext(f)[0] + ".o") for f in SOURCES]
 
+    hdrs = [os.path.join(CSRC, "rg_internal.h"), os.path.join(HERE, "..", "include", "recurgreen_hip.h")]
+    newest_hdr = max(os.path.getmtime(h) for h in hdrs)
+
+    def stale(i):
+        src = os.path.join(CSRC, SOURCES[i])
+        return (not only_changed or not os.path.exists(objs[i])
+                or os.path.getmtime(objs[i]) < max(os.path.getmtime(src), newest_hdr))
+
     def compile_one(i):
+        if not stale(i):
+            return None, None
         cmd = [HIPCC, f"--offload-arch={ARCH}", *FLAGS, "-c", os.path.join(CSRC, SOURCES[i]), "-o", objs[i]]
         if verbose:
             print(" ".join(cmd))
@@ -42,7 +53,7 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
         results = list(ex.map(compile_one, range(len(SOURCES))))
     for cmd, res in results:
-        if res.returncode != 0:
+        if res is not None and res.returncode != 0:
             sys.stderr.write(" ".join(cmd) + "\n" + res.stdout + res.stderr)
             raise RuntimeError(f"hipcc failed compiling {os.path.basename(cmd[-3])}")
     tmp = LIB + ".tmp"
@@ -82,5 +93,6 @@ def build_fortran(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    quick = "--changed" in sys.argv
+    print(build(verbose=True, only_changed=quick))
     print(build_fortran(verbose=True))

@@ -108,6 +108,8 @@ __device__ inline void rg_load_cq(double *s_cq, const rg_cqv &v, int b, int tid,
     }
 }
 
+enum rg_eigen_method { RG_EIG_GRAM = 0, RG_EIG_JACOBI = 1, RG_EIG_BIDIAG = 2, RG_EIG_AUTO = 3 };
+
 enum rg_inverse_kernel { RG_INV_REGISTER = 0, RG_INV_LOOKAHEAD = 1, RG_INV_BLOCKED = 2, RG_INV_NOPIV = 3 };
 
 struct rg_plan {
@@ -166,7 +168,10 @@ struct rg_plan {
     int profile_mask = ~0;  // bit k: record launches of timer k (rg_kernel_id) while profiling
     // slice-inversion kernel per state precision (RG_INV_*): FP64 state, complex64 state
     int inv_fp64 = 0, inv_mixed = 3;
-    bool eig_jacobi = false;  // transmission eigenvalues by one-sided Jacobi SVD of t instead of eig(t^H t)
+    // transmission eigenvalues (option eigen, rg_eigen_method): RG_EIG_GRAM eig(t^H t), RG_EIG_JACOBI one-sided
+    // Jacobi SVD of t, RG_EIG_BIDIAG Householder bidiagonalisation of t (order <= 128, Jacobi above), RG_EIG_AUTO
+    // the bidiagonal SVD for complex128 sweeps and the Gram eigensolver for complex64 sweeps
+    int eigen = 3;
     // complex64 Dyson product (option dyson): 0 64 x 64 tiles, 1 one realization per workgroup on FP32
     // MFMA (NP >= 80), 2 the same on BF16 MFMA with three-way split operands (NP 96 and 128, else 1)
     int gemm_wg = 1;
@@ -259,6 +264,13 @@ void rg_launch_topvec_jacobi(rg_plan *P, int nb, const float2 *gs1, double2 *xwo
                              int xstride);
 void rg_launch_modal_column(rg_plan *P, const double2 *y, double2 *psi);
 void rg_launch_add_c128(rg_plan *P, double2 *a, const double2 *b, size_t count);
+// rg_svd.hip: singular values of t by Householder bidiagonalisation, orders <= 128
+bool rg_tsvd_supported(const rg_plan *P);
+void rg_launch_tsvd_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status);
+void rg_launch_tsvd_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status);
+// Transmission eigenvalues of G(L;0) by the plan's method (rg_plan.hip)
+void rg_launch_teig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status);
+void rg_launch_teig_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status);
 // rg_eig.hip
 void rg_launch_tdt_eig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status);
 void rg_launch_tdt_eig_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status);

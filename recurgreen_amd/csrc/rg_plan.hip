@@ -289,13 +289,13 @@ void sweep_step(rg_plan *P, int nb, int ix, int slot, int ixd, int64_t seed0, co
 // teig, or the t matrices into tmat_out.
 void sweep_finish(rg_plan *P, int nb, int precision, int cur, double *teig, double2 *tmat_out)
 {
-    if (tmat_out || P->eig_jacobi || !rg_eig_supported(P)) {
+    if (tmat_out) {
         if (precision == RG_FP64) rg_launch_fisher_lee_svd_f64(P, nb, P->d_gs1[cur], teig, P->d_status, tmat_out);
         else rg_launch_fisher_lee_svd_f32(P, nb, P->d_gs1_32[cur], teig, P->d_status, tmat_out);
     } else if (precision == RG_FP64) {
-        rg_launch_tdt_eig_f64(P, nb, P->d_gs1[cur], teig, P->d_status);
+        rg_launch_teig_f64(P, nb, P->d_gs1[cur], teig, P->d_status);
     } else {
-        rg_launch_tdt_eig_f32(P, nb, P->d_gs1_32[cur], teig, P->d_status);
+        rg_launch_teig_f32(P, nb, P->d_gs1_32[cur], teig, P->d_status);
     }
 }
 
@@ -330,7 +330,7 @@ static void lane_view(const rg_plan *P, rg_plan *L, int off, int nb)
     L->inv_mixed = P->inv_mixed;
     L->gemm_wg = P->gemm_wg;
     L->dyson64 = P->dyson64;
-    L->eig_jacobi = P->eig_jacobi;
+    L->eigen = P->eigen;
     L->profiling = P->profiling;
     L->profile_mask = P->profile_mask;
     (void)nb;
@@ -343,7 +343,7 @@ __global__ void k_set_seed(int64_t *__restrict__ dst, int64_t v) { *dst = v; }
 // Options that change the kernels a sweep launches (part of a captured sweep's key).
 static int graph_opts(const rg_plan *P)
 {
-    return P->inv_fp64 | P->inv_mixed << 4 | P->cq_parts << 9 | (P->eig_jacobi ? 1 : 0) << 13 | P->dyson64 << 14 |
+    return P->inv_fp64 | P->inv_mixed << 4 | P->cq_parts << 9 | P->eigen << 20 | P->dyson64 << 14 |
            P->gemm_wg << 16;
 }
 
@@ -846,6 +846,14 @@ struct uneven_real {
 // realization taking its last diagonal on its own last slice (nx_r) -- split only where the
 // disorder region ends (free slices).  Realizations whose last slice it was have their G(L;0)
 // copied into their own (now unused) Gss slot; one eigenvalue launch per chunk follows the sweep.
+// HIP error inside run_uneven_group: free the group's device buffers before returning.
+#define UCHK(x)                                       \
+    do {                                              \
+        if ((x) != hipSuccess) {                      \
+            release();                                \
+            return RG_EDEVICE;                        \
+        }                                             \
+    } while (0)
 rg_status run_uneven_group(rg_plan *P, std::vector<uneven_real> &rs, int precision, double *tallvals,
                            const int64_t *offsets, int32_t *status, int32_t *worst)
 {
@@ -898,13 +906,13 @@ rg_status run_uneven_group(rg_plan *P, std::vector<uneven_real> &rs, int precisi
             hsd[i] = r.stdu;
             hnx[i] = r.nx;
         }
-        HIPCHK(hipMemcpy(d_ad, had.data(), sizeof(double2) * (size_t)nb * 3 * np, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_b, hb.data(), sizeof(double2) * (size_t)nb * np, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_sm, hsm.data(), sizeof(double2) * (size_t)nb * np, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_seeds, hseed.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_sd, hsd.data(), sizeof(double) * nb, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_nxr, hnx.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
-        HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
+        UCHK(hipMemcpy(d_ad, had.data(), sizeof(double2) * (size_t)nb * 3 * np, hipMemcpyHostToDevice));
+        UCHK(hipMemcpy(d_b, hb.data(), sizeof(double2) * (size_t)nb * np, hipMemcpyHostToDevice));
+        UCHK(hipMemcpy(d_sm, hsm.data(), sizeof(double2) * (size_t)nb * np, hipMemcpyHostToDevice));
+        UCHK(hipMemcpy(d_seeds, hseed.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice));
+        UCHK(hipMemcpy(d_sd, hsd.data(), sizeof(double) * nb, hipMemcpyHostToDevice));
+        UCHK(hipMemcpy(d_nxr, hnx.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
+        UCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
         int cur = 0;
         const int nxmax = rs[c0].nx;
         for (int ix = 1; ix <= nxmax; ix++) {
@@ -937,12 +945,12 @@ rg_status run_uneven_group(rg_plan *P, std::vector<uneven_real> &rs, int precisi
                     rg_launch_copy_f2(P, P->d_gs1_32[cur] + o, P->d_gss32 + o, n2);
             }
         }
-        if (precision == RG_FP64) rg_launch_tdt_eig_f64(&V, nb, P->d_gss, P->d_teig, P->d_status);
-        else rg_launch_tdt_eig_f32(&V, nb, P->d_gss32, P->d_teig, P->d_status);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(te.data(), P->d_teig, sizeof(double) * (size_t)nb * neigv, hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipMemcpyAsync(sv.data(), P->d_status, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
+        if (precision == RG_FP64) rg_launch_teig_f64(&V, nb, P->d_gss, P->d_teig, P->d_status);
+        else rg_launch_teig_f32(&V, nb, P->d_gss32, P->d_teig, P->d_status);
+        UCHK(hipGetLastError());
+        UCHK(hipMemcpyAsync(te.data(), P->d_teig, sizeof(double) * (size_t)nb * neigv, hipMemcpyDeviceToHost, P->stream));
+        UCHK(hipMemcpyAsync(sv.data(), P->d_status, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, P->stream));
+        UCHK(hipStreamSynchronize(P->stream));
         for (int i = 0; i < nb; i++) {
             const int idx = rs[c0 + i].idx;
             if (status) status[idx] = sv[i];
@@ -953,8 +961,37 @@ rg_status run_uneven_group(rg_plan *P, std::vector<uneven_real> &rs, int precisi
     release();
     return RG_OK;
 }
+#undef UCHK
 
 }  // namespace
+
+// Method of the transmission eigenvalues for a state precision (option eigen).
+static int eigen_method(const rg_plan *P, bool fp64)
+{
+    int e = P->eigen == RG_EIG_AUTO ? (fp64 ? RG_EIG_BIDIAG : RG_EIG_GRAM) : P->eigen;
+    if (e == RG_EIG_BIDIAG && !rg_tsvd_supported(P)) e = RG_EIG_JACOBI;
+    if (e == RG_EIG_GRAM && !rg_eig_supported(P)) e = RG_EIG_JACOBI;
+    return e;
+}
+
+void rg_launch_teig_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status)
+{
+    switch (eigen_method(P, true)) {
+        case RG_EIG_BIDIAG: rg_launch_tsvd_f64(P, nb, gs1, teig, status); break;
+        case RG_EIG_GRAM: rg_launch_tdt_eig_f64(P, nb, gs1, teig, status); break;
+        default: rg_launch_fisher_lee_svd_f64(P, nb, gs1, teig, status, nullptr);
+    }
+}
+
+void rg_launch_teig_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status)
+{
+    switch (eigen_method(P, false)) {
+        case RG_EIG_BIDIAG: rg_launch_tsvd_f32(P, nb, gs1, teig, status); break;
+        case RG_EIG_GRAM: rg_launch_tdt_eig_f32(P, nb, gs1, teig, status); break;
+        default: rg_launch_fisher_lee_svd_f32(P, nb, gs1, teig, status, nullptr);
+    }
+}
+
 
 // ------------------------------------------------------------------------------------------------
 // Timing.
@@ -1035,9 +1072,12 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
     // lookahead, blocked}, RG_INVERSE_MIXED in {lookahead, blocked, verified}.
     P->inv_fp64 = inverse_choice(std::getenv("RG_INVERSE_FP64"), P->inv_fp64, true);
     P->inv_mixed = inverse_choice(std::getenv("RG_INVERSE_MIXED"), P->inv_mixed, false);
-    // Transmission eigenvalues: Hermitian eigensolver of t^H t (default) or RG_EIGEN=jacobi.
+    // Transmission eigenvalues: RG_EIGEN in {gram, jacobi, bidiag} overrides the default (bidiagonal SVD
+    // of t for complex128 sweeps, eigenvalues of t^H t for complex64 sweeps).
     const char *eg = std::getenv("RG_EIGEN");
-    P->eig_jacobi = eg && std::strcmp(eg, "jacobi") == 0;
+    if (eg && std::strcmp(eg, "gram") == 0) P->eigen = RG_EIG_GRAM;
+    else if (eg && std::strcmp(eg, "jacobi") == 0) P->eigen = RG_EIG_JACOBI;
+    else if (eg && std::strcmp(eg, "bidiag") == 0) P->eigen = RG_EIG_BIDIAG;
     // complex64 Dyson product: RG_DYSON=tile64 for the 64 x 64 tiled kernel, RG_DYSON=bf16x6 for the
     // BF16 matrix-core kernel, instead of one workgroup per realization on FP32 MFMA.
     const char *dy = std::getenv("RG_DYSON");
@@ -1157,7 +1197,7 @@ rg_status rg_tdistrib_uneven(const rg_params *params, const int64_t *seeds, int3
         std::vector<uneven_real> &rs = kv.second;
         rg_plan *P = nullptr;
         const int cap = (int)std::min<size_t>((size_t)batch_max, rs.size());
-        // mode counts above 128 are refused before any device memory is taken
+        // mode counts above 256 are refused before any device memory is taken
         rg_status st = rs[0].b.size() > 256 ? RG_ENOTSUP : rg_plan_create(&params[rs[0].idx], cap, device, &P);
         if (!st) {
             DeviceGuard g(P->device);
@@ -1620,8 +1660,8 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
         }
         P->lanes = value;
     } else if (std::strcmp(key, "eigen") == 0) {
-        if (value != 0 && value != 1) return RG_EINVAL;
-        P->eig_jacobi = value == 1;
+        if (value < RG_EIG_GRAM || value > RG_EIG_AUTO) return RG_EINVAL;
+        P->eigen = value;
     } else if (std::strcmp(key, "profile_mask") == 0) {
         P->profile_mask = value;
     } else if (std::strcmp(key, "graph") == 0) {
@@ -1645,7 +1685,7 @@ rg_status rg_plan_get_option(const rg_plan *P, const char *key, int32_t *value)
     else if (std::strcmp(key, "dyson") == 0) *value = P->gemm_wg;
     else if (std::strcmp(key, "project_parts") == 0) *value = P->cq_parts;
     else if (std::strcmp(key, "lanes") == 0) *value = P->lanes;
-    else if (std::strcmp(key, "eigen") == 0) *value = P->eig_jacobi ? 1 : 0;
+    else if (std::strcmp(key, "eigen") == 0) *value = P->eigen;
     else if (std::strcmp(key, "profile_mask") == 0) *value = P->profile_mask;
     else if (std::strcmp(key, "graph") == 0) *value = P->use_graph ? 1 : 0;
     else if (std::strcmp(key, "adjoint_precision") == 0) *value = P->adj_prec;

@@ -145,7 +145,7 @@ contains
         histo = histo/(sum(histo)*(bins(2:nbins+1) - bins(1:nbins)))
         rho(:, 1) = (bins(1:nbins) + bins(2:nbins+1))/2
         rho(:, 2) = histo
-        call save_files(wguide, rho, tallvals, nseed, nbins, time, speed, save_eigvals)
+        call save_files(wguide, rho, tallvals, nseed, nbins, nthreads, time, speed, save_eigvals)
         call wguide%del()
     end subroutine
 
@@ -542,24 +542,19 @@ contains
         write (output_unit, '(a)') "[INFO] Average profiles written in file: " // trim(filename)
     end subroutine
 
-    subroutine save_files(wguide, rho, tallvals, nseed, nbins, time, speed, save_eigvals)
+    !! save_distrib and save_raw_eigvals (src/main.f90:257-328): the %% header (timestamp, parameters,
+    !! Nseed/Nbins/Nthreads/Time/Speed), then "tm, rho" rows, or the eigenvalue count and one value a
+    !! line; each file takes the smallest free N of its own name pattern, as unique_filename does.
+    subroutine save_files(wguide, rho, tallvals, nseed, nbins, nthreads, time, speed, save_eigvals)
         type(waveguide_gpu_t), intent(in) :: wguide
         real(wp), intent(in) :: rho(:, :), tallvals(:), time, speed
-        integer, intent(in) :: nseed, nbins
+        integer, intent(in) :: nseed, nbins, nthreads
         logical, intent(in) :: save_eigvals
         character(len=200) :: filename
-        integer :: outfp, k, ifile
-        logical :: fexist
-        call execute_command_line("mkdir -p out_gpu/distrib")
-        do ifile = 1, 1000000   !! unique_filename (src/base_utils.f90:825-840)
-            write (filename, '(a,i0,a)') "out_gpu/distrib/result_", ifile, ".distrib.csv"
-            inquire(file=filename, exist=fexist)
-            if (.not. fexist) exit
-        end do
-        open(newunit=outfp, file=filename, action='write')
-        call wguide%print_parameters(outfp, "%%")
-        write (outfp, '(2(a,i0),2(a,g0.6),a)') "%% Nseed=", nseed, ", Nbins=", nbins, ", Time=", time, &
-            " s, Speed=", speed, " seed/s"
+        integer :: outfp, k
+        call open_result(wguide, "distrib", ".distrib.csv", filename, outfp)
+        write (outfp, '(3(a,i0),2(a,g0.6),a)') "%% Nseed=", nseed, ", Nbins=", nbins, ", Nthreads=", nthreads, &
+            ", Time=", time, " s, Speed=", speed, " seed/s"
         write (outfp, '(a)') "tm, rho"
         do k = 1, size(rho, 1)
             write (outfp, '(g0.16,a,g0.16)') rho(k, 1), ", ", rho(k, 2)
@@ -567,9 +562,9 @@ contains
         close(outfp)
         write (output_unit, '(a)') "[INFO] Result written in file: " // trim(filename)
         if (save_eigvals) then
-            write (filename, '(a,i0,a)') "out_gpu/distrib/result_", ifile, ".eigvals.csv"
-            open(newunit=outfp, file=filename, action='write')
-            call wguide%print_parameters(outfp, "%%")
+            call open_result(wguide, "distrib", ".eigvals.csv", filename, outfp)
+            write (outfp, '(3(a,i0),2(a,g0.6),a)') "%% Nseed=", nseed, ", Nbins=", nbins, ", Nthreads=", nthreads, &
+                ", Time=", time, " s, Speed=", speed, " seed/s"
             write (outfp, '(a,i0,a)') "%% Transmission eigenvalues (", size(tallvals), " values)"
             do k = 1, size(tallvals)
                 write (outfp, '(g0.16)') tallvals(k)

@@ -60,3 +60,19 @@ def test_chunk_checks_of_compute_tdistrib():
         wg.compute_tdistrib(1, np.zeros(12))
     with pytest.raises(binding.RecurGreenError, match=r"expected in \[1, 2\]"):
         wg.compute_tdistrib(3, np.zeros(16))
+
+
+def test_potential_shape_checked_before_the_library_reads_it():
+    """Every explicit-potential entry point takes upot(ix, iy[, r]) of shape (nxdiso, ny[, nreal]); any
+    other shape is refused with RG_EINVAL on the host, before a pointer reaches the library."""
+    from types import SimpleNamespace
+
+    fake = SimpleNamespace(nxdiso=7, ny=5)
+    u, n = binding.Plan._upot(fake, np.zeros((7, 5)))
+    assert n == 1 and u.flags.f_contiguous and u.dtype == np.float64
+    u, n = binding.Plan._upot(fake, np.zeros((7, 5, 3), dtype=np.float32))
+    assert n == 3 and u.dtype == np.float64
+    for bad in (np.zeros((5, 7)), np.zeros(35), np.zeros((7, 5, 0)), np.zeros((7, 5, 2, 1)), np.zeros((7, 6, 2))):
+        with pytest.raises(binding.RecurGreenError) as e:
+            binding.Plan._upot(fake, bad)
+        assert e.value.code == binding.RG_EINVAL

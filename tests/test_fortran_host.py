@@ -78,6 +78,10 @@ def test_fortran_tdistrib_matches_python_path(tmp_path):
     ow = O.Waveguide("dirichlet", nxdiso=200, wol=10.25, aratio=10 / 10.25, dscat=2.0, ny=205, dos_model=1)
     To = ow.tdistrib_batch(1, 16, nthreads=4)
     np.testing.assert_allclose(T, To, rtol=1e-10, atol=1e-13)
+    for path in (eig, dis):  # the %% header of save_distrib / save_raw_eigvals (src/main.f90:277-281,317-320)
+        head = [ln for ln in path.read_text().splitlines() if ln.startswith("%%")]
+        assert head[0].startswith("%% Computed on ") and " at " in head[0], head[0]
+        assert any(ln.startswith("%% Nseed=16, Nbins=64, Nthreads=") and " seed/s" in ln for ln in head), head
     rho = distrib.load_distrib(str(dis))
     ref, miss = O.prepare_distrib(T, 64, "chebyshev")
     np.testing.assert_allclose(rho, ref, rtol=1e-12)

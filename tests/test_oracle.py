@@ -336,6 +336,35 @@ def test_filtered_apertures():
     assert T.size == w.neigv == min(w.imax, w.jmax)
 
 
+def test_fp64_sweep_resolves_sigma_to_eps_sigma_max():
+    """The error model behind the FP64 parity bound |dT| <= 1e-10 T + 2e-13 sqrt(T T_max) + 1e-26 T_max
+    (1e-10 relative, or |d sigma| <= 1e-13 sigma_max for sigma = sqrt(T)): on a long,
+    localized guide (N = 20, L / l_s = 25, T over ~20 decades) the oracle (its own sweep, Jacobi SVD)
+    and an independent restatement (numpy inverses and products in another order, LAPACK zgesvd as the
+    reference calls it, src/base_utils.f90:365) agree within that bound; they differ by more than 1e-10
+    relative on the smallest T, so no FP64 sweep can be held to a relative bar there; and the
+    eigenvalues of the Gram matrix t^H t fail the bound by orders of magnitude (src/base_utils.f90:385)."""
+    nx = 1000
+    w = O.Waveguide("dirichlet", nxdiso=nx, wol=10.25, aratio=nx * 0.05 / 10.25, dscat=25.0, ny=205, dos_model=1)
+    u = w.potential(3)
+    To = w.tdistrib_potential(u)
+    b = w.b
+    Gss = np.linalg.inv(w.slice_matrix(1, u))
+    Gs1 = Gss.copy()
+    for ix in range(2, w.nx + 1):
+        Gss = np.linalg.inv(w.slice_matrix(ix, u) - b[:, None] * Gss * b[None, :])
+        Gs1 = -Gss @ (b[:, None] * Gs1)
+    sq = np.sqrt(w.mu)
+    t = 2j * sq[:, None] * Gs1 * sq[None, :]
+    Tl = scipy.linalg.svd(t, compute_uv=False, lapack_driver="gesvd") ** 2
+    bound = 1e-10 * To + 2e-13 * np.sqrt(To * To.max()) + 1e-26 * To.max()
+    assert To.max() / To.min() > 1e18
+    assert np.all(np.abs(Tl - To) <= bound)
+    assert np.max(np.abs(Tl - To) / To) > 1e-10
+    Tg = np.sort(np.linalg.eigvalsh(t.conj().T @ t))[::-1]
+    assert np.max(np.abs(Tg - To) / bound) > 100
+
+
 def test_batch_matches_single_and_is_order_free():
     w = O.Waveguide("dirichlet", nxdiso=50, wol=4.3, dscat=1.0, ny=17)
     Tb = w.tdistrib_batch(5, 6, nthreads=3).reshape(6, -1)
