@@ -176,6 +176,18 @@ rg_status rg_histogram(rg_plan *plan, const double *samples, int64_t nsamp, int3
                        const double *bins, int32_t nbins, int64_t *counts, int64_t *missing,
                        int64_t *dropped);
 
+/* No counterpart in the reference: rg_histogram for device-resident samples that allocates nothing
+ * (no plan, no hipMalloc/hipFree, no host buffers), for callers that cannot allocate -- code that
+ * runs between a graph capture's set-up and its launch, serving loops, harnesses that forbid
+ * allocation.  samples_dev: nsamp doubles on GPU `device`; bins (host): nbins + 1 sorted edges,
+ * 1 <= nbins <= 4096 (RG_ENOTSUP above); stream: a hipStream_t on that device (NULL: the null
+ * stream).  Same counting rules and outputs (host) as rg_histogram; returns after the stream has
+ * reached the end of the call.  Scratch: a static 64 KiB device array of the library, one per GPU;
+ * calls that overlap on one GPU race on it. */
+rg_status rg_histogram_noalloc(int32_t device, void *stream, const double *samples_dev, int64_t nsamp,
+                               const double *bins, int32_t nbins, int64_t *counts, int64_t *missing,
+                               int64_t *dropped);
+
 /* Stream the plan launches on (a hipStream_t), for callers that order their own work. */
 rg_status rg_plan_stream(rg_plan *plan, void **stream);
 rg_status rg_plan_synchronize(rg_plan *plan);

@@ -24,7 +24,7 @@ EXPORTS = (
     "rg_tdistrib_potential", "rg_transmission_potential", "rg_refl_trans", "rg_refl_trans_potential", "rg_deposition", "rg_deposition_potential", "rg_tprofile_reserve", "rg_tprofile", "rg_tprofile_potential", "rg_wavefunction", "rg_wavefunction_potential",
     "rg_tdistrib_uneven",
     "rg_adjoint_reserve", "rg_adjoint_potential", "rg_adjoint_potential_device",
-    "rg_generate_potential", "rg_histogram",
+    "rg_generate_potential", "rg_histogram", "rg_histogram_noalloc",
     "rg_plan_stream", "rg_plan_synchronize", "rg_plan_profile", "rg_plan_kernel_times",
     "rg_plan_set_option", "rg_plan_get_option", "rg_status_string", "rg_version",
 )
@@ -92,6 +92,7 @@ def lib():
         "rg_wavefunction_potential": [vp, vp, i32, i32, vp, vp],
         "rg_generate_potential": [vp, i64, i32, vp],
         "rg_histogram": [vp, vp, i64, i32, dp, i32, i64p, i64p, i64p],
+        "rg_histogram_noalloc": [i32, vp, vp, i64, dp, i32, i64p, i64p, i64p],
         "rg_plan_stream": [vp, C.POINTER(vp)],
         "rg_plan_synchronize": [vp],
         "rg_plan_profile": [vp, i32],
@@ -167,6 +168,20 @@ def tdistrib_uneven(params: list, seeds, precision: str = "fp64", device: int = 
                                     st.ctypes.data_as(C.POINTER(C.c_int32)))
     vals = [out[off[r]:off[r + 1]].copy() for r in range(n)]
     return vals, st[:n], code
+
+
+def histogram_noalloc(samples, bins: np.ndarray, device: int = 0, stream: int = 0):
+    """rg_histogram_noalloc: (counts, missing, dropped) of device-resident samples (a CUDA tensor of
+    float64 on `device`) on the edges `bins`, with no plan and no allocation in the library; `stream`
+    is a hipStream_t handle (0: the null stream)."""
+    b = np.ascontiguousarray(bins, dtype=np.float64)
+    nb = len(b) - 1
+    counts = np.zeros(max(nb, 1), np.int64)
+    miss, drop = C.c_int64(0), C.c_int64(0)
+    check(lib().rg_histogram_noalloc(int(device), C.c_void_p(int(stream)), ptr(samples), int(samples.numel()),
+                                     b.ctypes.data_as(C.POINTER(C.c_double)), nb,
+                                     counts.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(miss), C.byref(drop)))
+    return counts[:nb], miss.value, drop.value
 
 
 def _neigv_of(p: rg_params) -> int:

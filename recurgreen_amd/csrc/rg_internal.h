@@ -300,6 +300,9 @@ bool rg_launch_gemm_bf3(rg_plan *P, int nb, const float2 *gss, const float2 *gs1
 void rg_launch_copy_f2(rg_plan *P, const float2 *src, float2 *dst, size_t n);
 void rg_launch_histogram(rg_plan *P, const double *d_samples, int64_t n, const double *d_bins, int nbins,
                          unsigned long long *d_counts);
+// the same on any stream (counts + missing + dropped added into d_counts, nbins <= 4096)
+void rg_launch_histogram_on(hipStream_t stream, const double *d_samples, int64_t n, const double *d_bins, int nbins,
+                            unsigned long long *d_counts);
 void rg_launch_potential(rg_plan *P, int nb, int64_t seed0, double *d_upot);
 
 rg_cqv rg_cq_view(const rg_plan *P, int nb, bool with_disorder);

@@ -656,16 +656,22 @@ void rg_launch_fisher_lee_svd_f32(rg_plan *P, int nb, const float2 *gs1, double 
     launch_fisher_lee_svd<float2>(P, nb, gs1, teig, status, tmat_out);
 }
 
+void rg_launch_histogram_on(hipStream_t stream, const double *d_samples, int64_t n, const double *d_bins, int nbins,
+                            unsigned long long *d_counts)
+{
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    const size_t shm = sizeof(double) * (nbins + 1) + sizeof(unsigned) * (nbins + 2);
+    hipLaunchKernelGGL(k_histogram, dim3((unsigned)blocks), dim3(256), shm, stream, d_samples, n, d_bins, nbins,
+                       d_counts);
+}
+
 void rg_launch_histogram(rg_plan *P, const double *d_samples, int64_t n, const double *d_bins, int nbins,
                          unsigned long long *d_counts)
 {
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_HIST, &stop);
-    int64_t blocks = (n + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
-    if (blocks < 1) blocks = 1;
-    const size_t shm = sizeof(double) * (nbins + 1) + sizeof(unsigned) * (nbins + 2);
-    hipLaunchKernelGGL(k_histogram, dim3((unsigned)blocks), dim3(256), shm, P->stream, d_samples, n, d_bins, nbins,
-                       d_counts);
+    rg_launch_histogram_on(P->stream, d_samples, n, d_bins, nbins, d_counts);
     rg_timer_end(P, RGK_HIST, stop);
 }
