@@ -167,7 +167,7 @@ struct rg_plan {
     bool profiling = false;
     int profile_mask = ~0;  // bit k: record launches of timer k (rg_kernel_id) while profiling
     // slice-inversion kernel per state precision (RG_INV_*): FP64 state, complex64 state
-    int inv_fp64 = 0, inv_mixed = 3;
+    int inv_fp64 = 3, inv_mixed = 3;
     // transmission eigenvalues (option eigen, rg_eigen_method): RG_EIG_GRAM eig(t^H t), RG_EIG_JACOBI one-sided
     // Jacobi SVD of t, RG_EIG_BIDIAG Householder bidiagonalisation of t (order <= 128, Jacobi above), RG_EIG_AUTO
     // the bidiagonal SVD for complex128 sweeps and the Gram eigensolver for complex64 sweeps
@@ -282,8 +282,10 @@ void rg_launch_tdt_topvec_f32(rg_plan *P, int nb, const float2 *gs1, int32_t *st
 // rg_gj.hip (only != nullptr: invert just the realizations with only[b] != 0)
 void rg_launch_inverse_gj(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision,
                           const int32_t *only = nullptr);
-// rg_nopiv.hip
+// rg_nopiv.hip (complex64 state) and rg_nopiv64.hip (complex128 state): blocked Gauss-Jordan on verified
+// diagonal pivots; realizations whose check fails are flagged in redo (inverted again by rg_launch_inverse_gj)
 void rg_launch_inverse_nopiv(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo);
+void rg_launch_inverse_nopiv64(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo);
 // rg_blocked.hip
 void rg_launch_inverse_blocked(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
 void rg_launch_gemm_sp(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new, int precision);

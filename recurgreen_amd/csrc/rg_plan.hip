@@ -259,7 +259,8 @@ void sweep_step(rg_plan *P, int nb, int ix, int slot, int ixd, int64_t seed0, co
     else if (inv == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
     else if (inv == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision);
     else if (inv == RG_INV_NOPIV) {
-        rg_launch_inverse_nopiv(P, nb, slot, ix == 1, dis, P->d_redo);
+        if (precision == RG_FP64) rg_launch_inverse_nopiv64(P, nb, slot, ix == 1, dis, P->d_redo);
+        else rg_launch_inverse_nopiv(P, nb, slot, ix == 1, dis, P->d_redo);
         rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision, P->d_redo);
     } else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, precision);
     hipEvent_t stop = nullptr;
@@ -443,8 +444,12 @@ void invert_f64(rg_plan *P, int nb, int slot, int first, bool dis)
 {
     if (P->np > 128) rg_launch_inverse_schur(P, nb, slot, first, dis, RG_FP64);
     else if (P->inv_fp64 == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, first, dis);
-    else if (P->inv_fp64 == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, first, dis, RG_FP64);
-    else rg_launch_inverse_blocked(P, nb, slot, first, dis, RG_FP64);
+    else if (P->inv_fp64 == RG_INV_LOOKAHEAD || (P->inv_fp64 == RG_INV_NOPIV && P->np < 32))
+        rg_launch_inverse_gj(P, nb, slot, first, dis, RG_FP64);
+    else if (P->inv_fp64 == RG_INV_NOPIV) {
+        rg_launch_inverse_nopiv64(P, nb, slot, first, dis, P->d_redo);
+        rg_launch_inverse_gj(P, nb, slot, first, dis, RG_FP64, P->d_redo);
+    } else rg_launch_inverse_blocked(P, nb, slot, first, dis, RG_FP64);
 }
 
 // The reflection/transmission sweep of compute_refl_trans_v1 (src/waveguide.f90:469-519), FP64:
@@ -805,7 +810,7 @@ int inverse_choice(const char *v, int dflt, bool fp64)
     if (fp64 && std::strcmp(v, "register") == 0) return RG_INV_REGISTER;
     if (std::strcmp(v, "lookahead") == 0) return RG_INV_LOOKAHEAD;
     if (std::strcmp(v, "blocked") == 0) return RG_INV_BLOCKED;
-    if (!fp64 && std::strcmp(v, "verified") == 0) return RG_INV_NOPIV;
+    if (std::strcmp(v, "verified") == 0) return RG_INV_NOPIV;
     return dflt;
 }
 
@@ -1068,8 +1073,8 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
     P->device = device;
     P->batch_max = batch_max;
     for (int i = 0; i < RGK_COUNT; i++) P->timers[i].name = kTimerNames[i];
-    // Slice-inversion kernels, overridable for measurement: RG_INVERSE_FP64 in {register,
-    // lookahead, blocked}, RG_INVERSE_MIXED in {lookahead, blocked, verified}.
+    // Slice-inversion kernels, overridable for measurement: RG_INVERSE_FP64 and RG_INVERSE_MIXED in
+    // {lookahead, blocked, verified}, RG_INVERSE_FP64 also register.
     P->inv_fp64 = inverse_choice(std::getenv("RG_INVERSE_FP64"), P->inv_fp64, true);
     P->inv_mixed = inverse_choice(std::getenv("RG_INVERSE_MIXED"), P->inv_mixed, false);
     // Transmission eigenvalues: RG_EIGEN in {gram, jacobi, bidiag} overrides the default (bidiagonal SVD
@@ -1394,7 +1399,8 @@ static rg_status tprofile_pass(rg_plan *P, int64_t seed0, const double *upot, in
     DeviceGuard g(P->device);
     const size_t nvr = filtered ? (size_t)(P->jmax - P->jmin + 1) : (size_t)P->nmu;
     const size_t per = (size_t)P->nxdiso * P->ny, n = nvr, nxn = (size_t)P->nx * nvr;
-    const int cap = upot ? std::min(P->upot_cap, P->tp_cap) : P->tp_cap;
+    // a pass also uses the batch buffers (state, status, redo flags) of batch_max realizations
+    const int cap = std::min(upot ? std::min(P->upot_cap, P->tp_cap) : P->tp_cap, P->batch_max);
     int32_t worst = 0;
     for (int32_t off = 0; off < nreal; off += cap) {
         const int nb = std::min(cap, nreal - off);
@@ -1638,7 +1644,8 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
     DeviceGuard g(P->device);
     (void)hipStreamSynchronize(P->stream);
     if (std::strcmp(key, "inverse_fp64") == 0) {
-        if (value != RG_INV_REGISTER && value != RG_INV_LOOKAHEAD && value != RG_INV_BLOCKED) return RG_EINVAL;
+        if (value != RG_INV_REGISTER && value != RG_INV_LOOKAHEAD && value != RG_INV_BLOCKED && value != RG_INV_NOPIV)
+            return RG_EINVAL;
         P->inv_fp64 = value;
     } else if (std::strcmp(key, "inverse_mixed") == 0) {
         if (value != RG_INV_LOOKAHEAD && value != RG_INV_BLOCKED && value != RG_INV_NOPIV) return RG_EINVAL;

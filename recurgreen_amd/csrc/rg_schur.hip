@@ -235,7 +235,12 @@ void invert_block(const rg_plan *P, int nb, int no, E *mats, int precision)
     V.d_b = P->d_schur_i;
     if (precision == RG_FP64) {
         V.d_gss = reinterpret_cast<double2 *>(mats);
-        rg_launch_inverse_f64(&V, nb, 0, 0, false);
+        if (P->inv_fp64 == RG_INV_NOPIV && no >= 32) {
+            rg_launch_inverse_nopiv64(&V, nb, 0, 0, false, V.d_redo);
+            rg_launch_inverse_gj(&V, nb, 0, 0, false, precision, V.d_redo);
+        } else {
+            rg_launch_inverse_f64(&V, nb, 0, 0, false);
+        }
     } else {
         V.d_gss32 = reinterpret_cast<float2 *>(mats);
         if (no >= 32) {
