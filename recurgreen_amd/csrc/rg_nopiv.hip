@@ -16,7 +16,7 @@
 // and imaginary planes), register v of lane l holding row 4(l/16) + v, column l%16
 // (v_mfma_f32_16x16x4_f32).  Block step K (D = tile (K,K), P = column block K):
 //   wavefront K: Gauss-Jordan on D in registers (16 unrolled steps, cross-lane broadcasts by
-//   v_permlane32/16_swap and DPP, pivot checks after the last step) -> D^-1, pivots, and
+//   ds_bpermute and DPP, pivot checks after the last step) -> D^-1, pivots, and
 //   W = D^-1 L_D diag(u) = U_D^-1 diag(u); writes P (rows outside K), D^-1 and W to LDS;
 //   every wavefront: Rt = D^-1 X[K, w] (its own tile K), X[K, w] := Rt, X[i, w] -= P_i Rt (i != K)
 //   (wavefront K: X[i, K] := -P_i D^-1, X[K, K] := D^-1), and checks the rows of its own tile
@@ -43,27 +43,16 @@ struct NopivShared {
     double ua[2][16];                    // |u_j|_1 (FP64)
     int bad;
     alignas(16) double cq[2 * 256 + 8];
+    double2 bv[NP], ad[NP];  // b and the slice diagonal
 };
 
 __device__ inline float readlane_f(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
-
-// Lane (c, g) <- lane (c, R) of x: row R of the 16 x 4 lane grid broadcast to every row
-// (v_permlane32_swap then v_permlane16_swap of x with itself).
-template <int R>
-__device__ __attribute__((always_inline)) inline float rows_bcast(float x)
-{
-    const unsigned u = __float_as_uint(x);
-    const auto p = __builtin_amdgcn_permlane32_swap(u, u, false, false);  // [r0 r1 r0 r1], [r2 r3 r2 r3]
-    const unsigned t = R < 2 ? p[0] : p[1];
-    const auto q = __builtin_amdgcn_permlane16_swap(t, t, false, false);  // even rows, odd rows
-    return __uint_as_float(R % 2 == 0 ? q[0] : q[1]);
-}
 
 // Lane (c, g) <- lane (J, g): DPP row_newbcast:J.
 template <int J>
 __device__ __attribute__((always_inline)) inline float lane_bcast(float x)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x150 + J, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x150 + J, 0xF, 0xF, true));
 }
 
 // Step J of the Gauss-Jordan elimination of a 16x16 block (xr, xi) held in accumulator layout by
@@ -79,7 +68,7 @@ __device__ __attribute__((always_inline)) inline void gj_step(np_f4 &xr, np_f4 &
     const int c = lane & 15, g = lane >> 4;
     // pivot D[J][J] from its lane; pivot row D[J][c]; column J of my rows D[4g+v][J]
     const float pvr = readlane_f(xr[VJ], J + 16 * G), pvi = readlane_f(xi[VJ], J + 16 * G);
-    const float prr = rows_bcast<G>(xr[VJ]), pri = rows_bcast<G>(xi[VJ]);
+    const float prr = __shfl(xr[VJ], c + 16 * G, 64), pri = __shfl(xi[VJ], c + 16 * G, 64);
     float mr[4], mi[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -99,26 +88,29 @@ __device__ __attribute__((always_inline)) inline void gj_step(np_f4 &xr, np_f4 &
         my_ua = fabs((double)pvr) + fabs((double)pvi);
         my_den = den > 0.f;
     }
+    if (c == J) {
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int row = 4 * g + v;
-        const bool cap = c == J && row >= J;
-        lr[v] = cap ? mr[v] : lr[v];
-        li[v] = cap ? mi[v] : li[v];
+        for (int v = 0; v < 4; v++) {
+            const int row = 4 * g + v;
+            lr[v] = row >= J ? mr[v] : 0.f;
+            li[v] = row >= J ? mi[v] : 0.f;
+        }
     }
+    asm volatile("" : "+v"(lr), "+v"(li));
     // scaled pivot row (column J of X reset to e_J first, so its entry is d)
     float rvr, rvi;
     if (c == J) { rvr = dr; rvi = di; }
     else { rvr = prr * dr - pri * di; rvi = prr * di + pri * dr; }
+    const float keep = c == J ? 0.0f : 1.0f;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const int row = 4 * g + v;
         float mur = mr[v], mui = mi[v];
-        if (row == J) { mur = pvr - 1.0f; mui = pvi; }
-        float xrv = xr[v], xiv = xi[v];
-        if (c == J) { xrv = row == J ? 1.0f : 0.0f; xiv = 0.0f; }
-        xr[v] = xrv - (mur * rvr - mui * rvi);
-        xi[v] = xiv - (mur * rvi + mui * rvr);
+        if (v == VJ && g == G) { mur = pvr - 1.0f; mui = pvi; }
+        float nr = fmaf(mui, rvi, -(mur * rvr));
+        const float ni = fmaf(-mur, rvi, -(mui * rvr));
+        if (v == VJ) nr += (g == G && c == J) ? 1.0f : 0.0f;
+        xr[v] = fmaf(keep, xr[v], nr);
+        xi[v] = fmaf(keep, xi[v], ni);
     }
 }
 
@@ -264,6 +256,10 @@ __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict
     float2 *G = gss + (size_t)b * NP * NP;
 
     if (cqv.cq) rg_load_cq(sh.cq, cqv, b, tid, NT);
+    for (int i = tid; i < NP; i += NT) {
+        sh.bv[i] = i < n ? bvec[i] : cmk(0.0, 0.0);
+        sh.ad[i] = i < n ? adiag[i] : cmk(0.0, 0.0);
+    }
     if (tid == 0) sh.bad = 0;
     __syncthreads();
 
@@ -277,7 +273,7 @@ __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict
             const int row = 16 * ti + 4 * g + v;
             double2 m;
             if (row < n && col < n) {
-                m = row == col ? adiag[row] : cmk(0.0, 0.0);
+                m = row == col ? sh.ad[row] : cmk(0.0, 0.0);
                 if (cqv.cq) {
                     if (boundtype == RG_DIRICHLET) {
                         m.x -= sh.cq[abs(row - col)] - sh.cq[row + col + 2];
@@ -290,7 +286,7 @@ __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict
                 }
                 if (!first) {
                     const float2 gv = G[(size_t)row * NP + col];
-                    m = csub(m, cmul(cmul(bvec[row], cmk((double)gv.x, (double)gv.y)), bvec[col]));
+                    m = csub(m, cmul(cmul(sh.bv[row], cmk((double)gv.x, (double)gv.y)), sh.bv[col]));
                 }
             } else {
                 m = cmk(row == col ? 1.0 : 0.0, 0.0);
