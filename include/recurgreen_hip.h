@@ -213,7 +213,7 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *   "lanes"         1 one stream, 2 the batch split in halves on two streams;
  *   "eigen"         transmission eigenvalues: 0 eigenvalues of t^H t (Householder tridiagonalisation
  *                   and Sturm bisection); 1 one-sided Jacobi SVD of t; 2 Householder bidiagonalisation
- *                   of t and Sturm bisection of its singular values (orders <= 128, Jacobi above);
+ *                   of t and Sturm bisection of its singular values (orders <= 256);
  *                   3 (default) 2 for complex128 sweeps, 0 for complex64 sweeps.  The SVD forms keep
  *                   each sigma within a few n eps sigma_max, as zgesvd (src/base_utils.f90:341-380);
  *                   t^H t loses closed channels (T ~ eps T_max absolute, src/base_utils.f90:385);

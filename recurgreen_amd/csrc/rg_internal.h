@@ -169,7 +169,7 @@ struct rg_plan {
     // slice-inversion kernel per state precision (RG_INV_*): FP64 state, complex64 state
     int inv_fp64 = 3, inv_mixed = 3;
     // transmission eigenvalues (option eigen, rg_eigen_method): RG_EIG_GRAM eig(t^H t), RG_EIG_JACOBI one-sided
-    // Jacobi SVD of t, RG_EIG_BIDIAG Householder bidiagonalisation of t (order <= 128, Jacobi above), RG_EIG_AUTO
+    // Jacobi SVD of t, RG_EIG_BIDIAG Householder bidiagonalisation of t (order <= 256), RG_EIG_AUTO
     // the bidiagonal SVD for complex128 sweeps and the Gram eigensolver for complex64 sweeps
     int eigen = 3;
     // complex64 Dyson product (option dyson): 0 64 x 64 tiles, 1 one realization per workgroup on FP32
@@ -264,7 +264,8 @@ void rg_launch_topvec_jacobi(rg_plan *P, int nb, const float2 *gs1, double2 *xwo
                              int xstride);
 void rg_launch_modal_column(rg_plan *P, const double2 *y, double2 *psi);
 void rg_launch_add_c128(rg_plan *P, double2 *a, const double2 *b, size_t count);
-// rg_svd.hip: singular values of t by Householder bidiagonalisation, orders <= 128
+// rg_svd.hip: singular values of t by Householder bidiagonalisation, orders <= 256 (above 128 with t in
+// the t buffer d_tmat)
 bool rg_tsvd_supported(const rg_plan *P);
 void rg_launch_tsvd_f64(rg_plan *P, int nb, const double2 *gs1, double *teig, int32_t *status);
 void rg_launch_tsvd_f32(rg_plan *P, int nb, const float2 *gs1, double *teig, int32_t *status);

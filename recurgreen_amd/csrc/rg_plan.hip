@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <new>
+#include <thread>
 
 #include "rg_internal.h"
 
@@ -1198,23 +1199,41 @@ rg_status rg_tdistrib_uneven(const rg_params *params, const int64_t *seeds, int3
         groups[{t.p.boundtype, t.nmu, t.ny, t.imin, t.imax, t.jmin, t.jmax, t.neigv, t.nxfree}].push_back(std::move(u));
     }
     if (!tallvals && offsets[nreal] > 0) return RG_EINVAL;
-    for (auto &kv : groups) {
-        std::vector<uneven_real> &rs = kv.second;
+    // The groups run at the same time, each from a host thread of its own on its own plan and stream: a
+    // group whose launches do not fill the device (few or short realizations, or the tail of its sweep
+    // where only the longest remain) leaves room to the others.  Every realization meets the same kernels
+    // and data as in a group of its own, so the results do not depend on this schedule.
+    std::vector<std::vector<uneven_real> *> gl;
+    for (auto &kv : groups) gl.push_back(&kv.second);
+    std::vector<int32_t> gworst(gl.size(), 0);
+    auto run_group = [&](size_t k) {
+        std::vector<uneven_real> &rs = *gl[k];
         rg_plan *P = nullptr;
         const int cap = (int)std::min<size_t>((size_t)batch_max, rs.size());
         // mode counts above 256 are refused before any device memory is taken
         rg_status st = rs[0].b.size() > 256 ? RG_ENOTSUP : rg_plan_create(&params[rs[0].idx], cap, device, &P);
         if (!st) {
             DeviceGuard g(P->device);
-            st = run_uneven_group(P, rs, precision, tallvals, offsets, status, &worst);
+            st = run_uneven_group(P, rs, precision, tallvals, offsets, status, &gworst[k]);
         }
         if (P) rg_plan_destroy(P);
         if (st) {  // the whole group: set-up or device failure
             for (const uneven_real &u : rs)
                 if (status) status[u.idx] = st;
-            if (!worst) worst = st;
+            if (!gworst[k]) gworst[k] = st;
         }
+    };
+    std::vector<std::thread> threads;
+    size_t k0 = 1;
+    try {
+        for (; k0 < gl.size(); k0++) threads.emplace_back(run_group, k0);
+    } catch (...) {  // no thread available: the remaining groups run on this one
     }
+    if (!gl.empty()) run_group(0);
+    for (size_t k = k0; k < gl.size(); k++) run_group(k);
+    for (auto &t : threads) t.join();
+    for (int32_t w : gworst)
+        if (!worst && w) worst = w;
     return worst ? (rg_status)worst : RG_OK;
 }
 
