@@ -49,26 +49,49 @@ def workload_params(args):
                 dscat=L / ls, dabso=0.0, napera=1.0, naperb=1.0, ny=int(round(W / dx)), dos_model=1)
 
 
-def cpu_baseline(params, seconds):
-    from oracle import oracle as O
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def cpu_baseline(params, seconds):
+    """The reference's FP64 slice loop on the host's LAPACK/BLAS (oracle/lapack_sweep.py: zgetrf + zgetri
+    and zgemm per slice, batched over realizations on MKL through torch's CPU backend), timed on a bounded
+    sample of the same workload; the textbook C restatement (oracle/rg_oracle.c, OpenMP over realizations)
+    is timed beside it on a smaller sample."""
+    from oracle import oracle as O
+    from oracle.lapack_sweep import tdistrib_lapack
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    nx = params["nxdiso"]
+    # size the sample from a short guide of the same width (time per realization scales with nx)
+    probe_nx = max(20, nx // 20)
+    pp = dict(params, nxdiso=probe_nx, aratio=params["aratio"] * probe_nx / nx, dscat=params["dscat"] * probe_nx / nx)
+    t0 = time.perf_counter()
+    tdistrib_lapack(O.Waveguide(**pp), range(1, 2 * threads + 1), threads)
+    per_real = (time.perf_counter() - t0) * nx / probe_nx / (2 * threads)
+    nreal = int(min(8 * threads, max(2 * threads, seconds / max(per_real, 1e-9))))
     ow = O.Waveguide(**params)
-    threads = O.omp_max_threads()
-    # one realization per thread (schedule(dynamic,1) as src/main.f90:153); size the sample so it
-    # takes about `seconds` of wall time given a one-realization probe on a reduced slice count
-    probe_nx = max(20, params["nxdiso"] // 50)
-    pp = dict(params, nxdiso=probe_nx, aratio=params["aratio"] * probe_nx / params["nxdiso"])
-    t0 = O.wtime()
-    O.Waveguide(**pp).tdistrib_batch(1, 1, nthreads=1)
-    per_real = (O.wtime() - t0) * params["nxdiso"] / probe_nx
-    nreal = max(threads, int(round(seconds / max(per_real, 1e-6))) * threads)
-    nreal = min(nreal, 4 * threads)
-    t0 = O.wtime()
-    ow.tdistrib_batch(1, nreal, nthreads=threads)
-    dt = O.wtime() - t0
+    t0 = time.perf_counter()
+    tdistrib_lapack(ow, range(1, nreal + 1), threads)
+    dt = time.perf_counter() - t0
+    # the C restatement: one realization per thread, a quarter of the sample
+    nc = max(threads, nreal // 4 // threads * threads)
+    t1 = O.wtime()
+    ow.tdistrib_batch(1, nc, nthreads=threads)
+    dtc = O.wtime() - t1
     return {"value": nreal / dt, "unit": "realizations/s", "cores": threads, "kind": "port",
-            "sample": f"{nreal} realizations of the same N={ow.nmu}, nx={params['nxdiso']}, ny={params['ny']} workload "
-                      f"(FP64 oracle, OpenMP over realizations, {dt:.1f} s wall)"}
+            "cpu_model": cpu_model(),
+            "sample": f"{nreal} realizations of the same N={ow.nmu}, nx={nx}, ny={params['ny']} workload: the "
+                      f"reference's slice loop on LAPACK/BLAS (MKL zgetrf+zgetri, zgemm, gesdd through torch's CPU "
+                      f"backend, {threads} threads), {dt:.1f} s wall",
+            "textbook_c_port": {"value": nc / dtc, "sample": f"{nc} realizations, oracle/rg_oracle.c, OpenMP over "
+                                                             f"realizations, {dtc:.1f} s wall"}}
 
 
 def timed_sweeps(plan, precision, B, steps, warmup, out, status):
@@ -112,8 +135,10 @@ def timed_sweeps(plan, precision, B, steps, warmup, out, status):
 
 
 def accuracy(plan, t_ref, t_mix):
-    """Errors of a mixed sweep against the FP64 sweep on identical seeds: mean T, max |dT|, and
-    the max bin deviation of rho(T) (50 linear bins and 256 Chebyshev bins on [0, 1])."""
+    """Errors of a mixed sweep against the FP64 sweep on identical seeds: mean T, max |dT|, relative errors
+    by decade of T, and the max bin deviation of rho(T) on 50 linear bins and on the Chebyshev bins of the
+    tdistrib task (50 and 256), with the largest count difference of a bin and what one count changes in
+    the narrowest bin (sum of counts times the bin width)."""
     from recurgreen_amd import distrib
 
     out = {"mean_T_fp64": float(t_ref.mean().item()), "mean_T_err": float((t_mix.mean() - t_ref.mean()).abs().item()),
@@ -121,13 +146,22 @@ def accuracy(plan, t_ref, t_mix):
            "max_abs_dT": float((t_mix - t_ref).abs().max().item())}
     out["fp64_T_above_1"] = int((t_ref > 1.0).sum().item())
     out["mixed_T_at_1"] = int((t_mix == 1.0).sum().item())
-    for bt, nb in (("linear", 50), ("chebyshev", 256)):
+    rel = (t_mix - t_ref).abs() / t_ref.clamp_min(1e-300)
+    for lo, hi in ((1e-3, 1.01), (1e-6, 1e-3), (1e-9, 1e-6)):
+        sel = (t_ref > lo) & (t_ref <= hi)
+        if int(sel.sum().item()):
+            out[f"rel_err_rms_T_{lo:g}_{hi:g}"] = float(rel[sel].pow(2).mean().sqrt().item())
+    for bt, nb in (("linear", 50), ("chebyshev", 50), ("chebyshev", 256)):
         e = distrib.bins(bt, nb)
-        rr = distrib.rho_from_counts(plan.histogram(t_ref, e)[0], e)[:, 1]
-        rm = distrib.rho_from_counts(plan.histogram(t_mix, e)[0], e)[:, 1]
+        cr = plan.histogram(t_ref, e)[0]
+        cm = plan.histogram(t_mix, e)[0]
+        rr = distrib.rho_from_counts(cr, e)[:, 1]
+        rm = distrib.rho_from_counts(cm, e)[:, 1]
         d = abs(rm - rr)
         out[f"max_rho_bin_err_{bt}{nb}"] = float(d.max())
         out[f"max_rho_bin_{bt}{nb}"] = int(d.argmax())
+        out[f"max_count_diff_{bt}{nb}"] = int(abs(cm.astype("int64") - cr.astype("int64")).max())
+        out[f"rho_of_one_count_narrowest_{bt}{nb}"] = float(1.0 / (cr.sum() * (e[1:] - e[:-1]).min()))
     return out
 
 
@@ -389,6 +423,12 @@ def main():
         "slice_updates_per_s": r["value"] * (nx - 1),
         "algorithmic_tflops": r["value"] * flops_real / 1e12,
         "fp64_sweep": runs["fp64"],
+        "fp64_roofline": {"bound": "mfma", "kernel": runs["fp64"]["dominant"], "achieved": runs["fp64"]["achieved_tflops"],
+                          "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                          "frac": runs["fp64"]["achieved_tflops"] / PEAK_TFLOPS["fp64"]},
+        # dense matrix-core peaks of the two precisions: the ceiling of the mixed/FP64 ratio for sweeps whose
+        # kernels run equally close to their rooflines
+        "mfma_peak_ratio_fp32_fp64": PEAK_TFLOPS["mixed_fp32"] / PEAK_TFLOPS["fp64"],
         "graph_sweep": graph_run,
         "speedup_vs_fp64": r["value"] / runs["fp64"]["value"],
         "roofline": {"bound": "mfma", "kernel": r["dominant"], "achieved": r["achieved_tflops"], "peak": peak,
