@@ -406,10 +406,12 @@ def main():
     graph_run = {"value": args.steps * B / gel, "ms_per_step": gel / args.steps * 1e3,
                  "note": "same steps replayed from a captured graph, no events recorded"}
     peak = PEAK_TFLOPS[main_prec]
-    traffic = None
+    traffic = traffic64 = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(f"{r['dominant']}:{main_prec}")
+        tj = json.load(open(tpath))  # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counters
+        traffic = tj.get(f"{args.config}:{r['dominant']}:{main_prec}")
+        traffic64 = tj.get(f"{args.config}:{runs['fp64']['dominant']}:fp64")
     line = {
         "metric": METRIC if args.config == "n128" else f"realizations/s at N={n} modes ({args.config})",
         "value": r["value"], "unit": "realizations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -425,7 +427,8 @@ def main():
         "fp64_sweep": runs["fp64"],
         "fp64_roofline": {"bound": "mfma", "kernel": runs["fp64"]["dominant"], "achieved": runs["fp64"]["achieved_tflops"],
                           "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
-                          "frac": runs["fp64"]["achieved_tflops"] / PEAK_TFLOPS["fp64"]},
+                          "frac": runs["fp64"]["achieved_tflops"] / PEAK_TFLOPS["fp64"],
+                          "traffic": traffic64, "avg_launch_ms": runs["fp64"]["avg_launch_ms"]},
         # dense matrix-core peaks of the two precisions: the ceiling of the mixed/FP64 ratio for sweeps whose
         # kernels run equally close to their rooflines
         "mfma_peak_ratio_fp32_fp64": PEAK_TFLOPS["mixed_fp32"] / PEAK_TFLOPS["fp64"],
