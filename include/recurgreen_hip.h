@@ -201,8 +201,11 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
 /* Launch options (no counterpart in the reference: the choices the autotuner makes, module
  * recurgreen_amd.autotune).  Keys and values:
  *   "inverse_fp64"  0 register-resident Gauss-Jordan, 1 look-ahead Gauss-Jordan, 2 blocked LU, 3 verified
- *                   diagonal pivots on the FP64 matrix cores (default; pivoted look-ahead for the rest);
- *   "inverse_mixed" 1 look-ahead, 2 blocked, 3 verified diagonal pivots (default);
+ *                   diagonal pivots on the FP64 matrix cores (pivoted look-ahead for the rest), 4 (default)
+ *                   the same on the lower triangle of the complex symmetric slice matrices of Dirichlet
+ *                   guides (block sweep operator; 3 for periodic guides);
+ *   "inverse_mixed" 1 look-ahead, 2 blocked, 3 verified diagonal pivots, 4 (default) the same on the
+ *                   lower triangle of complex symmetric slice matrices (Dirichlet; 3 for periodic guides);
  *   "dyson"         complex64 Dyson product: 0 64x64 output tiles, 1 one workgroup per realization
  *                   (default, NP 80..128), 2 the same on the BF16 matrix cores with every FP32
  *                   operand split into three BF16 values (six products, FP32 accuracy per

@@ -108,9 +108,29 @@ __device__ inline void rg_load_cq(double *s_cq, const rg_cqv &v, int b, int tid,
     }
 }
 
+// Tile t of the lower triangle of a matrix of wv x wv tiles, as (I, J) with I >= J: the diagonal tiles
+// first, then the off-diagonal ones by distance from the diagonal (the symmetric slice inversions).
+__device__ __forceinline__ void rg_sym_tile(int t, int wv, int &I, int &J)
+{
+    if (t < wv) {
+        I = J = t;
+        return;
+    }
+    t -= wv;
+    int d = 1;
+    while (t >= wv - d) {
+        t -= wv - d;
+        d++;
+    }
+    J = t;
+    I = t + d;
+}
+
 enum rg_eigen_method { RG_EIG_GRAM = 0, RG_EIG_JACOBI = 1, RG_EIG_BIDIAG = 2, RG_EIG_AUTO = 3 };
 
-enum rg_inverse_kernel { RG_INV_REGISTER = 0, RG_INV_LOOKAHEAD = 1, RG_INV_BLOCKED = 2, RG_INV_NOPIV = 3 };
+// RG_INV_SYM: verified diagonal pivots on the lower triangle (complex symmetric slice matrices,
+// Dirichlet guides); RG_INV_NOPIV elsewhere.
+enum rg_inverse_kernel { RG_INV_REGISTER = 0, RG_INV_LOOKAHEAD = 1, RG_INV_BLOCKED = 2, RG_INV_NOPIV = 3, RG_INV_SYM = 4 };
 
 struct rg_plan {
     rg_params p{};
@@ -167,7 +187,7 @@ struct rg_plan {
     bool profiling = false;
     int profile_mask = ~0;  // bit k: record launches of timer k (rg_kernel_id) while profiling
     // slice-inversion kernel per state precision (RG_INV_*): FP64 state, complex64 state
-    int inv_fp64 = 3, inv_mixed = 3;
+    int inv_fp64 = 4, inv_mixed = 4;
     // transmission eigenvalues (option eigen, rg_eigen_method): RG_EIG_GRAM eig(t^H t), RG_EIG_JACOBI one-sided
     // Jacobi SVD of t, RG_EIG_BIDIAG Householder bidiagonalisation of t (order <= 256), RG_EIG_AUTO
     // the bidiagonal SVD for complex128 sweeps and the Gram eigensolver for complex64 sweeps
@@ -287,6 +307,8 @@ void rg_launch_inverse_gj(rg_plan *P, int nb, int slot, int first, bool with_dis
 // diagonal pivots; realizations whose check fails are flagged in redo (inverted again by rg_launch_inverse_gj)
 void rg_launch_inverse_nopiv(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo);
 void rg_launch_inverse_nopiv64(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo);
+void rg_launch_inverse_sym64(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo);
+void rg_launch_inverse_sym32(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo);
 // rg_blocked.hip
 void rg_launch_inverse_blocked(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
 void rg_launch_gemm_sp(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new, int precision);

@@ -378,7 +378,255 @@ __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict
     }
 }
 
+// ---- Complex symmetric slice matrices (Dirichlet guides) -------------------------------------------
+//
+// The complex64 counterpart of k_inverse_sym64 (rg_nopiv64.hip): M = A_x - V_x - B Gss B is complex
+// symmetric on Dirichlet guides, and the block sweep operator on the lower triangle of tiles, with
+// pivot block K, D^-1 = S_KK^-1 and the panel Q_I = S_IK,
+//   S_IJ -= Q_I D^-1 Q_J^T (I >= J, I, J != K),  S_IK := R_I = Q_I D^-1,  S_KK := -D^-1,
+// leaves S = -M^-1; the pivot checks are those of k_inverse_nopiv, in FP64.  Wave w holds the tiles
+// w + NW s (rg_sym_tile) in slot s.  Phase A: wave I forms R_I into LDS, rows of block q > K are
+// checked; phase B: every wave updates or replaces its tiles and publishes those in block row/column
+// K + 1; the owner of tile (K+1, K+1) updates it first and inverts it.  With half the tiles of the
+// general kernel the state fits 128 registers and 76 KB of LDS, so two realizations share a CU.  The
+// diagonal tiles are written symmetrized, so every slice matrix is exactly symmetric.
+template <int WV>
+struct Sym32Shared {
+    static constexpr int NP = 16 * WV;
+    alignas(16) float qr[2][NP][LDW], qi[2][NP][LDW];  // panel Q_I = S_IK [row][k], by step parity
+    alignas(16) float rr[NP][LDW], ri[NP][LDW];        // R_I = Q_I D^-1 [row][k]
+    alignas(16) float dr[2][16][LDW], di[2][16][LDW];  // D^-1 row-major; symmetric, so also transposed
+    alignas(16) float wtr[2][16][LDW], wti[2][16][LDW];  // W = U_D^-1 diag(u), transposed
+    double ua[2][16];
+    int bad;
+    alignas(16) double cq[2 * 128 + 8];  // np <= 128: 2 n + 1 Dirichlet table entries
+    double2 bv[NP], ad[NP];
+};
+
+// The owner of diagonal block K inverts its tile in place and publishes D^-1, W and the pivot
+// magnitudes to buffer K & 1.
+template <int WV>
+__device__ __attribute__((always_inline)) inline void publish_sym32(np_f4 &xr, np_f4 &xi, int K, int lane,
+                                                                    Sym32Shared<WV> &sh)
+{
+    const int c = lane & 15, g = lane >> 4, buf = K & 1;
+    np_f4 lr, li;
+    const bool ok = block_gj(xr, xi, lr, li, sh.ua[buf], lane);
+    if (!ok && lane == 0) sh.bad = 1;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        sh.dr[buf][4 * g + v][c] = xr[v];
+        sh.di[buf][4 * g + v][c] = xi[v];
+    }
+    __builtin_amdgcn_wave_barrier();
+    np_f4 wr, wi;
+    tile_mul(wr, wi, sh.dr[buf], sh.di[buf], lr, li, lane);  // W = D^-1 (L_D diag(u))
+    *reinterpret_cast<np_f4 *>(&sh.wtr[buf][c][4 * g]) = wr;
+    *reinterpret_cast<np_f4 *>(&sh.wti[buf][c][4 * g]) = wi;
+}
+
+// Tile (I, J) into the panel of pivot block P (the tile lies in block row or column P, off the diagonal).
+template <int WV>
+__device__ __attribute__((always_inline)) inline void panel_store_sym32(np_f4 xr, np_f4 xi, int I, int J, int P,
+                                                                        int lane, Sym32Shared<WV> &sh)
+{
+    const int c = lane & 15, g = lane >> 4, buf = P & 1;
+    if (J == P) {  // Q_I = S_IP: the tile as it stands
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            sh.qr[buf][16 * I + 4 * g + v][c] = xr[v];
+            sh.qi[buf][16 * I + 4 * g + v][c] = xi[v];
+        }
+    } else {  // Q_J = S_PJ^T: lane (c, g) holds Q_J[c][4g + v]
+        *reinterpret_cast<np_f4 *>(&sh.qr[buf][16 * J + c][4 * g]) = xr;
+        *reinterpret_cast<np_f4 *>(&sh.qi[buf][16 * J + c][4 * g]) = xi;
+    }
+}
+
+template <int WV, int NW>
+__global__ __launch_bounds__(64 * NW, NW / 2) void k_inverse_sym32(float2 *__restrict__ gss, const rg_cqv cqv,
+                                                                   const double2 *__restrict__ adiag,
+                                                                   const double2 *__restrict__ bvec, int n, int first,
+                                                                   int32_t *__restrict__ redo, int cs,
+                                                                   const int32_t *__restrict__ nxr, int ix)
+{
+    adiag += (size_t)blockIdx.x * 3 * cs;
+    if (nxr && nxr[blockIdx.x] == ix) adiag += cs;
+    bvec += (size_t)blockIdx.x * cs;
+    constexpr int NP = 16 * WV, NT = 64 * NW, NTILE = WV * (WV + 1) / 2, NS = (NTILE + NW - 1) / NW;
+    static_assert(NW >= WV, "every diagonal tile is slot 0 of its own wave");
+    __shared__ Sym32Shared<WV> sh;
+    const int tid = threadIdx.x, lane = tid & 63, c = lane & 15, g = lane >> 4;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int b = blockIdx.x;
+    float2 *G = gss + (size_t)b * NP * NP;
+
+    if (cqv.cq) rg_load_cq(sh.cq, cqv, b, tid, NT);
+    for (int i = tid; i < NP; i += NT) {
+        sh.bv[i] = i < n ? bvec[i] : cmk(0.0, 0.0);
+        sh.ad[i] = i < n ? adiag[i] : cmk(0.0, 0.0);
+    }
+    if (tid == 0) sh.bad = 0;
+    __syncthreads();
+
+    int sI[NS], sJ[NS];
+    bool sv[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        sv[s] = w + NW * s < NTILE;
+        sI[s] = sJ[s] = 0;
+        if (sv[s]) rg_sym_tile(w + NW * s, WV, sI[s], sJ[s]);
+    }
+
+    // M = (diag(a) - V) - B Gss B in FP64 on the lower tiles, rounded; identity on the padding.
+    np_f4 ar[NS], ai[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        ar[s] = np_f4{0.f, 0.f, 0.f, 0.f};
+        ai[s] = np_f4{0.f, 0.f, 0.f, 0.f};
+        if (!sv[s]) continue;
+        const int col = 16 * sJ[s] + c;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = 16 * sI[s] + 4 * g + v;
+            double2 m;
+            if (row < n && col < n) {
+                m = row == col ? sh.ad[row] : cmk(0.0, 0.0);
+                if (cqv.cq) m.x -= sh.cq[abs(row - col)] - sh.cq[row + col + 2];
+                if (!first) {
+                    const float2 gv = G[(size_t)row * NP + col];
+                    m = csub(m, cmul(cmul(sh.bv[row], cmk((double)gv.x, (double)gv.y)), sh.bv[col]));
+                }
+            } else {
+                m = cmk(row == col ? 1.0 : 0.0, 0.0);
+            }
+            ar[s][v] = (float)m.x;
+            ai[s][v] = (float)m.y;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        if (sv[s] && sJ[s] == 0 && sI[s] > 0) panel_store_sym32(ar[s], ai[s], sI[s], 0, 0, lane, sh);
+    if (w == 0) publish_sym32(ar[0], ai[0], 0, lane, sh);
+    __syncthreads();
+
+    for (int K = 0; K < WV; K++) {
+        const int buf = K & 1;
+        // Phase A: R_w = Q_w D^-1; the rows of block q > K against the pivots of block K.
+        if (w < WV && w != K) {
+            np_f4 br, bi, xr, xi;
+            load_b(br, bi, sh.dr[buf], sh.di[buf], lane);
+            tile_mul(xr, xi, sh.qr[buf] + 16 * w, sh.qi[buf] + 16 * w, br, bi, lane);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                sh.rr[16 * w + 4 * g + v][c] = xr[v];
+                sh.ri[16 * w + 4 * g + v][c] = xi[v];
+            }
+        }
+        for (int q = K + 1; q < WV; q++) {
+            if ((WV + q) % NW != w) continue;
+            np_f4 br, bi, vr = np_f4{0.f, 0.f, 0.f, 0.f}, vi = np_f4{0.f, 0.f, 0.f, 0.f};
+            load_b(br, bi, sh.wtr[buf], sh.wti[buf], lane);
+            tile_fnms(vr, vi, sh.qr[buf], sh.qi[buf], 16 * q, br, bi, lane);  // V = -Q_q W
+            bool ok = true;
+            const double ua = sh.ua[buf][c];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if (fabs((double)vr[v]) + fabs((double)vi[v]) > NP_MARGIN * ua) ok = false;
+            if (!__all(ok) && lane == 0) sh.bad = 1;
+        }
+        __syncthreads();
+        // Phase B.
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            if (!sv[s]) continue;
+            const int I = sI[s], J = sJ[s];
+            if (I == K && J == K) {  // -D^-1 (this wave inverted the tile in place)
+                ar[s] = -ar[s];
+                ai[s] = -ai[s];
+            } else if (J == K) {  // R_I
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    ar[s][v] = sh.rr[16 * I + 4 * g + v][c];
+                    ai[s][v] = sh.ri[16 * I + 4 * g + v][c];
+                }
+            } else if (I == K) {  // R_J^T
+                load_b(ar[s], ai[s], sh.rr + 16 * J, sh.ri + 16 * J, lane);
+            } else {  // S_IJ -= Q_I R_J^T
+                np_f4 br, bi;
+                load_b(br, bi, sh.rr + 16 * J, sh.ri + 16 * J, lane);
+                tile_fnms(ar[s], ai[s], sh.qr[buf], sh.qi[buf], 16 * I, br, bi, lane);
+            }
+            if (K + 1 < WV) {
+                if (s == 0 && I == K + 1 && J == K + 1) publish_sym32(ar[0], ai[0], K + 1, lane, sh);
+                if ((I == K + 1) != (J == K + 1)) panel_store_sym32(ar[s], ai[s], I, J, K + 1, lane, sh);
+            }
+        }
+        __syncthreads();
+    }
+    if (sh.bad) {
+        if (tid == 0) redo[b] = 1;
+        return;
+    }
+    if (tid == 0) redo[b] = 0;
+    // Gss = -S, both triangles, each diagonal tile symmetrized (X + X^T) / 2 (the panel buffer is free).
+    if (w < WV) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            sh.qr[0][16 * w + 4 * g + v][c] = ar[0][v];
+            sh.qi[0][16 * w + 4 * g + v][c] = ai[0][v];
+        }
+    }
+    __syncthreads();
+    if (w < WV) {
+        const np_f4 tr = *reinterpret_cast<const np_f4 *>(&sh.qr[0][16 * w + c][4 * g]);
+        const np_f4 tq = *reinterpret_cast<const np_f4 *>(&sh.qi[0][16 * w + c][4 * g]);
+        ar[0] = 0.5f * (ar[0] + tr);
+        ai[0] = 0.5f * (ai[0] + tq);
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        if (!sv[s]) continue;
+        const int I = sI[s], J = sJ[s];
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            G[(size_t)(16 * I + 4 * g + v) * NP + 16 * J + c] = make_float2(-ar[s][v], -ai[s][v]);
+        if (I != J) {  // the transpose: row 16 J + c, columns 16 I + 4 g .. + 3 (32 contiguous bytes)
+            float4 *dst = reinterpret_cast<float4 *>(G + (size_t)(16 * J + c) * NP + 16 * I + 4 * g);
+            dst[0] = make_float4(-ar[s][0], -ai[s][0], -ar[s][1], -ai[s][1]);
+            dst[1] = make_float4(-ar[s][2], -ai[s][2], -ar[s][3], -ai[s][3]);
+        }
+    }
+}
+
 }  // namespace
+
+// Complex symmetric slice matrices (Dirichlet guides), np >= 32, complex64 state: 8 waves per
+// realization, two realizations per CU.
+void rg_launch_inverse_sym32(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo)
+{
+    hipEvent_t stop = nullptr;
+    rg_timer_begin(P, RGK_INVERSE, &stop);
+    const rg_cqv cq = rg_cq_view(P, nb, with_disorder);
+    const double2 *ad = P->d_adiag + (size_t)slot * P->np;
+    switch (P->np / 16) {
+#define RG_SYM(W, NW)                                                                                          \
+    case W:                                                                                                    \
+        hipLaunchKernelGGL((k_inverse_sym32<W, NW>), dim3(nb), dim3(64 * NW), 0, P->stream, P->d_gss32, cq, ad, \
+                           P->d_b, P->nmu, first, redo, P->cstride, P->d_nx_r, P->cur_ix);                     \
+        break;
+        RG_SYM(2, 2)
+        RG_SYM(3, 3)
+        RG_SYM(4, 4)
+        RG_SYM(5, 5)
+        RG_SYM(6, 6)
+        RG_SYM(7, 7)
+        RG_SYM(8, 8)
+#undef RG_SYM
+        default: break;
+    }
+    rg_timer_end(P, RGK_INVERSE, stop);
+}
 
 // np >= 32 (with a single 16-column block the pivoted kernel is used directly).
 void rg_launch_inverse_nopiv(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo)

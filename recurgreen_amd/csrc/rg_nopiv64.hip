@@ -373,7 +373,264 @@ __global__ __launch_bounds__(64 * WV, 2) void k_inverse_nopiv64(double2 *__restr
         for (int v = 0; v < 4; v++) G[(size_t)(16 * ti + 4 * v + g) * NP + col] = make_double2(ar[ti][v], ai[ti][v]);
 }
 
+// ---- Complex symmetric slice matrices (Dirichlet guides) -------------------------------------------
+//
+// With Dirichlet modes V_x is real symmetric and A_x, B are diagonal, so M = A_x - V_x - B Gss B is
+// complex symmetric whenever Gss is, and Gss = M^-1 is: every slice matrix of the sweep is.  The
+// elimination then runs as the block sweep operator on the lower triangle of tiles only: for pivot
+// block K with D^-1 = S_KK^-1 and the panel Q_I = S_IK,
+//   S_IJ -= Q_I D^-1 Q_J^T (I >= J, I, J != K),  S_IK := R_I = Q_I D^-1,  S_KK := -D^-1,
+// which after every block leaves S = -M^-1.  These are the diagonal-pivot Gauss-Jordan steps of
+// k_inverse_nopiv64 with the signs of the eliminated column flipped, on WV (WV + 1) / 2 tiles instead
+// of WV^2; the pivot checks are the same.  Wave w holds the tiles t = w + NW s (diagonal tiles first,
+// then the off-diagonal ones by distance from the diagonal) in register slot s.  Block step K:
+//   phase A: wave I (I != K) forms R_I into LDS; row blocks q > K are checked against V = -Q_q W;
+//   phase B: every wave updates or replaces its tiles, publishes those in block row/column K + 1 as the
+//   next panel; the owner of tile (K+1, K+1) updates it first and inverts it (look-ahead).
+// Two workgroup barriers per block step.  Only the lower triangle of the previous Gss is read; both
+// triangles of the new one are written.
+
+template <int WV>
+struct Sym64Shared {
+    static constexpr int NP = 16 * WV;
+    alignas(32) double qr[2][NP][LDD], qi[2][NP][LDD];     // panel Q_I = S_IK [row][kp(col)], by step parity
+    alignas(32) double rr[NP][LDD], ri[NP][LDD];           // R_I = Q_I D^-1 [row][kp(col)]
+    alignas(32) double dr[2][16][LDD], di[2][16][LDD];     // D^-1 [row][kp(col)]; symmetric, so also [col][kp(row)]
+    alignas(32) double wtr[2][16][LDD], wti[2][16][LDD];   // W = U_D^-1 diag(u), transposed
+    double ua[2][16];
+    int bad;
+    alignas(16) double cq[2 * 256 + 8];
+    double2 bv[NP], ad[NP];
+};
+
+// The owner of diagonal block K inverts its tile (xr, xi) in place and publishes D^-1, W and the pivot
+// magnitudes to buffer K & 1.
+template <int WV>
+__device__ __forceinline__ void publish_sym64(d4 &xr, d4 &xi, int K, int lane, Sym64Shared<WV> &sh)
+{
+    const int c = lane & 15, g = lane >> 4, buf = K & 1;
+    d4 lr, li;
+    const bool ok = block_gj64(xr, xi, lr, li, sh.ua[buf], lane);
+    if (!ok && lane == 0) sh.bad = 1;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        sh.dr[buf][4 * v + g][kp(c)] = xr[v];
+        sh.di[buf][4 * v + g][kp(c)] = xi[v];
+    }
+    __builtin_amdgcn_wave_barrier();
+    d4 wr, wi;
+    tile_mul64(wr, wi, sh.dr[buf], sh.di[buf], lr, li, lane);  // W = D^-1 (L_D diag(u))
+    *reinterpret_cast<d4 *>(&sh.wtr[buf][c][4 * g]) = wr;
+    *reinterpret_cast<d4 *>(&sh.wti[buf][c][4 * g]) = wi;
+}
+
+// Tile (I, J) into the panel of pivot block P (the tile lies in block row or column P, off the diagonal).
+template <int WV>
+__device__ __forceinline__ void panel_store_sym64(d4 xr, d4 xi, int I, int J, int P, int lane, Sym64Shared<WV> &sh)
+{
+    const int c = lane & 15, g = lane >> 4, buf = P & 1;
+    if (J == P) {  // Q_I = S_IP: the tile as it stands
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            sh.qr[buf][16 * I + 4 * v + g][kp(c)] = xr[v];
+            sh.qi[buf][16 * I + 4 * v + g][kp(c)] = xi[v];
+        }
+    } else {  // Q_J = S_JP = S_PJ^T: lane (c, g) holds Q_J[c][4v + g], stored at [c][kp(4v + g)] = [c][4g + v]
+        *reinterpret_cast<d4 *>(&sh.qr[buf][16 * J + c][4 * g]) = xr;
+        *reinterpret_cast<d4 *>(&sh.qi[buf][16 * J + c][4 * g]) = xi;
+    }
+}
+
+template <int WV, int NW>
+__global__ __launch_bounds__(64 * NW, (NW > 8 ? NW / 4 : 2)) void k_inverse_sym64(
+    double2 *__restrict__ gss, const rg_cqv cqv, const double2 *__restrict__ adiag, const double2 *__restrict__ bvec,
+    int n, int first, int32_t *__restrict__ redo, int cs, const int32_t *__restrict__ nxr, int ix)
+{
+    adiag += (size_t)blockIdx.x * 3 * cs;
+    if (nxr && nxr[blockIdx.x] == ix) adiag += cs;
+    bvec += (size_t)blockIdx.x * cs;
+    constexpr int NP = 16 * WV, NT = 64 * NW, NTILE = WV * (WV + 1) / 2, NS = (NTILE + NW - 1) / NW;
+    static_assert(NW >= WV, "every diagonal tile is slot 0 of its own wave");
+    __shared__ Sym64Shared<WV> sh;
+    const int tid = threadIdx.x, lane = tid & 63, c = lane & 15, g = lane >> 4;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int b = blockIdx.x;
+    double2 *G = gss + (size_t)b * NP * NP;
+
+    if (cqv.cq) rg_load_cq(sh.cq, cqv, b, tid, NT);
+    for (int i = tid; i < NP; i += NT) {
+        sh.bv[i] = i < n ? bvec[i] : cmk(0.0, 0.0);
+        sh.ad[i] = i < n ? adiag[i] : cmk(0.0, 0.0);
+    }
+    if (tid == 0) sh.bad = 0;
+    __syncthreads();
+
+    int sI[NS], sJ[NS];
+    bool sv[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        sv[s] = w + NW * s < NTILE;
+        sI[s] = sJ[s] = 0;
+        if (sv[s]) rg_sym_tile(w + NW * s, WV, sI[s], sJ[s]);
+    }
+
+    // M = (diag(a) - V) - B Gss B on the lower tiles; identity on the padding.
+    d4 ar[NS], ai[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        ar[s] = d4{0.0, 0.0, 0.0, 0.0};
+        ai[s] = d4{0.0, 0.0, 0.0, 0.0};
+        if (!sv[s]) continue;
+        const int col = 16 * sJ[s] + c;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = 16 * sI[s] + 4 * v + g;
+            double2 m;
+            if (row < n && col < n) {
+                m = row == col ? sh.ad[row] : cmk(0.0, 0.0);
+                if (cqv.cq) m.x -= sh.cq[abs(row - col)] - sh.cq[row + col + 2];
+                if (!first) m = csub(m, cmul(cmul(sh.bv[row], G[(size_t)row * NP + col]), sh.bv[col]));
+            } else {
+                m = cmk(row == col ? 1.0 : 0.0, 0.0);
+            }
+            ar[s][v] = m.x;
+            ai[s][v] = m.y;
+        }
+    }
+    // Panel of block 0 and its inverse.
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        if (sv[s] && sJ[s] == 0 && sI[s] > 0) panel_store_sym64(ar[s], ai[s], sI[s], 0, 0, lane, sh);
+    if (w == 0) publish_sym64(ar[0], ai[0], 0, lane, sh);
+    __syncthreads();
+
+    for (int K = 0; K < WV; K++) {
+        const int buf = K & 1;
+        // Phase A: R_w = Q_w D^-1; the rows of block q > K against the pivots of block K.
+        if (w < WV && w != K) {
+            d4 br, bi, rr, ri;
+            load_b64(br, bi, sh.dr[buf], sh.di[buf], lane);
+            tile_mul64(rr, ri, sh.qr[buf] + 16 * w, sh.qi[buf] + 16 * w, br, bi, lane);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                sh.rr[16 * w + 4 * v + g][kp(c)] = rr[v];
+                sh.ri[16 * w + 4 * v + g][kp(c)] = ri[v];
+            }
+        }
+        for (int q = K + 1; q < WV; q++) {
+            if ((WV + q) % NW != w) continue;
+            d4 br, bi, vr = d4{0.0, 0.0, 0.0, 0.0}, vi = d4{0.0, 0.0, 0.0, 0.0};
+            load_b64(br, bi, sh.wtr[buf], sh.wti[buf], lane);
+            tile_fnms64(vr, vi, sh.qr[buf], sh.qi[buf], 16 * q, br, bi, lane);  // V = -Q_q W
+            bool ok = true;
+            const double ua = sh.ua[buf][c];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if (fabs(vr[v]) + fabs(vi[v]) > N64_MARGIN * ua) ok = false;
+            if (!__all(ok) && lane == 0) sh.bad = 1;
+        }
+        __syncthreads();
+        // Phase B.
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            if (!sv[s]) continue;
+            const int I = sI[s], J = sJ[s];
+            if (I == K && J == K) {  // -D^-1 (this wave inverted the tile in place)
+                ar[s] = -ar[s];
+                ai[s] = -ai[s];
+            } else if (J == K) {  // R_I
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    ar[s][v] = sh.rr[16 * I + 4 * v + g][kp(c)];
+                    ai[s][v] = sh.ri[16 * I + 4 * v + g][kp(c)];
+                }
+            } else if (I == K) {  // R_J^T
+                load_b64(ar[s], ai[s], sh.rr + 16 * J, sh.ri + 16 * J, lane);
+            } else {  // S_IJ -= Q_I R_J^T
+                d4 br, bi;
+                load_b64(br, bi, sh.rr + 16 * J, sh.ri + 16 * J, lane);
+                tile_fnms64(ar[s], ai[s], sh.qr[buf], sh.qi[buf], 16 * I, br, bi, lane);
+            }
+            if (K + 1 < WV) {
+                if (s == 0 && I == K + 1 && J == K + 1) publish_sym64(ar[0], ai[0], K + 1, lane, sh);
+                if ((I == K + 1) != (J == K + 1)) panel_store_sym64(ar[s], ai[s], I, J, K + 1, lane, sh);
+            }
+        }
+        __syncthreads();
+    }
+    if (sh.bad) {
+        if (tid == 0) redo[b] = 1;
+        return;
+    }
+    if (tid == 0) redo[b] = 0;
+    // Gss = -S, both triangles, with each diagonal tile symmetrized, (X + X^T) / 2: the next slice reads
+    // the diagonal tiles whole, and the rounding asymmetry of a diagonal tile is not damped by the
+    // recursion (on some guides it doubles from slice to slice); written exactly symmetric, every slice
+    // matrix is.  The panel buffer is free by now.
+    if (w < WV) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            sh.qr[0][16 * w + 4 * v + g][c] = ar[0][v];
+            sh.qi[0][16 * w + 4 * v + g][c] = ai[0][v];
+        }
+    }
+    __syncthreads();
+    if (w < WV) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            ar[0][v] = 0.5 * (ar[0][v] + sh.qr[0][16 * w + c][4 * v + g]);
+            ai[0][v] = 0.5 * (ai[0][v] + sh.qi[0][16 * w + c][4 * v + g]);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        if (!sv[s]) continue;
+        const int I = sI[s], J = sJ[s];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const double2 x = make_double2(-ar[s][v], -ai[s][v]);
+            G[(size_t)(16 * I + 4 * v + g) * NP + 16 * J + c] = x;
+            if (I != J) G[(size_t)(16 * J + c) * NP + 16 * I + 4 * v + g] = x;
+        }
+    }
+}
+
 }  // namespace
+
+// Complex symmetric slice matrices (Dirichlet guides), np >= 32.  At np = 128 the workgroup has 12
+// waves (3 tiles each; measured 1.61 ms per 4096 realizations against 1.67 with 8 waves and 1.78 with
+// 16); RG_SYM64_WAVES = 8 or 16 selects the others, for measurement.
+void rg_launch_inverse_sym64(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo)
+{
+    static const int waves = [] {
+        const char *e = std::getenv("RG_SYM64_WAVES");
+        const int v = e ? std::atoi(e) : 12;
+        return v == 8 || v == 16 ? v : 12;
+    }();
+    hipEvent_t stop = nullptr;
+    rg_timer_begin(P, RGK_INVERSE, &stop);
+    const rg_cqv cq = rg_cq_view(P, nb, with_disorder);
+    const double2 *ad = P->d_adiag + (size_t)slot * P->np;
+#define RG_SYM(W, NW)                                                                                             \
+    hipLaunchKernelGGL((k_inverse_sym64<W, NW>), dim3(nb), dim3(64 * NW), 0, P->stream, P->d_gss, cq, ad, P->d_b,    \
+                       P->nmu, first, redo, P->cstride, P->d_nx_r, P->cur_ix)
+    switch (P->np / 16) {
+        case 2: RG_SYM(2, 2); break;
+        case 3: RG_SYM(3, 3); break;
+        case 4: RG_SYM(4, 4); break;
+        case 5: RG_SYM(5, 5); break;
+        case 6: RG_SYM(6, 6); break;
+        case 7: RG_SYM(7, 7); break;
+        case 8:
+            if (waves == 16) RG_SYM(8, 16);
+            else if (waves == 12) RG_SYM(8, 12);
+            else RG_SYM(8, 8);
+            break;
+        default: break;
+    }
+#undef RG_SYM
+    rg_timer_end(P, RGK_INVERSE, stop);
+}
 
 // np >= 32 (with a single 16-column block the pivoted kernel is used directly).
 void rg_launch_inverse_nopiv64(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo)

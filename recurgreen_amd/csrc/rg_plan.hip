@@ -245,6 +245,15 @@ void sweep_slice(rg_plan *P, int nb, int ix, int64_t seed0, const double *d_upot
     sweep_step(P, nb, ix, ix == 1 ? 0 : (ix == P->nx ? 2 : 1), dis ? id - 1 : -1, seed0, d_upot, precision, cur, true);
 }
 
+// The slice-inversion kernel of a sweep in `precision`: the plan's choice, where it applies.
+int inverse_kernel(const rg_plan *P, int precision)
+{
+    int inv = precision == RG_FP64 ? P->inv_fp64 : P->inv_mixed;
+    if (inv == RG_INV_SYM && P->p.boundtype != RG_DIRICHLET) inv = RG_INV_NOPIV;  // periodic: V is Hermitian
+    if ((inv == RG_INV_NOPIV || inv == RG_INV_SYM) && P->np < 32) inv = RG_INV_LOOKAHEAD;
+    return inv;
+}
+
 // One slice step with the slice's diagonal (slot 0 first, 1 bulk, 2 last) and disordered-slice index
 // ixd (-1: no disorder) given; flip = false leaves *cur for the caller to flip (several ranges of
 // one batch stepping together).
@@ -254,14 +263,18 @@ void sweep_step(rg_plan *P, int nb, int ix, int slot, int ixd, int64_t seed0, co
     const size_t mat = (size_t)nb * P->np * P->np;
     const bool dis = ixd >= 0;
     if (dis) rg_launch_project(P, nb, ixd, seed0, d_upot, nb);
-    int inv = precision == RG_FP64 ? P->inv_fp64 : P->inv_mixed;
-    if (inv == RG_INV_NOPIV && P->np < 32) inv = RG_INV_LOOKAHEAD;
+    int inv = inverse_kernel(P, precision);
     if (P->np > 128) rg_launch_inverse_schur(P, nb, slot, ix == 1, dis, precision);
     else if (inv == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
     else if (inv == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision);
-    else if (inv == RG_INV_NOPIV) {
-        if (precision == RG_FP64) rg_launch_inverse_nopiv64(P, nb, slot, ix == 1, dis, P->d_redo);
-        else rg_launch_inverse_nopiv(P, nb, slot, ix == 1, dis, P->d_redo);
+    else if (inv == RG_INV_NOPIV || inv == RG_INV_SYM) {
+        if (precision == RG_FP64) {
+            if (inv == RG_INV_SYM) rg_launch_inverse_sym64(P, nb, slot, ix == 1, dis, P->d_redo);
+            else rg_launch_inverse_nopiv64(P, nb, slot, ix == 1, dis, P->d_redo);
+        } else {
+            if (inv == RG_INV_SYM) rg_launch_inverse_sym32(P, nb, slot, ix == 1, dis, P->d_redo);
+            else rg_launch_inverse_nopiv(P, nb, slot, ix == 1, dis, P->d_redo);
+        }
         rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision, P->d_redo);
     } else rg_launch_inverse_blocked(P, nb, slot, ix == 1, dis, precision);
     hipEvent_t stop = nullptr;
@@ -444,13 +457,16 @@ void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int prec
 void invert_f64(rg_plan *P, int nb, int slot, int first, bool dis)
 {
     if (P->np > 128) rg_launch_inverse_schur(P, nb, slot, first, dis, RG_FP64);
-    else if (P->inv_fp64 == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, first, dis);
-    else if (P->inv_fp64 == RG_INV_LOOKAHEAD || (P->inv_fp64 == RG_INV_NOPIV && P->np < 32))
-        rg_launch_inverse_gj(P, nb, slot, first, dis, RG_FP64);
-    else if (P->inv_fp64 == RG_INV_NOPIV) {
-        rg_launch_inverse_nopiv64(P, nb, slot, first, dis, P->d_redo);
-        rg_launch_inverse_gj(P, nb, slot, first, dis, RG_FP64, P->d_redo);
-    } else rg_launch_inverse_blocked(P, nb, slot, first, dis, RG_FP64);
+    else {
+        const int inv = inverse_kernel(P, RG_FP64);
+        if (inv == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, first, dis);
+        else if (inv == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, first, dis, RG_FP64);
+        else if (inv == RG_INV_NOPIV || inv == RG_INV_SYM) {
+            if (inv == RG_INV_SYM) rg_launch_inverse_sym64(P, nb, slot, first, dis, P->d_redo);
+            else rg_launch_inverse_nopiv64(P, nb, slot, first, dis, P->d_redo);
+            rg_launch_inverse_gj(P, nb, slot, first, dis, RG_FP64, P->d_redo);
+        } else rg_launch_inverse_blocked(P, nb, slot, first, dis, RG_FP64);
+    }
 }
 
 // The reflection/transmission sweep of compute_refl_trans_v1 (src/waveguide.f90:469-519), FP64:
@@ -659,7 +675,8 @@ void fwd_step_f32(rg_plan *P, int nb, int ix, const double *d_upot, const float2
     if (P->np > 128) {
         rg_launch_inverse_schur(P, nb, slot, ix == 1, dis, RG_MIXED_FP32);
     } else if (P->np >= 32) {
-        rg_launch_inverse_nopiv(P, nb, slot, ix == 1, dis, P->d_redo);
+        if (inverse_kernel(P, RG_MIXED_FP32) == RG_INV_SYM) rg_launch_inverse_sym32(P, nb, slot, ix == 1, dis, P->d_redo);
+        else rg_launch_inverse_nopiv(P, nb, slot, ix == 1, dis, P->d_redo);
         rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, RG_MIXED_FP32, P->d_redo);
     } else {
         rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, RG_MIXED_FP32);
@@ -812,6 +829,7 @@ int inverse_choice(const char *v, int dflt, bool fp64)
     if (std::strcmp(v, "lookahead") == 0) return RG_INV_LOOKAHEAD;
     if (std::strcmp(v, "blocked") == 0) return RG_INV_BLOCKED;
     if (std::strcmp(v, "verified") == 0) return RG_INV_NOPIV;
+    if (std::strcmp(v, "symmetric") == 0) return RG_INV_SYM;
     return dflt;
 }
 
@@ -1075,7 +1093,7 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
     P->batch_max = batch_max;
     for (int i = 0; i < RGK_COUNT; i++) P->timers[i].name = kTimerNames[i];
     // Slice-inversion kernels, overridable for measurement: RG_INVERSE_FP64 and RG_INVERSE_MIXED in
-    // {lookahead, blocked, verified}, RG_INVERSE_FP64 also register.
+    // {lookahead, blocked, verified}, RG_INVERSE_FP64 also register and symmetric.
     P->inv_fp64 = inverse_choice(std::getenv("RG_INVERSE_FP64"), P->inv_fp64, true);
     P->inv_mixed = inverse_choice(std::getenv("RG_INVERSE_MIXED"), P->inv_mixed, false);
     // Transmission eigenvalues: RG_EIGEN in {gram, jacobi, bidiag} overrides the default (bidiagonal SVD
@@ -1663,11 +1681,10 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
     DeviceGuard g(P->device);
     (void)hipStreamSynchronize(P->stream);
     if (std::strcmp(key, "inverse_fp64") == 0) {
-        if (value != RG_INV_REGISTER && value != RG_INV_LOOKAHEAD && value != RG_INV_BLOCKED && value != RG_INV_NOPIV)
-            return RG_EINVAL;
+        if (value < RG_INV_REGISTER || value > RG_INV_SYM) return RG_EINVAL;
         P->inv_fp64 = value;
     } else if (std::strcmp(key, "inverse_mixed") == 0) {
-        if (value != RG_INV_LOOKAHEAD && value != RG_INV_BLOCKED && value != RG_INV_NOPIV) return RG_EINVAL;
+        if (value < RG_INV_LOOKAHEAD || value > RG_INV_SYM) return RG_EINVAL;
         P->inv_mixed = value;
     } else if (std::strcmp(key, "dyson") == 0) {
         if (value < 0 || value > 2) return RG_EINVAL;

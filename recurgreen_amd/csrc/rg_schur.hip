@@ -235,7 +235,7 @@ void invert_block(const rg_plan *P, int nb, int no, E *mats, int precision)
     V.d_b = P->d_schur_i;
     if (precision == RG_FP64) {
         V.d_gss = reinterpret_cast<double2 *>(mats);
-        if (P->inv_fp64 == RG_INV_NOPIV && no >= 32) {
+        if ((P->inv_fp64 == RG_INV_NOPIV || P->inv_fp64 == RG_INV_SYM) && no >= 32) {
             rg_launch_inverse_nopiv64(&V, nb, 0, 0, false, V.d_redo);
             rg_launch_inverse_gj(&V, nb, 0, 0, false, precision, V.d_redo);
         } else {

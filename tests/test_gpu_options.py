@@ -20,19 +20,19 @@ pytestmark = pytest.mark.gpu
 def test_option_round_trip_and_rejection():
     say("options round trip")
     plan = binding.Plan(CASES["c1_N20_ny205"], batch_max=2)
-    for key, values in (("inverse_fp64", (0, 1, 2, 3)), ("inverse_mixed", (1, 2, 3)), ("dyson", (0, 1, 2)),
+    for key, values in (("inverse_fp64", (0, 1, 2, 3, 4)), ("inverse_mixed", (1, 2, 3, 4)), ("dyson", (0, 1, 2)),
                         ("project_parts", (1, 3, 8)), ("lanes", (2, 1)), ("eigen", (1, 0, 2, 3))):
         for v in values:
             plan.set_option(key, v)
             assert plan.get_option(key) == v
-    for key, v in (("inverse_fp64", 4), ("inverse_mixed", 0), ("dyson", 3), ("project_parts", 9), ("lanes", 3),
+    for key, v in (("inverse_fp64", 5), ("inverse_mixed", 0), ("dyson", 3), ("project_parts", 9), ("lanes", 3),
                    ("eigen", 4), ("nonsense", 0)):
         with pytest.raises(binding.RecurGreenError):
             plan.set_option(key, v)
     plan.close()
 
 
-@pytest.mark.parametrize("name", ["N37_absorb", "N100_filtered", "N128"])
+@pytest.mark.parametrize("name", ["N37_absorb", "N100_filtered", "N128", "periodic_N41_ny60"])
 def test_every_option_value_matches_oracle(name):
     p = CASES[name]
     say(f"option values against the oracle {name}")
@@ -40,7 +40,7 @@ def test_every_option_value_matches_oracle(name):
     U = upot_batch(ow, [4, 5])
     To = np.stack([ow.tdistrib_potential(U[:, :, k].T) for k in range(2)])
     plan = binding.Plan(p, batch_max=2)
-    for key, values in (("inverse_fp64", (0, 1, 2, 3)), ("dyson_fp64", (1, 0)), ("project_parts", (1, 2, 4, 8)),
+    for key, values in (("inverse_fp64", (0, 1, 2, 3, 4)), ("dyson_fp64", (1, 0)), ("project_parts", (1, 2, 4, 8)),
                         ("eigen", (1, 0, 2, 3))):
         for v in values:
             plan.set_option(key, v)
@@ -49,7 +49,7 @@ def test_every_option_value_matches_oracle(name):
             gram = key == "eigen" and v == 0  # the Gram eigensolver: ~ n eps T_max absolute
             for k in range(2):
                 assert_T_close(Tg.reshape(2, -1)[k], To[k], floor=GRAM_FLOOR if gram else 0.0)
-    for key, values in (("inverse_mixed", (1, 2, 3)), ("dyson", (0, 1, 2)), ("eigen", (2, 1, 0, 3))):
+    for key, values in (("inverse_mixed", (1, 2, 3, 4)), ("dyson", (0, 1, 2)), ("eigen", (2, 1, 0, 3))):
         for v in values:
             plan.set_option(key, v)
             Tg, st, code = plan.tdistrib_potential(U, "mixed_fp32")

@@ -80,6 +80,50 @@ def test_explicit_potential_parity(name):
     plan.close()
 
 
+def zgetrf_rejects_diagonal(M, margin=1e-4):
+    """True if partial pivoting on M (max |Re| + |Im| in the column, src/base_utils.f90:235-274) picks an
+    off-diagonal row at some step, with the verified kernels' relative margin."""
+    A = np.array(M, dtype=complex)
+    for j in range(len(A)):
+        col = np.abs(A[j:, j].real) + np.abs(A[j:, j].imag)
+        if col.size > 1 and col[1:].max() > (1.0 - margin) * col[0]:
+            return True
+        A[j + 1:, j:] -= np.outer(A[j + 1:, j] / A[j, j], A[j, j:])
+    return False
+
+
+@pytest.mark.parametrize("name", ["N64", "N128"])
+def test_pivot_check_fallback(name):
+    """A potential spike of 1e5 standard deviations in one cell makes zgetrf pivot off the diagonal: the
+    verified kernels must flag those realizations and the pivoted kernel invert them again from the
+    unchanged Gss, in the first slice (realization 0) and in slice 20 (realization 1, with B Gss B)."""
+    p = CASES[name]
+    say(f"pivot-check fallback {name}")
+    ow = O.Waveguide(**p)
+    U = upot_batch(ow, [4, 5, 6])
+    amp = 1e5 * U.std()
+    U[0, 10, 0] += amp
+    U[19, 10, 1] += amp
+    assert zgetrf_rejects_diagonal(ow.slice_matrix(1, U[:, :, 0].T))
+    assert zgetrf_rejects_diagonal(ow.slice_matrix(20, U[:, :, 1].T))
+    To = np.stack([ow.tdistrib_potential(U[:, :, k].T) for k in range(3)])
+    plan = binding.Plan(p, batch_max=3)
+    for v in (3, 4):
+        plan.set_option("inverse_fp64", v)
+        Tg, st, code = plan.tdistrib_potential(U, "fp64")
+        assert code == 0 and not st.any(), v
+        for k in range(3):
+            assert_T_close(Tg.reshape(3, -1)[k], To[k])
+    for v in (3, 4):
+        plan.set_option("inverse_mixed", v)
+        Tg, st, code = plan.tdistrib_potential(U, "mixed_fp32")
+        assert code == 0 and not st.any(), v
+        err = np.max(np.abs(Tg.reshape(3, -1) - To))
+        say(f"{name} mixed (inverse_mixed={v}) with the fallback: max |dT| = {err:.3e}")
+        assert err <= MIXED_TOL["mixed_fp32"]
+    plan.close()
+
+
 # Long guides whose transmission eigenvalues span 19 to 20 decades (localized, L / l_s = 25, or strongly
 # absorbing), and 7 decades through the filtered apertures (t with fewer columns, or fewer rows, than modes).
 CLOSED = {
