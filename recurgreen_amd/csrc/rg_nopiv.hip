@@ -154,8 +154,9 @@ __device__ __attribute__((always_inline)) inline bool block_gj(np_f4 &xr, np_f4 
 
 // C (tile) -= A B for one 16x16 complex tile: A rows from LDS ([row][k], k = block column, four
 // consecutive k per lane in one read), B in accumulator layout (k-step v = rows 4g + v).
-__device__ __attribute__((always_inline)) inline void tile_fnms(np_f4 &cr, np_f4 &ci, const float (*asr)[LDW],
-                                                                const float (*asi)[LDW], int r0, np_f4 br, np_f4 bi,
+template <int S>
+__device__ __attribute__((always_inline)) inline void tile_fnms(np_f4 &cr, np_f4 &ci, const float (*asr)[S],
+                                                                const float (*asi)[S], int r0, np_f4 br, np_f4 bi,
                                                                 int lane)
 {
     const int i = lane & 15, g = lane >> 4;
@@ -171,8 +172,9 @@ __device__ __attribute__((always_inline)) inline void tile_fnms(np_f4 &cr, np_f4
 }
 
 // C = A B for A from LDS [16][LDW] (rows i, k = 4g+v) and B in accumulator layout.
-__device__ __attribute__((always_inline)) inline void tile_mul(np_f4 &cr, np_f4 &ci, const float (*asr)[LDW],
-                                                               const float (*asi)[LDW], np_f4 br, np_f4 bi, int lane)
+template <int S>
+__device__ __attribute__((always_inline)) inline void tile_mul(np_f4 &cr, np_f4 &ci, const float (*asr)[S],
+                                                               const float (*asi)[S], np_f4 br, np_f4 bi, int lane)
 {
     const int i = lane & 15, g = lane >> 4;
     const np_f4 a_r = *reinterpret_cast<const np_f4 *>(&asr[i][4 * g]);
@@ -190,7 +192,8 @@ __device__ __attribute__((always_inline)) inline void tile_mul(np_f4 &cr, np_f4 
 
 // B operand (accumulator layout) from a transposed LDS 16x16 matrix St[col][row]: reg v of lane l
 // = B[4g+v][l%16] = St[l%16][4g+v], one read.
-__device__ inline void load_b(np_f4 &br, np_f4 &bi, const float (*tr)[LDW], const float (*ti)[LDW], int lane)
+template <int S>
+__device__ inline void load_b(np_f4 &br, np_f4 &bi, const float (*tr)[S], const float (*ti)[S], int lane)
 {
     const int c = lane & 15, g = lane >> 4;
     br = *reinterpret_cast<const np_f4 *>(&tr[c][4 * g]);
@@ -385,89 +388,126 @@ __global__ __launch_bounds__(64 * WV, 4) void k_inverse_nopiv(float2 *__restrict
 // pivot block K, D^-1 = S_KK^-1 and the panel Q_I = S_IK,
 //   S_IJ -= Q_I D^-1 Q_J^T (I >= J, I, J != K),  S_IK := R_I = Q_I D^-1,  S_KK := -D^-1,
 // leaves S = -M^-1; the pivot checks are those of k_inverse_nopiv, in FP64.  Wave w holds the tiles
-// w + NW s (rg_sym_tile) in slot s.  Phase A: wave I forms R_I into LDS, rows of block q > K are
-// checked; phase B: every wave updates or replaces its tiles and publishes those in block row/column
-// K + 1; the owner of tile (K+1, K+1) updates it first and inverts it.  With half the tiles of the
-// general kernel the state fits 128 registers and 76 KB of LDS, so two realizations share a CU.  The
-// diagonal tiles are written symmetrized, so every slice matrix is exactly symmetric.
-template <int WV>
+// w + NW s (rg_sym_tile) in slot s, so with NW < WV a wave holds several diagonal tiles.  Block step K:
+//   A: wave w forms R_I = Q_I D^-1 into LDS for its I = w, w + NW, ... (I != K); rows of block
+//      q > K are checked against V = -Q_q W;
+//   B1: the owner of tile (K+1, K+1) updates it and inverts it first (D^-1, W and the pivots of the
+//      next block to LDS), then every wave updates or replaces its tiles;
+//   B2: the tiles in block row/column K+1 are stored as the next panel.
+// One panel buffer and one D^-1 buffer, three barriers per step; the slice constants share the R
+// buffer (they are read only while the slice matrix is formed).  The per-realization state is small
+// (36 tiles over NW waves, 36 KB of LDS at a row stride of 16), so several realizations share a CU
+// and hide each other's serial block inversions and barrier waits.  The diagonal tiles are written
+// symmetrized, so every slice matrix is exactly symmetric.
+template <int WV, int S>
 struct Sym32Shared {
     static constexpr int NP = 16 * WV;
-    alignas(16) float qr[2][NP][LDW], qi[2][NP][LDW];  // panel Q_I = S_IK [row][k], by step parity
-    alignas(16) float rr[NP][LDW], ri[NP][LDW];        // R_I = Q_I D^-1 [row][k]
-    alignas(16) float dr[2][16][LDW], di[2][16][LDW];  // D^-1 row-major; symmetric, so also transposed
-    alignas(16) float wtr[2][16][LDW], wti[2][16][LDW];  // W = U_D^-1 diag(u), transposed
-    double ua[2][16];
+    alignas(16) float qr[NP][S], qi[NP][S];  // panel Q_I = S_IK [row][k] of the current block
+    union {
+        struct {
+            alignas(16) float rr[NP][S], ri[NP][S];  // R_I = Q_I D^-1 [row][k]
+        } r;
+        struct {  // the slice constants, read only while the slice matrix is formed
+            alignas(16) double cq[2 * 128 + 8];  // np <= 128: 2 n + 1 Dirichlet table entries
+            double2 bv[NP], ad[NP];
+        } m;
+    } u;
+    alignas(16) float dr[16][S], di[16][S];    // D^-1 of the current block, row-major (symmetric)
+    alignas(16) float wtr[16][S], wti[16][S];  // W = U_D^-1 diag(u), transposed
+    double ua[16];
     int bad;
-    alignas(16) double cq[2 * 128 + 8];  // np <= 128: 2 n + 1 Dirichlet table entries
-    double2 bv[NP], ad[NP];
 };
 
-// The owner of diagonal block K inverts its tile in place and publishes D^-1, W and the pivot
-// magnitudes to buffer K & 1.
-template <int WV>
-__device__ __attribute__((always_inline)) inline void publish_sym32(np_f4 &xr, np_f4 &xi, int K, int lane,
-                                                                    Sym32Shared<WV> &sh)
+// The owner of the next diagonal block inverts its tile in place and publishes D^-1, W and the pivot
+// magnitudes.
+template <int WV, int S>
+__device__ __attribute__((always_inline)) inline void publish_sym32(np_f4 &xr, np_f4 &xi, int lane,
+                                                                    Sym32Shared<WV, S> &sh)
 {
-    const int c = lane & 15, g = lane >> 4, buf = K & 1;
+    const int c = lane & 15, g = lane >> 4;
     np_f4 lr, li;
-    const bool ok = block_gj(xr, xi, lr, li, sh.ua[buf], lane);
+    const bool ok = block_gj(xr, xi, lr, li, sh.ua, lane);
     if (!ok && lane == 0) sh.bad = 1;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        sh.dr[buf][4 * g + v][c] = xr[v];
-        sh.di[buf][4 * g + v][c] = xi[v];
+        sh.dr[4 * g + v][c] = xr[v];
+        sh.di[4 * g + v][c] = xi[v];
     }
     __builtin_amdgcn_wave_barrier();
     np_f4 wr, wi;
-    tile_mul(wr, wi, sh.dr[buf], sh.di[buf], lr, li, lane);  // W = D^-1 (L_D diag(u))
-    *reinterpret_cast<np_f4 *>(&sh.wtr[buf][c][4 * g]) = wr;
-    *reinterpret_cast<np_f4 *>(&sh.wti[buf][c][4 * g]) = wi;
+    tile_mul(wr, wi, sh.dr, sh.di, lr, li, lane);  // W = D^-1 (L_D diag(u))
+    *reinterpret_cast<np_f4 *>(&sh.wtr[c][4 * g]) = wr;
+    *reinterpret_cast<np_f4 *>(&sh.wti[c][4 * g]) = wi;
 }
 
 // Tile (I, J) into the panel of pivot block P (the tile lies in block row or column P, off the diagonal).
-template <int WV>
+template <int WV, int S>
 __device__ __attribute__((always_inline)) inline void panel_store_sym32(np_f4 xr, np_f4 xi, int I, int J, int P,
-                                                                        int lane, Sym32Shared<WV> &sh)
+                                                                        int lane, Sym32Shared<WV, S> &sh)
 {
-    const int c = lane & 15, g = lane >> 4, buf = P & 1;
+    I = __builtin_amdgcn_readfirstlane(I);  // addresses recomputed per step (see sweep_tile32)
+    J = __builtin_amdgcn_readfirstlane(J);
+    asm volatile("" : "+s"(I), "+s"(J));
+    const int c = lane & 15, g = lane >> 4;
     if (J == P) {  // Q_I = S_IP: the tile as it stands
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            sh.qr[buf][16 * I + 4 * g + v][c] = xr[v];
-            sh.qi[buf][16 * I + 4 * g + v][c] = xi[v];
+            sh.qr[16 * I + 4 * g + v][c] = xr[v];
+            sh.qi[16 * I + 4 * g + v][c] = xi[v];
         }
     } else {  // Q_J = S_PJ^T: lane (c, g) holds Q_J[c][4g + v]
-        *reinterpret_cast<np_f4 *>(&sh.qr[buf][16 * J + c][4 * g]) = xr;
-        *reinterpret_cast<np_f4 *>(&sh.qi[buf][16 * J + c][4 * g]) = xi;
+        *reinterpret_cast<np_f4 *>(&sh.qr[16 * J + c][4 * g]) = xr;
+        *reinterpret_cast<np_f4 *>(&sh.qi[16 * J + c][4 * g]) = xi;
     }
 }
 
-template <int WV, int NW>
-__global__ __launch_bounds__(64 * NW, NW / 2) void k_inverse_sym32(float2 *__restrict__ gss, const rg_cqv cqv,
-                                                                   const double2 *__restrict__ adiag,
-                                                                   const double2 *__restrict__ bvec, int n, int first,
-                                                                   int32_t *__restrict__ redo, int cs,
-                                                                   const int32_t *__restrict__ nxr, int ix)
+// Block step K on tile (I, J): replaced when it lies in block row or column K, else updated.
+template <int WV, int S>
+__device__ __attribute__((always_inline)) inline void sweep_tile32(np_f4 &xr, np_f4 &xi, int I, int J, int K,
+                                                                   int lane, Sym32Shared<WV, S> &sh)
+{
+    // the tile's LDS addresses are recomputed at every step: hoisted out of the step loop, those of all
+    // slots would stay live and spill
+    I = __builtin_amdgcn_readfirstlane(I);
+    J = __builtin_amdgcn_readfirstlane(J);
+    asm volatile("" : "+s"(I), "+s"(J));
+    const int c = lane & 15, g = lane >> 4;
+    if (I == K && J == K) {  // -D^-1 (this wave inverted the tile in place)
+        xr = -xr;
+        xi = -xi;
+    } else if (J == K) {  // R_I
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            xr[v] = sh.u.r.rr[16 * I + 4 * g + v][c];
+            xi[v] = sh.u.r.ri[16 * I + 4 * g + v][c];
+        }
+    } else if (I == K) {  // R_J^T
+        load_b(xr, xi, sh.u.r.rr + 16 * J, sh.u.r.ri + 16 * J, lane);
+    } else {  // S_IJ -= Q_I R_J^T
+        np_f4 br, bi;
+        load_b(br, bi, sh.u.r.rr + 16 * J, sh.u.r.ri + 16 * J, lane);
+        tile_fnms(xr, xi, sh.qr, sh.qi, 16 * I, br, bi, lane);
+    }
+}
+
+template <int WV, int NW, int S, int OCC>
+__global__ __launch_bounds__(64 * NW, OCC) void k_inverse_sym32(float2 *__restrict__ gss, const rg_cqv cqv,
+                                                                const double2 *__restrict__ adiag,
+                                                                const double2 *__restrict__ bvec, int n, int first,
+                                                                int32_t *__restrict__ redo, int cs,
+                                                                const int32_t *__restrict__ nxr, int ix)
 {
     adiag += (size_t)blockIdx.x * 3 * cs;
     if (nxr && nxr[blockIdx.x] == ix) adiag += cs;
     bvec += (size_t)blockIdx.x * cs;
     constexpr int NP = 16 * WV, NT = 64 * NW, NTILE = WV * (WV + 1) / 2, NS = (NTILE + NW - 1) / NW;
-    static_assert(NW >= WV, "every diagonal tile is slot 0 of its own wave");
-    __shared__ Sym32Shared<WV> sh;
+    constexpr int ND = (WV + NW - 1) / NW;  // slots that can hold a diagonal tile
+    static_assert(NW <= WV, "every wave holds a diagonal tile");
+    __shared__ Sym32Shared<WV, S> sh;
     const int tid = threadIdx.x, lane = tid & 63, c = lane & 15, g = lane >> 4;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int b = blockIdx.x;
     float2 *G = gss + (size_t)b * NP * NP;
-
-    if (cqv.cq) rg_load_cq(sh.cq, cqv, b, tid, NT);
-    for (int i = tid; i < NP; i += NT) {
-        sh.bv[i] = i < n ? bvec[i] : cmk(0.0, 0.0);
-        sh.ad[i] = i < n ? adiag[i] : cmk(0.0, 0.0);
-    }
-    if (tid == 0) sh.bad = 0;
-    __syncthreads();
 
     int sI[NS], sJ[NS];
     bool sv[NS];
@@ -477,13 +517,32 @@ __global__ __launch_bounds__(64 * NW, NW / 2) void k_inverse_sym32(float2 *__res
         sI[s] = sJ[s] = 0;
         if (sv[s]) rg_sym_tile(w + NW * s, WV, sI[s], sJ[s]);
     }
-
-    // M = (diag(a) - V) - B Gss B in FP64 on the lower tiles, rounded; identity on the padding.
+    // The previous Gss on the lower tiles, loaded before the constants are staged so that the loads
+    // overlap it.
     np_f4 ar[NS], ai[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         ar[s] = np_f4{0.f, 0.f, 0.f, 0.f};
         ai[s] = np_f4{0.f, 0.f, 0.f, 0.f};
+        if (!sv[s] || first) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const float2 gv = G[(size_t)(16 * sI[s] + 4 * g + v) * NP + 16 * sJ[s] + c];
+            ar[s][v] = gv.x;
+            ai[s][v] = gv.y;
+        }
+    }
+    if (cqv.cq) rg_load_cq(sh.u.m.cq, cqv, b, tid, NT);
+    for (int i = tid; i < NP; i += NT) {
+        sh.u.m.bv[i] = i < n ? bvec[i] : cmk(0.0, 0.0);
+        sh.u.m.ad[i] = i < n ? adiag[i] : cmk(0.0, 0.0);
+    }
+    if (tid == 0) sh.bad = 0;
+    __syncthreads();
+
+    // M = (diag(a) - V) - B Gss B in FP64 on the lower tiles, rounded; identity on the padding.
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
         if (!sv[s]) continue;
         const int col = 16 * sJ[s] + c;
 #pragma unroll
@@ -491,12 +550,10 @@ __global__ __launch_bounds__(64 * NW, NW / 2) void k_inverse_sym32(float2 *__res
             const int row = 16 * sI[s] + 4 * g + v;
             double2 m;
             if (row < n && col < n) {
-                m = row == col ? sh.ad[row] : cmk(0.0, 0.0);
-                if (cqv.cq) m.x -= sh.cq[abs(row - col)] - sh.cq[row + col + 2];
-                if (!first) {
-                    const float2 gv = G[(size_t)row * NP + col];
-                    m = csub(m, cmul(cmul(sh.bv[row], cmk((double)gv.x, (double)gv.y)), sh.bv[col]));
-                }
+                m = row == col ? sh.u.m.ad[row] : cmk(0.0, 0.0);
+                if (cqv.cq) m.x -= sh.u.m.cq[abs(row - col)] - sh.u.m.cq[row + col + 2];
+                if (!first)
+                    m = csub(m, cmul(cmul(sh.u.m.bv[row], cmk((double)ar[s][v], (double)ai[s][v])), sh.u.m.bv[col]));
             } else {
                 m = cmk(row == col ? 1.0 : 0.0, 0.0);
             }
@@ -504,85 +561,93 @@ __global__ __launch_bounds__(64 * NW, NW / 2) void k_inverse_sym32(float2 *__res
             ai[s][v] = (float)m.y;
         }
     }
+    // Panel of block 0 and its inverse (the constants are not read after this point: the barrier
+    // below hands their space to R).
 #pragma unroll
     for (int s = 0; s < NS; s++)
         if (sv[s] && sJ[s] == 0 && sI[s] > 0) panel_store_sym32(ar[s], ai[s], sI[s], 0, 0, lane, sh);
-    if (w == 0) publish_sym32(ar[0], ai[0], 0, lane, sh);
+    if (w == 0) publish_sym32(ar[0], ai[0], lane, sh);
     __syncthreads();
 
     for (int K = 0; K < WV; K++) {
-        const int buf = K & 1;
-        // Phase A: R_w = Q_w D^-1; the rows of block q > K against the pivots of block K.
-        if (w < WV && w != K) {
+        // A: R_I = Q_I D^-1 for this wave's I; the rows of block q > K against the pivots of block K.
+#pragma unroll
+        for (int r = 0; r < ND; r++) {
+            int I = w + NW * r;
+            if (I >= WV || I == K) continue;
+            asm volatile("" : "+s"(I));  // addresses recomputed per step (see sweep_tile32)
             np_f4 br, bi, xr, xi;
-            load_b(br, bi, sh.dr[buf], sh.di[buf], lane);
-            tile_mul(xr, xi, sh.qr[buf] + 16 * w, sh.qi[buf] + 16 * w, br, bi, lane);
+            load_b(br, bi, sh.dr, sh.di, lane);
+            tile_mul(xr, xi, sh.qr + 16 * I, sh.qi + 16 * I, br, bi, lane);
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                sh.rr[16 * w + 4 * g + v][c] = xr[v];
-                sh.ri[16 * w + 4 * g + v][c] = xi[v];
+                sh.u.r.rr[16 * I + 4 * g + v][c] = xr[v];
+                sh.u.r.ri[16 * I + 4 * g + v][c] = xi[v];
             }
         }
         for (int q = K + 1; q < WV; q++) {
             if ((WV + q) % NW != w) continue;
             np_f4 br, bi, vr = np_f4{0.f, 0.f, 0.f, 0.f}, vi = np_f4{0.f, 0.f, 0.f, 0.f};
-            load_b(br, bi, sh.wtr[buf], sh.wti[buf], lane);
-            tile_fnms(vr, vi, sh.qr[buf], sh.qi[buf], 16 * q, br, bi, lane);  // V = -Q_q W
+            load_b(br, bi, sh.wtr, sh.wti, lane);
+            tile_fnms(vr, vi, sh.qr, sh.qi, 16 * q, br, bi, lane);  // V = -Q_q W
             bool ok = true;
-            const double ua = sh.ua[buf][c];
+            const double ua = sh.ua[c];
 #pragma unroll
             for (int v = 0; v < 4; v++)
                 if (fabs((double)vr[v]) + fabs((double)vi[v]) > NP_MARGIN * ua) ok = false;
             if (!__all(ok) && lane == 0) sh.bad = 1;
         }
         __syncthreads();
-        // Phase B.
+        // B1: the next diagonal tile first, then the rest.
+        if (K + 1 < WV) {
 #pragma unroll
-        for (int s = 0; s < NS; s++) {
-            if (!sv[s]) continue;
-            const int I = sI[s], J = sJ[s];
-            if (I == K && J == K) {  // -D^-1 (this wave inverted the tile in place)
-                ar[s] = -ar[s];
-                ai[s] = -ai[s];
-            } else if (J == K) {  // R_I
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    ar[s][v] = sh.rr[16 * I + 4 * g + v][c];
-                    ai[s][v] = sh.ri[16 * I + 4 * g + v][c];
-                }
-            } else if (I == K) {  // R_J^T
-                load_b(ar[s], ai[s], sh.rr + 16 * J, sh.ri + 16 * J, lane);
-            } else {  // S_IJ -= Q_I R_J^T
-                np_f4 br, bi;
-                load_b(br, bi, sh.rr + 16 * J, sh.ri + 16 * J, lane);
-                tile_fnms(ar[s], ai[s], sh.qr[buf], sh.qi[buf], 16 * I, br, bi, lane);
-            }
-            if (K + 1 < WV) {
-                if (s == 0 && I == K + 1 && J == K + 1) publish_sym32(ar[0], ai[0], K + 1, lane, sh);
-                if ((I == K + 1) != (J == K + 1)) panel_store_sym32(ar[s], ai[s], I, J, K + 1, lane, sh);
+            for (int s = 0; s < ND; s++) {
+                if (!sv[s] || sI[s] != K + 1 || sJ[s] != K + 1) continue;
+                sweep_tile32(ar[s], ai[s], K + 1, K + 1, K, lane, sh);
+                publish_sym32(ar[s], ai[s], lane, sh);
             }
         }
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            if (!sv[s] || (s < ND && sI[s] == K + 1 && sJ[s] == K + 1)) continue;
+            sweep_tile32(ar[s], ai[s], sI[s], sJ[s], K, lane, sh);
+        }
         __syncthreads();
+        // B2: the next panel.
+        if (K + 1 < WV) {
+#pragma unroll
+            for (int s = 0; s < NS; s++)
+                if (sv[s] && (sI[s] == K + 1) != (sJ[s] == K + 1))
+                    panel_store_sym32(ar[s], ai[s], sI[s], sJ[s], K + 1, lane, sh);
+            __syncthreads();
+        }
     }
     if (sh.bad) {
         if (tid == 0) redo[b] = 1;
         return;
     }
     if (tid == 0) redo[b] = 0;
-    // Gss = -S, both triangles, each diagonal tile symmetrized (X + X^T) / 2 (the panel buffer is free).
-    if (w < WV) {
+    // Gss = -S, both triangles, each diagonal tile symmetrized (X + X^T) / 2 through the (free) panel
+    // buffer.
+#pragma unroll
+    for (int s = 0; s < ND; s++) {
+        if (!sv[s] || sI[s] != sJ[s]) continue;
+        const int I = sI[s];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            sh.qr[0][16 * w + 4 * g + v][c] = ar[0][v];
-            sh.qi[0][16 * w + 4 * g + v][c] = ai[0][v];
+            sh.qr[16 * I + 4 * g + v][c] = ar[s][v];
+            sh.qi[16 * I + 4 * g + v][c] = ai[s][v];
         }
     }
     __syncthreads();
-    if (w < WV) {
-        const np_f4 tr = *reinterpret_cast<const np_f4 *>(&sh.qr[0][16 * w + c][4 * g]);
-        const np_f4 tq = *reinterpret_cast<const np_f4 *>(&sh.qi[0][16 * w + c][4 * g]);
-        ar[0] = 0.5f * (ar[0] + tr);
-        ai[0] = 0.5f * (ai[0] + tq);
+#pragma unroll
+    for (int s = 0; s < ND; s++) {
+        if (!sv[s] || sI[s] != sJ[s]) continue;
+        const int I = sI[s];
+        const np_f4 tr = *reinterpret_cast<const np_f4 *>(&sh.qr[16 * I + c][4 * g]);
+        const np_f4 tq = *reinterpret_cast<const np_f4 *>(&sh.qi[16 * I + c][4 * g]);
+        ar[s] = 0.5f * (ar[s] + tr);
+        ai[s] = 0.5f * (ai[s] + tq);
     }
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -601,8 +666,8 @@ __global__ __launch_bounds__(64 * NW, NW / 2) void k_inverse_sym32(float2 *__res
 
 }  // namespace
 
-// Complex symmetric slice matrices (Dirichlet guides), np >= 32, complex64 state: 8 waves per
-// realization, two realizations per CU.
+// Complex symmetric slice matrices (Dirichlet guides), np >= 32, complex64 state: 4 waves per
+// realization (NW = 4 for np >= 64) and 36 KB of LDS at np = 128, so that four realizations share a CU.
 void rg_launch_inverse_sym32(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo)
 {
     hipEvent_t stop = nullptr;
@@ -610,18 +675,18 @@ void rg_launch_inverse_sym32(rg_plan *P, int nb, int slot, int first, bool with_
     const rg_cqv cq = rg_cq_view(P, nb, with_disorder);
     const double2 *ad = P->d_adiag + (size_t)slot * P->np;
     switch (P->np / 16) {
-#define RG_SYM(W, NW)                                                                                          \
-    case W:                                                                                                    \
-        hipLaunchKernelGGL((k_inverse_sym32<W, NW>), dim3(nb), dim3(64 * NW), 0, P->stream, P->d_gss32, cq, ad, \
-                           P->d_b, P->nmu, first, redo, P->cstride, P->d_nx_r, P->cur_ix);                     \
+#define RG_SYM(W, NW)                                                                                        \
+    case W:                                                                                                  \
+        hipLaunchKernelGGL((k_inverse_sym32<W, NW, 16, 4>), dim3(nb), dim3(64 * NW), 0, P->stream, P->d_gss32, \
+                           cq, ad, P->d_b, P->nmu, first, redo, P->cstride, P->d_nx_r, P->cur_ix);           \
         break;
         RG_SYM(2, 2)
         RG_SYM(3, 3)
         RG_SYM(4, 4)
-        RG_SYM(5, 5)
-        RG_SYM(6, 6)
-        RG_SYM(7, 7)
-        RG_SYM(8, 8)
+        RG_SYM(5, 4)
+        RG_SYM(6, 4)
+        RG_SYM(7, 4)
+        RG_SYM(8, 4)
 #undef RG_SYM
         default: break;
     }

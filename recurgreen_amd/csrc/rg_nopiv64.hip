@@ -382,88 +382,176 @@ __global__ __launch_bounds__(64 * WV, 2) void k_inverse_nopiv64(double2 *__restr
 //   S_IJ -= Q_I D^-1 Q_J^T (I >= J, I, J != K),  S_IK := R_I = Q_I D^-1,  S_KK := -D^-1,
 // which after every block leaves S = -M^-1.  These are the diagonal-pivot Gauss-Jordan steps of
 // k_inverse_nopiv64 with the signs of the eliminated column flipped, on WV (WV + 1) / 2 tiles instead
-// of WV^2; the pivot checks are the same.  Wave w holds the tiles t = w + NW s (diagonal tiles first,
-// then the off-diagonal ones by distance from the diagonal) in register slot s.  Block step K:
-//   phase A: wave I (I != K) forms R_I into LDS; row blocks q > K are checked against V = -Q_q W;
-//   phase B: every wave updates or replaces its tiles, publishes those in block row/column K + 1 as the
-//   next panel; the owner of tile (K+1, K+1) updates it first and inverts it (look-ahead).
-// Two workgroup barriers per block step.  Only the lower triangle of the previous Gss is read; both
-// triangles of the new one are written.
+// of WV^2; the pivot checks are the same.  Wave w holds the tiles t = w + NW s (rg_sym_tile: diagonal
+// tiles first, then the off-diagonal ones by distance) in register slot s.  Block step K:
+//   A: wave w forms R_I = Q_I D^-1 into LDS for its I = w, w + NW, ... (I != K); rows of block q > K
+//      are checked against V = -Q_q W;
+//   B1: the owner of tile (K+1, K+1) updates it and inverts it first (D^-1, W and the pivots of the
+//      next block to LDS), then every wave updates or replaces its tiles;
+//   B2: the tiles in block row/column K+1 are stored as the next panel.
+// One panel buffer and one D^-1 buffer, three barriers per step; the slice constants share the R
+// buffer.  At np = 128 with 4 waves of 9 tiles and a row stride of 16 doubles (72 KB of LDS) two
+// realizations share a CU and hide each other's serial block inversions and barrier waits.  With that
+// stride every row starts in the same bank, so the 32-byte chunks of a row are XOR-swizzled by the
+// row (sw).  Only the lower triangle of the previous Gss is read; both triangles of the new one are
+// written, each diagonal tile symmetrized.
 
-template <int WV>
+// Column `col` (already in kp order) of row `row`: the 4-double chunk index XOR the row's low bits.
+__device__ __forceinline__ constexpr int sw(int row, int col) { return col ^ (4 * (row & 3)); }
+
+// C (tile) -= A B with A rows r0.. from a swizzled LDS matrix ([row][sw(kp(k))]), B in accumulator layout.
+template <int S>
+__device__ __forceinline__ void tile_fnms64s(d4 &cr, d4 &ci, const double (*asr)[S], const double (*asi)[S], int r0,
+                                             d4 br, d4 bi, int lane)
+{
+    const int i = lane & 15, g = lane >> 4;
+    const d4 a_r = *reinterpret_cast<const d4 *>(&asr[r0 + i][sw(i, 4 * g)]);
+    const d4 a_i = *reinterpret_cast<const d4 *>(&asi[r0 + i][sw(i, 4 * g)]);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        cr = mfma64(-a_r[v], br[v], cr);
+        cr = mfma64(a_i[v], bi[v], cr);
+        ci = mfma64(-a_r[v], bi[v], ci);
+        ci = mfma64(-a_i[v], br[v], ci);
+    }
+}
+
+// C = A B, A rows r0.. from a swizzled LDS matrix.
+template <int S>
+__device__ __forceinline__ void tile_mul64s(d4 &cr, d4 &ci, const double (*asr)[S], const double (*asi)[S], int r0,
+                                            d4 br, d4 bi, int lane)
+{
+    const int i = lane & 15, g = lane >> 4;
+    const d4 a_r = *reinterpret_cast<const d4 *>(&asr[r0 + i][sw(i, 4 * g)]);
+    const d4 a_i = *reinterpret_cast<const d4 *>(&asi[r0 + i][sw(i, 4 * g)]);
+    cr = d4{0.0, 0.0, 0.0, 0.0};
+    ci = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        cr = mfma64(a_r[v], br[v], cr);
+        cr = mfma64(-a_i[v], bi[v], cr);
+        ci = mfma64(a_r[v], bi[v], ci);
+        ci = mfma64(a_i[v], br[v], ci);
+    }
+}
+
+// B operand (accumulator layout: register v = B[4v + g][c]) from the swizzled LDS rows r0 + c holding
+// B^T, i.e. [r0 + c][sw(kp(k))] = B[k][c]: one 32-byte read per plane.
+template <int S>
+__device__ __forceinline__ void load_b64s(d4 &br, d4 &bi, const double (*tr)[S], const double (*ti)[S], int r0,
+                                          int lane)
+{
+    const int c = lane & 15, g = lane >> 4;
+    br = *reinterpret_cast<const d4 *>(&tr[r0 + c][sw(c, 4 * g)]);
+    bi = *reinterpret_cast<const d4 *>(&ti[r0 + c][sw(c, 4 * g)]);
+}
+
+template <int WV, int S>
 struct Sym64Shared {
     static constexpr int NP = 16 * WV;
-    alignas(32) double qr[2][NP][LDD], qi[2][NP][LDD];     // panel Q_I = S_IK [row][kp(col)], by step parity
-    alignas(32) double rr[NP][LDD], ri[NP][LDD];           // R_I = Q_I D^-1 [row][kp(col)]
-    alignas(32) double dr[2][16][LDD], di[2][16][LDD];     // D^-1 [row][kp(col)]; symmetric, so also [col][kp(row)]
-    alignas(32) double wtr[2][16][LDD], wti[2][16][LDD];   // W = U_D^-1 diag(u), transposed
-    double ua[2][16];
+    alignas(32) double qr[NP][S], qi[NP][S];  // panel Q_I = S_IK [row][sw(kp(col))] of the current block
+    union {
+        struct {
+            alignas(32) double rr[NP][S], ri[NP][S];  // R_I = Q_I D^-1 [row][sw(kp(col))]
+        } r;
+        struct {  // the slice constants, read only while the slice matrix is formed
+            alignas(16) double cq[2 * 128 + 8];  // np <= 128: 2 n + 1 Dirichlet table entries
+            double2 bv[NP], ad[NP];
+        } m;
+    } u;
+    alignas(32) double dr[16][S], di[16][S];    // D^-1 of the current block [row][sw(kp(col))] (symmetric)
+    alignas(32) double wtr[16][S], wti[16][S];  // W = U_D^-1 diag(u), transposed
+    double ua[16];
     int bad;
-    alignas(16) double cq[2 * 256 + 8];
-    double2 bv[NP], ad[NP];
 };
 
-// The owner of diagonal block K inverts its tile (xr, xi) in place and publishes D^-1, W and the pivot
-// magnitudes to buffer K & 1.
-template <int WV>
-__device__ __forceinline__ void publish_sym64(d4 &xr, d4 &xi, int K, int lane, Sym64Shared<WV> &sh)
+// The owner of the next diagonal block inverts its tile in place and publishes D^-1, W and the pivot
+// magnitudes.
+template <int WV, int S>
+__device__ __forceinline__ void publish_sym64(d4 &xr, d4 &xi, int lane, Sym64Shared<WV, S> &sh)
 {
-    const int c = lane & 15, g = lane >> 4, buf = K & 1;
+    const int c = lane & 15, g = lane >> 4;
     d4 lr, li;
-    const bool ok = block_gj64(xr, xi, lr, li, sh.ua[buf], lane);
+    const bool ok = block_gj64(xr, xi, lr, li, sh.ua, lane);
     if (!ok && lane == 0) sh.bad = 1;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        sh.dr[buf][4 * v + g][kp(c)] = xr[v];
-        sh.di[buf][4 * v + g][kp(c)] = xi[v];
+        sh.dr[4 * v + g][sw(4 * v + g, kp(c))] = xr[v];
+        sh.di[4 * v + g][sw(4 * v + g, kp(c))] = xi[v];
     }
     __builtin_amdgcn_wave_barrier();
     d4 wr, wi;
-    tile_mul64(wr, wi, sh.dr[buf], sh.di[buf], lr, li, lane);  // W = D^-1 (L_D diag(u))
-    *reinterpret_cast<d4 *>(&sh.wtr[buf][c][4 * g]) = wr;
-    *reinterpret_cast<d4 *>(&sh.wti[buf][c][4 * g]) = wi;
+    tile_mul64s(wr, wi, sh.dr, sh.di, 0, lr, li, lane);  // W = D^-1 (L_D diag(u))
+    *reinterpret_cast<d4 *>(&sh.wtr[c][sw(c, 4 * g)]) = wr;  // [c][kp(4v + g)] = [c][4g + v]
+    *reinterpret_cast<d4 *>(&sh.wti[c][sw(c, 4 * g)]) = wi;
 }
 
 // Tile (I, J) into the panel of pivot block P (the tile lies in block row or column P, off the diagonal).
-template <int WV>
-__device__ __forceinline__ void panel_store_sym64(d4 xr, d4 xi, int I, int J, int P, int lane, Sym64Shared<WV> &sh)
+template <int WV, int S>
+__device__ __forceinline__ void panel_store_sym64(d4 xr, d4 xi, int I, int J, int P, int lane, Sym64Shared<WV, S> &sh)
 {
-    const int c = lane & 15, g = lane >> 4, buf = P & 1;
+    I = __builtin_amdgcn_readfirstlane(I);  // addresses recomputed per step (see sweep_tile64)
+    J = __builtin_amdgcn_readfirstlane(J);
+    asm volatile("" : "+s"(I), "+s"(J));
+    const int c = lane & 15, g = lane >> 4;
     if (J == P) {  // Q_I = S_IP: the tile as it stands
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            sh.qr[buf][16 * I + 4 * v + g][kp(c)] = xr[v];
-            sh.qi[buf][16 * I + 4 * v + g][kp(c)] = xi[v];
+            sh.qr[16 * I + 4 * v + g][sw(4 * v + g, kp(c))] = xr[v];
+            sh.qi[16 * I + 4 * v + g][sw(4 * v + g, kp(c))] = xi[v];
         }
-    } else {  // Q_J = S_JP = S_PJ^T: lane (c, g) holds Q_J[c][4v + g], stored at [c][kp(4v + g)] = [c][4g + v]
-        *reinterpret_cast<d4 *>(&sh.qr[buf][16 * J + c][4 * g]) = xr;
-        *reinterpret_cast<d4 *>(&sh.qi[buf][16 * J + c][4 * g]) = xi;
+    } else {  // Q_J = S_PJ^T: lane (c, g) holds Q_J[c][4v + g], stored at [c][kp(4v + g)] = [c][4g + v]
+        *reinterpret_cast<d4 *>(&sh.qr[16 * J + c][sw(c, 4 * g)]) = xr;
+        *reinterpret_cast<d4 *>(&sh.qi[16 * J + c][sw(c, 4 * g)]) = xi;
     }
 }
 
-template <int WV, int NW>
-__global__ __launch_bounds__(64 * NW, (NW > 8 ? NW / 4 : 2)) void k_inverse_sym64(
-    double2 *__restrict__ gss, const rg_cqv cqv, const double2 *__restrict__ adiag, const double2 *__restrict__ bvec,
-    int n, int first, int32_t *__restrict__ redo, int cs, const int32_t *__restrict__ nxr, int ix)
+// Block step K on tile (I, J): replaced when it lies in block row or column K, else updated.
+template <int WV, int S>
+__device__ __forceinline__ void sweep_tile64(d4 &xr, d4 &xi, int I, int J, int K, int lane, Sym64Shared<WV, S> &sh)
+{
+    // the tile's LDS addresses are recomputed at every step: hoisted out of the step loop, those of all
+    // slots would stay live and spill
+    I = __builtin_amdgcn_readfirstlane(I);
+    J = __builtin_amdgcn_readfirstlane(J);
+    asm volatile("" : "+s"(I), "+s"(J));
+    const int c = lane & 15, g = lane >> 4;
+    if (I == K && J == K) {  // -D^-1 (this wave inverted the tile in place)
+        xr = -xr;
+        xi = -xi;
+    } else if (J == K) {  // R_I
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            xr[v] = sh.u.r.rr[16 * I + 4 * v + g][sw(4 * v + g, kp(c))];
+            xi[v] = sh.u.r.ri[16 * I + 4 * v + g][sw(4 * v + g, kp(c))];
+        }
+    } else if (I == K) {  // R_J^T
+        load_b64s(xr, xi, sh.u.r.rr, sh.u.r.ri, 16 * J, lane);
+    } else {  // S_IJ -= Q_I R_J^T
+        d4 br, bi;
+        load_b64s(br, bi, sh.u.r.rr, sh.u.r.ri, 16 * J, lane);
+        tile_fnms64s(xr, xi, sh.qr, sh.qi, 16 * I, br, bi, lane);
+    }
+}
+
+template <int WV, int NW, int S, int OCC>
+__global__ __launch_bounds__(64 * NW, OCC) void k_inverse_sym64(double2 *__restrict__ gss, const rg_cqv cqv,
+                                                                const double2 *__restrict__ adiag,
+                                                                const double2 *__restrict__ bvec, int n, int first,
+                                                                int32_t *__restrict__ redo, int cs,
+                                                                const int32_t *__restrict__ nxr, int ix)
 {
     adiag += (size_t)blockIdx.x * 3 * cs;
     if (nxr && nxr[blockIdx.x] == ix) adiag += cs;
     bvec += (size_t)blockIdx.x * cs;
     constexpr int NP = 16 * WV, NT = 64 * NW, NTILE = WV * (WV + 1) / 2, NS = (NTILE + NW - 1) / NW;
-    static_assert(NW >= WV, "every diagonal tile is slot 0 of its own wave");
-    __shared__ Sym64Shared<WV> sh;
+    constexpr int ND = (WV + NW - 1) / NW;  // slots that can hold a diagonal tile
+    static_assert(NW <= WV, "every wave holds a diagonal tile");
+    __shared__ Sym64Shared<WV, S> sh;
     const int tid = threadIdx.x, lane = tid & 63, c = lane & 15, g = lane >> 4;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int b = blockIdx.x;
     double2 *G = gss + (size_t)b * NP * NP;
-
-    if (cqv.cq) rg_load_cq(sh.cq, cqv, b, tid, NT);
-    for (int i = tid; i < NP; i += NT) {
-        sh.bv[i] = i < n ? bvec[i] : cmk(0.0, 0.0);
-        sh.ad[i] = i < n ? adiag[i] : cmk(0.0, 0.0);
-    }
-    if (tid == 0) sh.bad = 0;
-    __syncthreads();
 
     int sI[NS], sJ[NS];
     bool sv[NS];
@@ -473,13 +561,32 @@ __global__ __launch_bounds__(64 * NW, (NW > 8 ? NW / 4 : 2)) void k_inverse_sym6
         sI[s] = sJ[s] = 0;
         if (sv[s]) rg_sym_tile(w + NW * s, WV, sI[s], sJ[s]);
     }
-
-    // M = (diag(a) - V) - B Gss B on the lower tiles; identity on the padding.
+    // The previous Gss on the lower tiles, loaded before the constants are staged so that the loads
+    // overlap it.
     d4 ar[NS], ai[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         ar[s] = d4{0.0, 0.0, 0.0, 0.0};
         ai[s] = d4{0.0, 0.0, 0.0, 0.0};
+        if (!sv[s] || first) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const double2 gv = G[(size_t)(16 * sI[s] + 4 * v + g) * NP + 16 * sJ[s] + c];
+            ar[s][v] = gv.x;
+            ai[s][v] = gv.y;
+        }
+    }
+    if (cqv.cq) rg_load_cq(sh.u.m.cq, cqv, b, tid, NT);
+    for (int i = tid; i < NP; i += NT) {
+        sh.u.m.bv[i] = i < n ? bvec[i] : cmk(0.0, 0.0);
+        sh.u.m.ad[i] = i < n ? adiag[i] : cmk(0.0, 0.0);
+    }
+    if (tid == 0) sh.bad = 0;
+    __syncthreads();
+
+    // M = (diag(a) - V) - B Gss B on the lower tiles; identity on the padding.
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
         if (!sv[s]) continue;
         const int col = 16 * sJ[s] + c;
 #pragma unroll
@@ -487,9 +594,9 @@ __global__ __launch_bounds__(64 * NW, (NW > 8 ? NW / 4 : 2)) void k_inverse_sym6
             const int row = 16 * sI[s] + 4 * v + g;
             double2 m;
             if (row < n && col < n) {
-                m = row == col ? sh.ad[row] : cmk(0.0, 0.0);
-                if (cqv.cq) m.x -= sh.cq[abs(row - col)] - sh.cq[row + col + 2];
-                if (!first) m = csub(m, cmul(cmul(sh.bv[row], G[(size_t)row * NP + col]), sh.bv[col]));
+                m = row == col ? sh.u.m.ad[row] : cmk(0.0, 0.0);
+                if (cqv.cq) m.x -= sh.u.m.cq[abs(row - col)] - sh.u.m.cq[row + col + 2];
+                if (!first) m = csub(m, cmul(cmul(sh.u.m.bv[row], cmk(ar[s][v], ai[s][v])), sh.u.m.bv[col]));
             } else {
                 m = cmk(row == col ? 1.0 : 0.0, 0.0);
             }
@@ -497,66 +604,66 @@ __global__ __launch_bounds__(64 * NW, (NW > 8 ? NW / 4 : 2)) void k_inverse_sym6
             ai[s][v] = m.y;
         }
     }
-    // Panel of block 0 and its inverse.
+    // Panel of block 0 and its inverse (the constants are not read after this point: the barrier
+    // below hands their space to R).
 #pragma unroll
     for (int s = 0; s < NS; s++)
         if (sv[s] && sJ[s] == 0 && sI[s] > 0) panel_store_sym64(ar[s], ai[s], sI[s], 0, 0, lane, sh);
-    if (w == 0) publish_sym64(ar[0], ai[0], 0, lane, sh);
+    if (w == 0) publish_sym64(ar[0], ai[0], lane, sh);
     __syncthreads();
 
     for (int K = 0; K < WV; K++) {
-        const int buf = K & 1;
-        // Phase A: R_w = Q_w D^-1; the rows of block q > K against the pivots of block K.
-        if (w < WV && w != K) {
-            d4 br, bi, rr, ri;
-            load_b64(br, bi, sh.dr[buf], sh.di[buf], lane);
-            tile_mul64(rr, ri, sh.qr[buf] + 16 * w, sh.qi[buf] + 16 * w, br, bi, lane);
+        // A: R_I = Q_I D^-1 for this wave's I; the rows of block q > K against the pivots of block K.
+#pragma unroll
+        for (int r = 0; r < ND; r++) {
+            int I = w + NW * r;
+            if (I >= WV || I == K) continue;
+            asm volatile("" : "+s"(I));  // addresses recomputed per step (see sweep_tile64)
+            d4 br, bi, xr, xi;
+            load_b64s(br, bi, sh.dr, sh.di, 0, lane);  // D^-1 is symmetric: its rows serve as B^T
+            tile_mul64s(xr, xi, sh.qr, sh.qi, 16 * I, br, bi, lane);
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                sh.rr[16 * w + 4 * v + g][kp(c)] = rr[v];
-                sh.ri[16 * w + 4 * v + g][kp(c)] = ri[v];
+                sh.u.r.rr[16 * I + 4 * v + g][sw(4 * v + g, kp(c))] = xr[v];
+                sh.u.r.ri[16 * I + 4 * v + g][sw(4 * v + g, kp(c))] = xi[v];
             }
         }
         for (int q = K + 1; q < WV; q++) {
             if ((WV + q) % NW != w) continue;
             d4 br, bi, vr = d4{0.0, 0.0, 0.0, 0.0}, vi = d4{0.0, 0.0, 0.0, 0.0};
-            load_b64(br, bi, sh.wtr[buf], sh.wti[buf], lane);
-            tile_fnms64(vr, vi, sh.qr[buf], sh.qi[buf], 16 * q, br, bi, lane);  // V = -Q_q W
+            load_b64s(br, bi, sh.wtr, sh.wti, 0, lane);
+            tile_fnms64s(vr, vi, sh.qr, sh.qi, 16 * q, br, bi, lane);  // V = -Q_q W
             bool ok = true;
-            const double ua = sh.ua[buf][c];
+            const double ua = sh.ua[c];
 #pragma unroll
             for (int v = 0; v < 4; v++)
                 if (fabs(vr[v]) + fabs(vi[v]) > N64_MARGIN * ua) ok = false;
             if (!__all(ok) && lane == 0) sh.bad = 1;
         }
         __syncthreads();
-        // Phase B.
+        // B1: the next diagonal tile first, then the rest.
+        if (K + 1 < WV) {
 #pragma unroll
-        for (int s = 0; s < NS; s++) {
-            if (!sv[s]) continue;
-            const int I = sI[s], J = sJ[s];
-            if (I == K && J == K) {  // -D^-1 (this wave inverted the tile in place)
-                ar[s] = -ar[s];
-                ai[s] = -ai[s];
-            } else if (J == K) {  // R_I
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    ar[s][v] = sh.rr[16 * I + 4 * v + g][kp(c)];
-                    ai[s][v] = sh.ri[16 * I + 4 * v + g][kp(c)];
-                }
-            } else if (I == K) {  // R_J^T
-                load_b64(ar[s], ai[s], sh.rr + 16 * J, sh.ri + 16 * J, lane);
-            } else {  // S_IJ -= Q_I R_J^T
-                d4 br, bi;
-                load_b64(br, bi, sh.rr + 16 * J, sh.ri + 16 * J, lane);
-                tile_fnms64(ar[s], ai[s], sh.qr[buf], sh.qi[buf], 16 * I, br, bi, lane);
-            }
-            if (K + 1 < WV) {
-                if (s == 0 && I == K + 1 && J == K + 1) publish_sym64(ar[0], ai[0], K + 1, lane, sh);
-                if ((I == K + 1) != (J == K + 1)) panel_store_sym64(ar[s], ai[s], I, J, K + 1, lane, sh);
+            for (int s = 0; s < ND; s++) {
+                if (!sv[s] || sI[s] != K + 1 || sJ[s] != K + 1) continue;
+                sweep_tile64(ar[s], ai[s], K + 1, K + 1, K, lane, sh);
+                publish_sym64(ar[s], ai[s], lane, sh);
             }
         }
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            if (!sv[s] || (s < ND && sI[s] == K + 1 && sJ[s] == K + 1)) continue;
+            sweep_tile64(ar[s], ai[s], sI[s], sJ[s], K, lane, sh);
+        }
         __syncthreads();
+        // B2: the next panel.
+        if (K + 1 < WV) {
+#pragma unroll
+            for (int s = 0; s < NS; s++)
+                if (sv[s] && (sI[s] == K + 1) != (sJ[s] == K + 1))
+                    panel_store_sym64(ar[s], ai[s], sI[s], sJ[s], K + 1, lane, sh);
+            __syncthreads();
+        }
     }
     if (sh.bad) {
         if (tid == 0) redo[b] = 1;
@@ -567,19 +674,25 @@ __global__ __launch_bounds__(64 * NW, (NW > 8 ? NW / 4 : 2)) void k_inverse_sym6
     // the diagonal tiles whole, and the rounding asymmetry of a diagonal tile is not damped by the
     // recursion (on some guides it doubles from slice to slice); written exactly symmetric, every slice
     // matrix is.  The panel buffer is free by now.
-    if (w < WV) {
+#pragma unroll
+    for (int s = 0; s < ND; s++) {
+        if (!sv[s] || sI[s] != sJ[s]) continue;
+        const int I = sI[s];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            sh.qr[0][16 * w + 4 * v + g][c] = ar[0][v];
-            sh.qi[0][16 * w + 4 * v + g][c] = ai[0][v];
+            sh.qr[16 * I + 4 * v + g][c] = ar[s][v];
+            sh.qi[16 * I + 4 * v + g][c] = ai[s][v];
         }
     }
     __syncthreads();
-    if (w < WV) {
+#pragma unroll
+    for (int s = 0; s < ND; s++) {
+        if (!sv[s] || sI[s] != sJ[s]) continue;
+        const int I = sI[s];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            ar[0][v] = 0.5 * (ar[0][v] + sh.qr[0][16 * w + c][4 * v + g]);
-            ai[0][v] = 0.5 * (ai[0][v] + sh.qi[0][16 * w + c][4 * v + g]);
+            ar[s][v] = 0.5 * (ar[s][v] + sh.qr[16 * I + c][4 * v + g]);
+            ai[s][v] = 0.5 * (ai[s][v] + sh.qi[16 * I + c][4 * v + g]);
         }
     }
 #pragma unroll
@@ -597,38 +710,30 @@ __global__ __launch_bounds__(64 * NW, (NW > 8 ? NW / 4 : 2)) void k_inverse_sym6
 
 }  // namespace
 
-// Complex symmetric slice matrices (Dirichlet guides), np >= 32.  At np = 128 the workgroup has 12
-// waves (3 tiles each; measured 1.61 ms per 4096 realizations against 1.67 with 8 waves and 1.78 with
-// 16); RG_SYM64_WAVES = 8 or 16 selects the others, for measurement.
+// Complex symmetric slice matrices (Dirichlet guides), np >= 32: 4 waves per realization (NW = np / 16
+// below 64) and 72 KB of LDS at np = 128, so that two realizations share a CU.
 void rg_launch_inverse_sym64(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo)
 {
-    static const int waves = [] {
-        const char *e = std::getenv("RG_SYM64_WAVES");
-        const int v = e ? std::atoi(e) : 12;
-        return v == 8 || v == 16 ? v : 12;
-    }();
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_INVERSE, &stop);
     const rg_cqv cq = rg_cq_view(P, nb, with_disorder);
     const double2 *ad = P->d_adiag + (size_t)slot * P->np;
-#define RG_SYM(W, NW)                                                                                             \
-    hipLaunchKernelGGL((k_inverse_sym64<W, NW>), dim3(nb), dim3(64 * NW), 0, P->stream, P->d_gss, cq, ad, P->d_b,    \
-                       P->nmu, first, redo, P->cstride, P->d_nx_r, P->cur_ix)
     switch (P->np / 16) {
-        case 2: RG_SYM(2, 2); break;
-        case 3: RG_SYM(3, 3); break;
-        case 4: RG_SYM(4, 4); break;
-        case 5: RG_SYM(5, 5); break;
-        case 6: RG_SYM(6, 6); break;
-        case 7: RG_SYM(7, 7); break;
-        case 8:
-            if (waves == 16) RG_SYM(8, 16);
-            else if (waves == 12) RG_SYM(8, 12);
-            else RG_SYM(8, 8);
-            break;
+#define RG_SYM(W, NW)                                                                                         \
+    case W:                                                                                                   \
+        hipLaunchKernelGGL((k_inverse_sym64<W, NW, 16, 2>), dim3(nb), dim3(64 * NW), 0, P->stream, P->d_gss, cq, \
+                           ad, P->d_b, P->nmu, first, redo, P->cstride, P->d_nx_r, P->cur_ix);               \
+        break;
+        RG_SYM(2, 2)
+        RG_SYM(3, 3)
+        RG_SYM(4, 4)
+        RG_SYM(5, 4)
+        RG_SYM(6, 4)
+        RG_SYM(7, 4)
+        RG_SYM(8, 4)
+#undef RG_SYM
         default: break;
     }
-#undef RG_SYM
     rg_timer_end(P, RGK_INVERSE, stop);
 }
 
