@@ -7,7 +7,9 @@
 // whole batch and one slice this is the product  C[b][q] = sum_i U[b][i] tab[i][q]:
 // M = realizations, N = table columns (2N+1 or 2N), K = ny transverse cells.
 //
-// Each workgroup (4 wavefronts) owns 32 realizations (two 16-row MFMA tiles), one contiguous range
+// Batches of 2048 realizations and more use k_project_wide (below: 128 realizations per workgroup, the
+// table staged straight into LDS, the seeded disorder generated in the kernel).  Smaller batches:
+// each workgroup (4 wavefronts) owns 32 realizations (two 16-row MFMA tiles), one contiguous range
 // of 16-cell chunks of K and one group of up to 20 table column tiles (blockIdx.z); wavefront w owns
 // the tiles {w, w+4, ...} of its group.  The slice
 // values U (generated for the slice by k_noise_slice, or read from an explicit potential batch) are
@@ -28,6 +30,17 @@ constexpr int PR_MAXNT = 20;  // table column tiles supported (ntab <= 320)
 constexpr int PR_TPW = (PR_MAXNT + 3) / 4;
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Disorder of realization b, cell iy of slice ixd: stdu N(seed, ixd, iy) with the seed seed0 + b (or
+// *seedp + seed0 + b in a captured sweep, or seeds[b] and stdus[b] in an uneven batch).
+__device__ __forceinline__ double noise_cell(int b, int ixd, int iy, int64_t seed0, const int64_t *__restrict__ seedp,
+                                             double stdu, const int64_t *__restrict__ seeds,
+                                             const double *__restrict__ stdus)
+{
+    const int64_t s = seeds ? seeds[b] : (seedp ? *seedp + seed0 : seed0) + b;
+    const double sd = stdus ? stdus[b] : stdu;
+    return sd * rg_normal_cell((uint64_t)s, 0u, ixd, iy);
+}
 
 // One chunk of slice values into s_u[KC][RB]: two cells per thread, u(b, iy) = us[b sb + iy si].
 __device__ inline void stage_chunk(double (*s_u)[PR_RB + 1], int chunk, int b0, int nb, const double *__restrict__ us,
@@ -52,9 +65,90 @@ __global__ __launch_bounds__(256) void k_noise_slice(int ixd, int64_t seed0, con
                                                      const double *__restrict__ stdus, double *__restrict__ us)
 {
     const int b = blockIdx.y, iy = blockIdx.x * 256 + threadIdx.x;
-    const int64_t s = seeds ? seeds[b] : (seedp ? *seedp + seed0 : seed0) + b;
-    const double sd = stdus ? stdus[b] : stdu;
-    if (iy < ny) us[(size_t)b * ny + iy] = sd * rg_normal_cell((uint64_t)s, 0u, ixd, iy);
+    if (iy < ny) us[(size_t)b * ny + iy] = noise_cell(b, ixd, iy, seed0, seedp, stdu, seeds, stdus);
+}
+
+// The projection of large batches: each workgroup (8 wavefronts) owns 128 realizations and one
+// contiguous range of 16-cell chunks; wavefront w computes rows 16w.. against all NT column tiles.  The
+// table chunk (16 x 16 NT doubles) is staged once per workgroup in LDS, one chunk ahead, so the table
+// is read from L2 once per 128 realizations; the slice values of the next chunk are loaded (or, with
+// gen, generated here: the values of k_noise_slice) while the current chunk's products run.  MFMA
+// k-step s of a chunk takes cell 4 (lane / 16) + s of it in lane group lane / 16, for A and B alike.
+constexpr int PW_RB = 128;
+constexpr int PW_MAXNT = 17;  // ntab <= 272 (Dirichlet and periodic guides up to 128 modes)
+
+template <int NT>
+__global__ __launch_bounds__(512, 1) void k_project_wide(const double *__restrict__ us, int64_t sb, int si, int ny,
+                                                         int gen, int ixd, int64_t seed0,
+                                                         const int64_t *__restrict__ seedp, double stdu,
+                                                         const int64_t *__restrict__ seeds,
+                                                         const double *__restrict__ stdus,
+                                                         const double *__restrict__ tab, int ntp, int ntab,
+                                                         int nchunk, int parts, double *__restrict__ cq, int nb)
+{
+    __shared__ double s_t[2][PR_KC][16 * NT];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
+    const int b0 = blockIdx.x * PW_RB, part = blockIdx.y;
+    const int c0 = (int)((int64_t)part * nchunk / parts), c1 = (int)((int64_t)(part + 1) * nchunk / parts);
+    const int b = b0 + 16 * w + li;  // the realization of this lane's A operand rows
+    const bool bok = b < nb;
+
+    dbl4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    double a_cur[4], a_nxt[4];
+    auto a_chunk = [&](int c, double (&a)[4]) {
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int iy = c * PR_KC + 4 * lk + s;
+            a[s] = 0.0;
+            if (bok && iy < ny)
+                a[s] = gen ? noise_cell(b, ixd, iy, seed0, seedp, stdu, seeds, stdus) : us[(size_t)b * sb + (size_t)iy * si];
+        }
+    };
+    // Rows 16 c .. 16 c + 15 of the zero-padded table (ntp = 16 NT: one contiguous block here and in LDS),
+    // loaded straight into LDS: lane l of a wave writes the 16 bytes at the wave's LDS base + 16 l.
+    auto stage_tab = [&](int c, int buf) {
+        constexpr int UNITS = PR_KC * 16 * NT / 2;  // 16-byte units, a multiple of 64
+        const double *src = tab + (size_t)c * PR_KC * ntp;
+        double *dst = &s_t[buf][0][0];
+        for (int r = 0; r < UNITS; r += 512) {
+            const int u0 = r + 64 * w;  // this wave's first unit (wave-uniform test)
+            if (u0 < UNITS) __builtin_amdgcn_global_load_lds(src + 2 * (u0 + lane), dst + 2 * u0, 16, 0, 0);
+        }
+    };
+    if (c0 < c1) {
+        stage_tab(c0, 0);
+        a_chunk(c0, a_cur);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // the staged chunk is in LDS
+    __syncthreads();
+    for (int c = c0; c < c1; c++) {
+        const int buf = (c - c0) & 1;
+        if (c + 1 < c1) {
+            stage_tab(c + 1, buf ^ 1);
+            a_chunk(c + 1, a_nxt);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_cur[s], s_t[buf][4 * lk + s][16 * t + li], acc[t], 0, 0, 0);
+        __builtin_amdgcn_s_waitcnt(0);  // the next chunk is in LDS
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 4; s++) a_cur[s] = a_nxt[s];
+    }
+    double *out = cq + (size_t)part * nb * ntab;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int q = 16 * t + li;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int bb = b0 + 16 * w + 4 * v + lk;
+            if (bb < nb && q < ntab) out[(size_t)bb * ntab + q] = acc[t][v];
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_project_mfma(const double *__restrict__ us, int64_t sb, int si, int ny,
@@ -131,6 +225,38 @@ void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double 
     const double *us = d_upot ? d_upot + ixd : nullptr;  // explicit potentials upot(ix, iy, b)
     int64_t sb = (int64_t)P->nxdiso * P->ny;
     int si = P->nxdiso;
+    // Large batches (one 128-realization workgroup per CU and partial sum at least): the wide kernel,
+    // generating the seeded disorder itself unless RG_PROJECT_GEN=0.
+    static const int gen_env = [] {
+        const char *e = std::getenv("RG_PROJECT_GEN");
+        return e ? std::atoi(e) : 1;
+    }();
+    const int nti = P->ntp / 16;
+    if (nb >= 16 * PW_RB && nti <= PW_MAXNT) {
+        const int gen = !d_upot && gen_env;
+        if (!d_upot && !gen) {
+            hipLaunchKernelGGL(k_noise_slice, dim3((P->ny + 255) / 256, nb), dim3(256), 0, P->stream, ixd, seed0,
+                               P->seed_dev, P->ny, P->stdu, P->d_seeds_r, P->d_stdu_r, P->d_us);
+            us = P->d_us;
+            sb = P->ny;
+            si = 1;
+        }
+        const dim3 grid((nb + PW_RB - 1) / PW_RB, P->cq_parts);
+        switch (nti) {
+#define RG_PW(T)                                                                                                   \
+    case T:                                                                                                        \
+        hipLaunchKernelGGL(k_project_wide<T>, grid, dim3(512), 0, P->stream, us, sb, si, P->ny, gen, ixd, seed0,    \
+                           P->seed_dev, P->stdu, P->d_seeds_r, P->d_stdu_r, P->d_tab, P->ntp, P->ntab, P->nchunk, \
+                           P->cq_parts, P->d_cq, nb);                                                              \
+        break;
+            RG_PW(1) RG_PW(2) RG_PW(3) RG_PW(4) RG_PW(5) RG_PW(6) RG_PW(7) RG_PW(8) RG_PW(9)
+            RG_PW(10) RG_PW(11) RG_PW(12) RG_PW(13) RG_PW(14) RG_PW(15) RG_PW(16) RG_PW(17)
+#undef RG_PW
+            default: break;
+        }
+        rg_timer_end(P, RGK_PROJECT, stop);
+        return;
+    }
     if (!d_upot) {  // seeded: this slice's disorder first
         hipLaunchKernelGGL(k_noise_slice, dim3((P->ny + 255) / 256, nb), dim3(256), 0, P->stream, ixd, seed0,
                            P->seed_dev, P->ny, P->stdu, P->d_seeds_r, P->d_stdu_r, P->d_us);

@@ -156,3 +156,35 @@ def test_bf16x6_dyson_matches_oracle(wol):
     T1, st1, code1 = plan.tdistrib_potential(U, "mixed_fp32")
     assert code1 == 0 and np.max(np.abs(T1 - Tg)) <= MIXED_TOL["mixed_fp32"]
     plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed_fp32"])
+def test_wide_projection_matches_small_batches(precision):
+    """Batches of 2048 and more take the wide projection kernel (128 realizations per workgroup, the
+    seeded disorder generated inside it; explicit potentials loaded); smaller batches the 32-realization
+    kernel fed by k_noise_slice.  The same realizations through both must agree: complex128 within the
+    FP64 bar, complex64 within the mixed bound (the two kernels add the cells in different orders)."""
+    p = dict(CASES["N128"], nxdiso=6)
+    say(f"wide projection against small batches, {precision}")
+    big = binding.Plan(p, batch_max=2048)
+    small = binding.Plan(p, batch_max=4)
+    picks = [0, 1, 1023, 2047]
+    Tb, st, code = big.tdistrib(11, 2048, precision)
+    assert code == 0 and not st.any()
+    Tb = Tb.reshape(2048, -1)
+    U = big.generate_potential(11, 2048)  # (nxdiso, ny, 2048)
+    Tbu, st, code = big.tdistrib_potential(U, precision)
+    assert code == 0 and not st.any()
+    Tbu = Tbu.reshape(2048, -1)
+    for k in picks:
+        Ts, st, code = small.tdistrib(11 + k, 1, precision)
+        assert code == 0 and not st.any()
+        Tsu, st, code = small.tdistrib_potential(U[:, :, k:k + 1], precision)
+        assert code == 0 and not st.any()
+        for a, ref in ((Tb[k], Ts), (Tbu[k], Tsu), (Tb[k], Tsu)):
+            if precision == "fp64":
+                assert_T_close(a, ref)
+            else:
+                assert np.max(np.abs(a - ref)) <= MIXED_TOL["mixed_fp32"]
+    big.close()
+    small.close()
