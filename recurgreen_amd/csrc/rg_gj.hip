@@ -280,6 +280,14 @@ static void launch_gj(rg_plan *P, int nb, typename CT<T>::C *g, const rg_cqv cq,
         RG_GJ(6)
         RG_GJ(7)
         RG_GJ(8)
+        RG_GJ(9)
+        RG_GJ(10)
+        RG_GJ(11)
+        RG_GJ(12)
+        RG_GJ(13)
+        RG_GJ(14)
+        RG_GJ(15)
+        RG_GJ(16)
 #undef RG_GJ
         default: break;
     }

@@ -408,7 +408,7 @@ struct Sym32Shared {
             alignas(16) float rr[NP][S], ri[NP][S];  // R_I = Q_I D^-1 [row][k]
         } r;
         struct {  // the slice constants, read only while the slice matrix is formed
-            alignas(16) double cq[2 * 128 + 8];  // np <= 128: 2 n + 1 Dirichlet table entries
+            alignas(16) double cq[2 * NP + 8];  // 2 n + 1 Dirichlet table entries
             double2 bv[NP], ad[NP];
         } m;
     } u;
@@ -666,8 +666,9 @@ __global__ __launch_bounds__(64 * NW, OCC) void k_inverse_sym32(float2 *__restri
 
 }  // namespace
 
-// Complex symmetric slice matrices (Dirichlet guides), np >= 32, complex64 state: 4 waves per
-// realization (NW = 4 for np >= 64) and 36 KB of LDS at np = 128, so that four realizations share a CU.
+// Complex symmetric slice matrices (Dirichlet guides), 32 <= np <= 256, complex64 state: 4 waves per
+// realization up to np = 128 (36 KB of LDS at np = 128, so that four realizations share a CU), one wave
+// per 16-column block above (np = 256: 16 waves, 70 KB).
 void rg_launch_inverse_sym32(rg_plan *P, int nb, int slot, int first, bool with_disorder, int32_t *redo)
 {
     hipEvent_t stop = nullptr;
@@ -687,6 +688,14 @@ void rg_launch_inverse_sym32(rg_plan *P, int nb, int slot, int first, bool with_
         RG_SYM(6, 4)
         RG_SYM(7, 4)
         RG_SYM(8, 4)
+        RG_SYM(9, 9)
+        RG_SYM(10, 10)
+        RG_SYM(11, 11)
+        RG_SYM(12, 12)
+        RG_SYM(13, 13)
+        RG_SYM(14, 14)
+        RG_SYM(15, 15)
+        RG_SYM(16, 16)
 #undef RG_SYM
         default: break;
     }

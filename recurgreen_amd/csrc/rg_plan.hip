@@ -254,6 +254,17 @@ int inverse_kernel(const rg_plan *P, int precision)
     return inv;
 }
 
+// Orders above 128 invert through the Schur complement, except complex64 sweeps with symmetric slice
+// matrices, which the symmetric kernel inverts directly (one wave per 16-column block) with the pivot
+// checks over the whole matrix: the Schur complement keeps the pivoting inside its two blocks, and in
+// FP32 the complement D - C A^-1 B loses what zgetrf's pivoting across the blocks would keep (a
+// potential spike of 1e5 standard deviations: T off by 4e-2 at N = 256).
+bool schur_inversion(const rg_plan *P, int precision)
+{
+    if (P->np <= 128) return false;
+    return !(precision != RG_FP64 && inverse_kernel(P, precision) == RG_INV_SYM);
+}
+
 // One slice step with the slice's diagonal (slot 0 first, 1 bulk, 2 last) and disordered-slice index
 // ixd (-1: no disorder) given; flip = false leaves *cur for the caller to flip (several ranges of
 // one batch stepping together).
@@ -264,7 +275,7 @@ void sweep_step(rg_plan *P, int nb, int ix, int slot, int ixd, int64_t seed0, co
     const bool dis = ixd >= 0;
     if (dis) rg_launch_project(P, nb, ixd, seed0, d_upot, nb);
     int inv = inverse_kernel(P, precision);
-    if (P->np > 128) rg_launch_inverse_schur(P, nb, slot, ix == 1, dis, precision);
+    if (schur_inversion(P, precision)) rg_launch_inverse_schur(P, nb, slot, ix == 1, dis, precision);
     else if (inv == RG_INV_REGISTER) rg_launch_inverse_f64(P, nb, slot, ix == 1, dis);
     else if (inv == RG_INV_LOOKAHEAD) rg_launch_inverse_gj(P, nb, slot, ix == 1, dis, precision);
     else if (inv == RG_INV_NOPIV || inv == RG_INV_SYM) {
@@ -672,7 +683,7 @@ void fwd_step_f32(rg_plan *P, int nb, int ix, const double *d_upot, const float2
     const bool dis = d_upot && id >= 1 && id <= P->nxdiso;
     if (dis) rg_launch_project(P, nb, id - 1, 0, d_upot, nb);
     const int slot = ix == 1 ? 0 : (ix == P->nx ? 2 : 1);
-    if (P->np > 128) {
+    if (schur_inversion(P, RG_MIXED_FP32)) {
         rg_launch_inverse_schur(P, nb, slot, ix == 1, dis, RG_MIXED_FP32);
     } else if (P->np >= 32) {
         if (inverse_kernel(P, RG_MIXED_FP32) == RG_INV_SYM) rg_launch_inverse_sym32(P, nb, slot, ix == 1, dis, P->d_redo);
