@@ -207,10 +207,12 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *   "inverse_mixed" 1 look-ahead, 2 blocked, 3 verified diagonal pivots, 4 (default) the same on the
  *                   lower triangle of complex symmetric slice matrices (Dirichlet; 3 for periodic guides);
  *   "dyson"         complex64 Dyson product: 0 64x64 output tiles, 1 one workgroup per realization
- *                   (default, NP 80..128), 2 the same on the BF16 matrix cores with every FP32
+ *                   (NP 80..128), 2 the same on the BF16 matrix cores with every FP32
  *                   operand split into three BF16 values (six products, FP32 accuracy per
  *                   product; NP 96 and 128, else 1; faster, but the N = 128 sweep's mean T moved
- *                   4.7e-7 from complex128 against 2.0e-8 with 1, so not the default);
+ *                   4.7e-7 from complex128 against 2.0e-8 with 1, so not the default), 3 (default)
+ *                   one workgroup per realization with three real products per complex product
+ *                   (Gauss's form; NP 96 and 128, else 1);
  *   "dyson_fp64"    the same choice for the complex128 product (default 1);
  *   "project_parts" 1..8 partial sums of the projection over transverse cells;
  *   "lanes"         1 one stream, 2 the batch split in halves on two streams;

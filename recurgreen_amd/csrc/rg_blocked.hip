@@ -603,6 +603,7 @@ void rg_launch_gemm_sp(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_
     rg_timer_begin(P, RGK_GEMM, &stop);
     if (precision == RG_MIXED_FP32) {
         const bool done = (P->gemm_wg == 2 && rg_launch_gemm_bf3(P, nb, gss, gs1_old, gs1_new)) ||
+                          (P->gemm_wg == 3 && rg_launch_gemm_3m(P, nb, gss, gs1_old, gs1_new)) ||
                           (P->gemm_wg >= 1 && rg_launch_gemm_wg(P, nb, gss, gs1_old, gs1_new));
         if (!done) launch_gemm_sp<RG_MIXED_FP32>(P, nb, gss, gs1_old, gs1_new);
     }

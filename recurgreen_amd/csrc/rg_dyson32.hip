@@ -132,6 +132,168 @@ __global__ __launch_bounds__(512, 4) void k_dyson_gemm_wg(const float2 *__restri
         }
 }
 
+// ---- Three real products per complex product (option dyson = 3) ---------------------------------------
+//
+// The same product with the complex multiplication of Gauss: for A = Ar + i Ai, B = Br + i Bi,
+//   P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi),  Re AB = P1 - P2,  Im AB = P3 - P1 - P2,
+// three real products instead of four (25% fewer matrix-core instructions).  Each product is summed
+// on its own FP32 accumulator; the sums Ar + Ai and Br + Bi are formed in FP32 from the staged
+// operands.  The error of Im AB is bounded by eps32 (|P1| + |P2| + |P3|) sums, of the size of the
+// four-product form's for operands of random phase.  The three accumulators of a wavefront's 2 x 4
+// output tiles take 96 registers, so a workgroup of 8 wavefronts runs at two wavefronts per SIMD, one
+// workgroup per CU: the kernel is persistent (realizations blockIdx.x + k gridDim.x), and the chunk
+// after the current one -- the next realization's first after a realization's last -- is loaded into
+// registers while the current one is multiplied, as in the BF16 kernel.  k runs in chunks of 32
+// through a double-buffered LDS stage ([row][k] for A, [col][k] for B, row length 36 floats); k-step
+// s of a chunk takes k = 8 g + s in lane group g, the same relabelling for both operands, so each lane
+// reads its eight k values of a row (or column) as two 16-byte words.
+template <int NP>
+struct DysonShared3m {
+    static constexpr int S = 36;
+    alignas(16) float a[2][2][NP][S];  // [buffer][re, im][row][k]
+    alignas(16) float b[2][2][NP][S];  // [buffer][re, im][col][k]
+};
+
+template <int NP>
+__global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restrict__ gss, const float2 *__restrict__ gs1,
+                                                          float2 *__restrict__ out, const double2 *__restrict__ bvec,
+                                                          int n, int cs, int nb)
+{
+    static_assert(NP % 32 == 0, "k chunks of 32");
+    constexpr int T = NP / 16, NK = NP / 32, NQ = NP * 16;  // tiles, k chunks, float4 per operand chunk
+    constexpr int LOADS = NQ / 512;
+    __shared__ DysonShared3m<NP> sh;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1, li = lane & 15, lg = lane >> 4;
+    if ((int)blockIdx.x >= nb) return;  // the grid never exceeds the batch; uniform per workgroup
+
+    // chunk kc of A: row idx >> 4, complex pair idx & 15 of the row's 32 k.  Of B: the 64 loads of
+    // wave slot s = idx >> 6 cover k rows 8 (s & 3) + (lane & 7) and column pairs 8 (s >> 2) + (lane >> 3).
+    // The next chunk's A and B are staged one after the other (A loaded before the first half of the
+    // products and stored after it, then B), so that only one operand's registers are in flight.
+    float4 ra[LOADS], rb[LOADS];
+    auto load_a = [&](int r, int kc) __attribute__((always_inline)) {
+        const float4 *A4 = reinterpret_cast<const float4 *>(gss + (size_t)r * NP * NP);
+#pragma unroll
+        for (int e = 0; e < LOADS; e++) {
+            const int idx = tid + 512 * e;
+            ra[e] = A4[(size_t)(idx >> 4) * (NP / 2) + kc * 16 + (idx & 15)];
+        }
+    };
+    auto load_b = [&](int r, int kc) __attribute__((always_inline)) {
+        const float4 *B4 = reinterpret_cast<const float4 *>(gs1 + (size_t)r * NP * NP);
+#pragma unroll
+        for (int e = 0; e < LOADS; e++) {
+            const int idx = tid + 512 * e, s = idx >> 6, l = idx & 63;
+            rb[e] = B4[(size_t)(kc * 32 + 8 * (s & 3) + (l & 7)) * (NP / 2) + 8 * (s >> 2) + (l >> 3)];
+        }
+    };
+    auto store_a = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int e = 0; e < LOADS; e++) {
+            const int idx = tid + 512 * e;
+            const int row = idx >> 4, k2 = 2 * (idx & 15);
+            *reinterpret_cast<float2 *>(&sh.a[buf][0][row][k2]) = make_float2(ra[e].x, ra[e].z);
+            *reinterpret_cast<float2 *>(&sh.a[buf][1][row][k2]) = make_float2(ra[e].y, ra[e].w);
+        }
+    };
+    auto store_b = [&](int r, int kc, int buf) __attribute__((always_inline)) {
+        const double2 *bv = bvec + (size_t)r * cs;  // cs = np: constants of each realization, else shared
+#pragma unroll
+        for (int e = 0; e < LOADS; e++) {
+            const int idx = tid + 512 * e, s = idx >> 6, l = idx & 63;
+            const int k = 8 * (s & 3) + (l & 7), c = 2 * (8 * (s >> 2) + (l >> 3)), gk = kc * 32 + k;
+            const double2 bk = gk < n ? bv[gk] : cmk(0.0, 0.0);
+            sh.b[buf][0][c][k] = (float)(bk.x * rb[e].x - bk.y * rb[e].y);
+            sh.b[buf][1][c][k] = (float)(bk.x * rb[e].y + bk.y * rb[e].x);
+            sh.b[buf][0][c + 1][k] = (float)(bk.x * rb[e].z - bk.y * rb[e].w);
+            sh.b[buf][1][c + 1][k] = (float)(bk.x * rb[e].w + bk.y * rb[e].z);
+        }
+    };
+    // the eight k values 8 lg .. 8 lg + 7 of row (or column) 16 tile + li of a plane
+    auto frag = [&](const float (*plane)[DysonShared3m<NP>::S], int tile, float (&v)[8]) __attribute__((always_inline)) {
+        const float4 *p = reinterpret_cast<const float4 *>(&plane[16 * tile + li][8 * lg]);
+        const float4 x0 = p[0], x1 = p[1];
+        v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w;
+        v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
+    };
+
+    dy_f4 p1[2][4], p2[2][4], p3[2][4];
+    const int stride = gridDim.x;
+    int buf = 0;
+    load_a(blockIdx.x, 0);
+    load_b(blockIdx.x, 0);
+    store_a(0);
+    store_b(blockIdx.x, 0, 0);
+    __syncthreads();
+    for (int r = blockIdx.x; r < nb; r += stride) {
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 4; y++) {
+                p1[x][y] = dy_f4{0.f, 0.f, 0.f, 0.f};
+                p2[x][y] = dy_f4{0.f, 0.f, 0.f, 0.f};
+                p3[x][y] = dy_f4{0.f, 0.f, 0.f, 0.f};
+            }
+#pragma nounroll
+        for (int kc = 0; kc < NK; kc++) {
+            // the chunk after this one: (r, kc + 1), or the first of the next realization
+            const int nr = kc + 1 < NK ? r : r + stride, nk = kc + 1 < NK ? kc + 1 : 0;
+            const bool more = nr < nb;
+            if (more) load_a(nr, nk);
+            float ar[2][8], ai[2][8];
+#pragma unroll
+            for (int x = 0; x < 2; x++) {
+                const int rt = wr + 4 * x;
+                if (rt >= T) continue;  // uniform per wavefront
+                frag(sh.a[buf][0], rt, ar[x]);
+                frag(sh.a[buf][1], rt, ai[x]);
+            }
+#pragma unroll
+            for (int y = 0; y < 4; y++) {  // one column tile at a time: fewer live fragments
+                const int ct = wc + 2 * y;
+                if (ct >= T) continue;  // uniform per wavefront
+                float br[8], bi[8];
+                frag(sh.b[buf][0], ct, br);
+                frag(sh.b[buf][1], ct, bi);
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const float bs = br[j] + bi[j];  // the sums are formed where they are used
+#pragma unroll
+                    for (int x = 0; x < 2; x++) {
+                        if (wr + 4 * x >= T) continue;
+                        p1[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[x][j], br[j], p1[x][y], 0, 0, 0);
+                        p2[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[x][j], bi[j], p2[x][y], 0, 0, 0);
+                        p3[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[x][j] + ai[x][j], bs, p3[x][y], 0, 0, 0);
+                    }
+                }
+                // keeps the next column tile's fragment reads below this one's products (register pressure)
+                __builtin_amdgcn_sched_barrier(0);
+                if (y == 1 && more) {  // the other buffer was last read before the previous barrier
+                    store_a(buf ^ 1);
+                    load_b(nr, nk);
+                }
+            }
+            if (more) store_b(nr, nk, buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+        float2 *C = out + (size_t)r * NP * NP;
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 4; y++) {
+                const int rt = wr + 4 * x, ct = wc + 2 * y;
+                if (rt >= T || ct >= T) continue;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const float re = p1[x][y][v] - p2[x][y][v], im = p3[x][y][v] - p1[x][y][v] - p2[x][y][v];
+                    C[(size_t)(16 * rt + 4 * lg + v) * NP + 16 * ct + li] = make_float2(-re, -im);
+                }
+            }
+    }
+}
+
 }  // namespace
 
 bool rg_launch_gemm_wg(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new)
@@ -148,6 +310,23 @@ bool rg_launch_gemm_wg(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_
             return true;
         case 128:
             hipLaunchKernelGGL(k_dyson_gemm_wg<128>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b, P->nmu, P->cstride);
+            return true;
+        default: return false;
+    }
+}
+
+// Option dyson = 3: NP 96 and 128 (k chunks of 32); false otherwise.
+bool rg_launch_gemm_3m(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new)
+{
+    const int grid = nb < P->ncu ? (nb < 1 ? 1 : nb) : P->ncu;
+    switch (P->np) {
+        case 96:
+            hipLaunchKernelGGL(k_dyson_gemm_3m<96>, dim3(grid), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b,
+                               P->nmu, P->cstride, nb);
+            return true;
+        case 128:
+            hipLaunchKernelGGL(k_dyson_gemm_3m<128>, dim3(grid), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b,
+                               P->nmu, P->cstride, nb);
             return true;
         default: return false;
     }

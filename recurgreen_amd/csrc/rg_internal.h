@@ -193,8 +193,9 @@ struct rg_plan {
     // the bidiagonal SVD for complex128 sweeps and the Gram eigensolver for complex64 sweeps
     int eigen = 3;
     // complex64 Dyson product (option dyson): 0 64 x 64 tiles, 1 one realization per workgroup on FP32
-    // MFMA (NP >= 80), 2 the same on BF16 MFMA with three-way split operands (NP 96 and 128, else 1)
-    int gemm_wg = 1;
+    // MFMA (NP >= 80), 2 the same on BF16 MFMA with three-way split operands (NP 96 and 128, else 1),
+    // 3 one realization per workgroup with three real products per complex product (NP 96 and 128, else 1)
+    int gemm_wg = 3;
     int ncu = 256;  // compute units of the device: the grid of the persistent kernels
     rg_timer timers[RGK_COUNT];
     // Captured sweeps (option "graph"): one executable graph per shape/options/output, replayed with
@@ -319,6 +320,8 @@ size_t rg_schur_elems(int np);
 // rg_dyson32.hip: the FP32 Dyson product with one realization per workgroup; false if NP is not
 // one of 80, 96, 112, 128
 bool rg_launch_gemm_wg(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new);
+// the same with three real products per complex product, persistent (NP 96 and 128; false otherwise)
+bool rg_launch_gemm_3m(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new);
 // rg_dyson64.hip: the complex128 product, one workgroup per realization (NP 80..128; false otherwise)
 bool rg_launch_gemm_wg64(rg_plan *P, int nb, const double2 *gss, const double2 *gs1_old, double2 *gs1_new);
 bool rg_launch_gemm_bf3(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new);

@@ -1114,9 +1114,13 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
     else if (eg && std::strcmp(eg, "jacobi") == 0) P->eigen = RG_EIG_JACOBI;
     else if (eg && std::strcmp(eg, "bidiag") == 0) P->eigen = RG_EIG_BIDIAG;
     // complex64 Dyson product: RG_DYSON=tile64 for the 64 x 64 tiled kernel, RG_DYSON=bf16x6 for the
-    // BF16 matrix-core kernel, instead of one workgroup per realization on FP32 MFMA.
+    // BF16 matrix-core kernel, RG_DYSON=gauss3 for three real products per complex product, instead of
+    // one workgroup per realization on FP32 MFMA with four.
     const char *dy = std::getenv("RG_DYSON");
-    P->gemm_wg = dy && std::strcmp(dy, "tile64") == 0 ? 0 : (dy && std::strcmp(dy, "bf16x6") == 0 ? 2 : P->gemm_wg);
+    P->gemm_wg = dy && std::strcmp(dy, "tile64") == 0 ? 0
+                 : dy && std::strcmp(dy, "bf16x6") == 0 ? 2
+                 : dy && std::strcmp(dy, "gauss3") == 0 ? 3
+                                                        : P->gemm_wg;
     rg_status st = init_modes(P);  // argument checks first, like init_waveguide
     if (st) { delete P; return st; }
     int ndev = 0;
@@ -1698,7 +1702,7 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
         if (value < RG_INV_LOOKAHEAD || value > RG_INV_SYM) return RG_EINVAL;
         P->inv_mixed = value;
     } else if (std::strcmp(key, "dyson") == 0) {
-        if (value < 0 || value > 2) return RG_EINVAL;
+        if (value < 0 || value > 3) return RG_EINVAL;
         P->gemm_wg = value;
     } else if (std::strcmp(key, "dyson_fp64") == 0) {
         if (value != 0 && value != 1) return RG_EINVAL;

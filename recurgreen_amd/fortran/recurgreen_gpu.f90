@@ -268,22 +268,36 @@ contains
         call wguide%print_parameters(output_unit, "[INFO]")
     end subroutine
 
-    !! out_gpu/<sub>/result_N<ext> with the smallest free N (unique_filename, src/base_utils.f90:825-840),
-    !! opened with the timestamp and parameter header (print_timestamp, print_parameters).
+    !! <dir>/<sub>/result_N<ext>, <dir> the parameter-named folder of generate_directory
+    !! (src/waveguide.f90:1046-1067) under "out", with the smallest free N (unique_filename,
+    !! src/base_utils.f90:825-840); the directory is created first (create_directory, :846).
+    subroutine unique_result(wguide, sub, ext, filename)
+        type(waveguide_gpu_t), intent(in) :: wguide
+        character(len=*), intent(in) :: sub, ext
+        character(len=*), intent(out) :: filename
+        character(len=200) :: outputdir
+        integer :: ifile
+        logical :: fexist
+        call wguide%generate_directory("out", outputdir)
+        outputdir = trim(outputdir) // sub // "/"
+        call execute_command_line("mkdir -p '" // trim(outputdir) // "'")
+        do ifile = 1, 1000000
+            write (filename, '(a,i0,a)') trim(outputdir) // "result_", ifile, ext
+            inquire(file=filename, exist=fexist)
+            if (.not. fexist) exit
+        end do
+    end subroutine
+
+    !! The file of unique_result opened with the timestamp and parameter header (print_timestamp,
+    !! print_parameters).
     subroutine open_result(wguide, sub, ext, filename, outfp)
         type(waveguide_gpu_t), intent(in) :: wguide
         character(len=*), intent(in) :: sub, ext
         character(len=*), intent(out) :: filename
         integer, intent(out) :: outfp
         character(len=40) :: zone
-        integer :: ifile, dt(8)
-        logical :: fexist
-        call execute_command_line("mkdir -p out_gpu/" // sub)
-        do ifile = 1, 1000000
-            write (filename, '(a,i0,a)') "out_gpu/" // sub // "/result_", ifile, ext
-            inquire(file=filename, exist=fexist)
-            if (.not. fexist) exit
-        end do
+        integer :: dt(8)
+        call unique_result(wguide, sub, ext, filename)
         open(newunit=outfp, file=filename, action='write')
         call date_and_time(values=dt, zone=zone)
         write (outfp, '(a,i4,a,5(i0.2,a))') "%% Computed on ", dt(1), "-", dt(2), "-", dt(3), " at ", &
@@ -292,7 +306,7 @@ contains
     end subroutine
 
     !! task_wavefunction and save_wavefunction (src/main.f90:335-380): the field of one realization
-    !! in out_gpu/wavefun/result_N.wavefun.csv, rows x, y, Re psi, Im psi, Upot.
+    !! in <dir>/wavefun/result_N.wavefun.csv, rows x, y, Re psi, Im psi, Upot.
     subroutine task_wavefunction(funit)
         integer, intent(in) :: funit
         type(waveguide_gpu_t) :: wguide
@@ -331,7 +345,7 @@ contains
 
     !! task_tprofile and save_tprofile (src/main.f90:402-545): the profiles of realizations 1..nseed
     !! in batched device passes, their average over the realizations with nt > 0, and the raw and
-    !! average files under out_gpu/tprofile.
+    !! average files under <dir>/tprofile.
     subroutine task_tprofile(funit)
         integer, intent(in) :: funit
         type(waveguide_gpu_t) :: wguide
@@ -421,7 +435,7 @@ contains
 
     !! task_tprofile_group and save_tprofile_group (src/main.f90:564-735): four windows |T - tm| <= dt
     !! of the aperture-filtered matrix, averaged over the realizations with nt > 0, the mean
-    !! transmission tavg with its uncertainty, in out_gpu/tprofile_group/result_N.csv.
+    !! transmission tavg with its uncertainty, in <dir>/tprofile_group/result_N.csv.
     subroutine task_tprofile_group(funit)
         integer, intent(in) :: funit
         integer, parameter :: nprofile = 4
@@ -519,14 +533,8 @@ contains
         integer, intent(in) :: imode, nseed
         character(len=200) :: filename
         character(len=40) :: zone
-        integer :: outfp, imu, ifile, dt(8)
-        logical :: fexist
-        call execute_command_line("mkdir -p out_gpu/tpattern")
-        do ifile = 1, 1000000   !! unique_filename (src/base_utils.f90:825-840)
-            write (filename, '(a,i0,a)') "out_gpu/tpattern/result_", ifile, ".csv"
-            inquire(file=filename, exist=fexist)
-            if (.not. fexist) exit
-        end do
+        integer :: outfp, imu, dt(8)
+        call unique_result(wguide, "tpattern", ".csv", filename)
         open(newunit=outfp, file=filename, action='write')
         call date_and_time(values=dt, zone=zone)   !! print_timestamp (src/base_utils.f90:703-716)
         write (outfp, '(a,i4,a,5(i0.2,a))') "%% Computed on ", dt(1), "-", dt(2), "-", dt(3), " at ", &

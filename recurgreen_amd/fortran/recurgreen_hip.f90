@@ -142,6 +142,7 @@ module recurgreen_hip
         procedure :: get_neigv_filtered => get_neigv_filtered_waveguide_gpu
         procedure :: histogram => histogram_waveguide_gpu
         procedure :: print_parameters => print_parameters_waveguide_gpu
+        procedure :: generate_directory => generate_directory_waveguide_gpu
         procedure :: del => del_waveguide_gpu
     end type
 
@@ -438,6 +439,35 @@ contains
             trim(prefix) // " Medium: L/lscat=", self%params%dscat, ", L/labso=", self%params%dabso, &
             ", Nx_diso=", self%params%nxdiso, ", Nx_free=", self%params%nxfree, ", aratio=L/W=", self%params%aratio
     end subroutine
+
+    !! The result folder of generate_directory_waveguide (src/waveguide.f90:1046-1067):
+    !! <root>/dscat_<L/ls>/dabso_<L/la>/wol_<W/lambda>/<boundtype>/, each number written with six
+    !! decimals, a zero before the point below one, and the trailing zeros dropped ("dabso_0.").
+    subroutine generate_directory_waveguide_gpu(self, root, outputdir)
+        class(waveguide_gpu_t), intent(in) :: self
+        character(len=*), intent(in) :: root
+        character(len=*), intent(out) :: outputdir
+        outputdir = trim(root) // "/" // folder_value("dscat_", self%params%dscat) // "/" // &
+            folder_value("dabso_", self%params%dabso) // "/" // folder_value("wol_", self%params%wol) // &
+            "/" // trim(self%boundtype) // "/"
+    end subroutine
+
+    !! "<name><value>" with the value written as above.
+    function folder_value(name, x) result(res)
+        character(len=*), intent(in) :: name
+        real(c_double), intent(in) :: x
+        character(len=:), allocatable :: res
+        character(len=40) :: buf
+        integer :: n
+        write (buf, '(f0.6)') x
+        buf = adjustl(buf)
+        if (buf(1:1) == ".") buf = "0" // buf
+        n = len_trim(buf)
+        do while (n > 1 .and. buf(n:n) == "0")
+            n = n - 1
+        end do
+        res = name // buf(1:n)
+    end function
 
     subroutine del_waveguide_gpu(self)
         class(waveguide_gpu_t), intent(inout) :: self

@@ -131,11 +131,14 @@ def load_tpattern(path: str) -> np.ndarray:
     return np.array(rows)
 
 
-def task_tpattern(wguide, nseed: int, imode: int, probatype: str = "incident", outputdir: str = "out_gpu"):
-    """Compute and save: returns (path, tprob, rprob)."""
+def task_tpattern(wguide, nseed: int, imode: int, probatype: str = "incident", outputdir: str | None = None):
+    """Compute and save under <outputdir>/tpattern/ (by default the reference's parameter-named
+    folder, wguide.generate_directory()): returns (path, tprob, rprob)."""
     t0 = time.perf_counter()
     tprob, rprob, _ = compute_tpattern(wguide, nseed, imode, probatype)
     elapsed = time.perf_counter() - t0
+    if outputdir is None:
+        outputdir = wguide.generate_directory()
     d = os.path.join(outputdir, "tpattern")
     k = 1
     while os.path.exists(os.path.join(d, f"result_{k}.csv")):  # unique_filename (src/base_utils.f90:825-840)

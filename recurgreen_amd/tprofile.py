@@ -183,9 +183,11 @@ def save_tprofile(directory: str, wguide, profiles: np.ndarray, avg: np.ndarray,
     return raw, avgp
 
 
-def task_tprofile(wguide, nseed: int, tmin: float, tmax: float, outputdir: str = "out_gpu", log=print):
+def task_tprofile(wguide, nseed: int, tmin: float, tmax: float, outputdir: str | None = None, log=print):
     """task_tprofile (src/main.f90:402-493): profiles of realizations 1..nseed, their average over
-    the realizations with nt > 0, and the two files.  Returns (raw path, avg path, avg, nsample)."""
+    the realizations with nt > 0, and the two files under <outputdir>/tprofile/ (by default the
+    reference's parameter-named folder, wguide.generate_directory()).  Returns (raw path, avg path,
+    avg, nsample)."""
     if nseed <= 0:
         raise RecurGreenError(binding.RG_EINVAL, f"Invalid number of disorder realizations (received {nseed}), aborting...")
     t0 = time.perf_counter()
@@ -196,6 +198,8 @@ def task_tprofile(wguide, nseed: int, tmin: float, tmax: float, outputdir: str =
         log(f"[WARN] Found no transmission eigenfunction (nsample={nsample}), results will be undetermined...")
     else:
         log(f"[INFO] Found {nsample}/{nseed} transmission eigenfunctions, now averaging...")
+    if outputdir is None:
+        outputdir = wguide.generate_directory()
     raw, avgp = save_tprofile(os.path.join(outputdir, "tprofile"), wguide, profiles, avg, nsample, tmin, tmax,
                               nseed, elapsed)
     return raw, avgp, avg, nsample

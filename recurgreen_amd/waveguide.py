@@ -17,6 +17,18 @@ from .binding import RecurGreenError
 ERRMSG_INVALID_ARG = "RecurGreen: Invalid arguments, aborting..."  # src/base_utils.f90:56
 
 
+def _folder_value(name: str, x: float) -> str:
+    """name + x with six decimals, a zero before the point below one and the trailing zeros
+    dropped (trim_zero, src/base_utils.f90:782-795): 0.2 -> "0.2", 5 -> "5.", 0 -> "0."."""
+    return name + f"{x:.6f}".rstrip("0")
+
+
+def generate_directory(params: dict, root: str = "out") -> str:
+    """Result folder of a waveguide with these parameters (src/waveguide.f90:1046-1067)."""
+    return "/".join([root, _folder_value("dscat_", params["dscat"]), _folder_value("dabso_", params["dabso"]),
+                     _folder_value("wol_", params["wol"]), params["boundtype"]]) + "/"
+
+
 class Waveguide:
     def __init__(self, boundtype: str = "dirichlet", nxdiso: int = 100, nxfree: int = 0, wol: float = 4.3,
                  exfac: float = 1.0, aratio: float = 1.0, dscat: float = 0.2, dabso: float = 0.0,
@@ -48,6 +60,11 @@ class Waveguide:
 
     def get_parameters(self) -> dict:
         return dict(self.params)
+
+    def generate_directory(self, root: str = "out") -> str:
+        """generate_directory_waveguide (src/waveguide.f90:1046-1067): the result folder
+        <root>/dscat_<L/ls>/dabso_<L/la>/wol_<W/lambda>/<boundtype>/."""
+        return generate_directory(self.params, root)
 
     # -- transmission eigenvalues -------------------------------------------------------------
     def compute_tdistrib(self, iseed: int, tallvals: np.ndarray, precision: str = "fp64") -> None:

@@ -76,3 +76,17 @@ def test_potential_shape_checked_before_the_library_reads_it():
         with pytest.raises(binding.RecurGreenError) as e:
             binding.Plan._upot(fake, bad)
         assert e.value.code == binding.RG_EINVAL
+
+
+def test_result_folder_follows_generate_directory():
+    """The result folder of generate_directory_waveguide (src/waveguide.f90:1046-1067), whose
+    docstring example is out/dscat_0.2/dabso_0./wol_25.5/periodic/: six decimals, a zero before the
+    point below one, trailing zeros dropped (trim_zero, src/base_utils.f90:782-795)."""
+    from recurgreen_amd.waveguide import generate_directory
+
+    assert generate_directory(dict(dscat=0.2, dabso=0.0, wol=25.5, boundtype="periodic")) == \
+        "out/dscat_0.2/dabso_0./wol_25.5/periodic/"
+    assert generate_directory(dict(dscat=5.0, dabso=0.125, wol=64.25, boundtype="dirichlet"), "res") == \
+        "res/dscat_5./dabso_0.125/wol_64.25/dirichlet/"
+    assert generate_directory(dict(dscat=12.3456789, dabso=1e-7, wol=0.75, boundtype="dirichlet")) == \
+        "out/dscat_12.345679/dabso_0./wol_0.75/dirichlet/"

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from recurgreen_amd import build as rgbuild
+from recurgreen_amd.waveguide import generate_directory
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -55,7 +56,7 @@ def test_invalid_width_stops_like_the_reference(tmp_path):
     r = run(tmp_path, NML.format(wol=10.5, aratio=1.0))
     # `stop "<message>"` exits with status 0, as the reference's `stop errmsg_invalid_arg` does
     assert "Invalid arguments" in r.stderr + r.stdout
-    assert not (tmp_path / "out_gpu").exists()
+    assert not (tmp_path / "out").exists()
 
 
 def test_unknown_task_rejected(tmp_path):
@@ -72,8 +73,12 @@ def test_fortran_tdistrib_matches_python_path(tmp_path):
     print("[gpu-case] fortran host tdistrib", flush=True)
     r = run(tmp_path, NML.format(wol=10.25, aratio=10 / 10.25) + GPU_GROUP)
     assert r.returncode == 0 and "Result written" in r.stdout, r.stdout + r.stderr
-    eig = tmp_path / "out_gpu" / "distrib" / "result_1.eigvals.csv"
-    dis = tmp_path / "out_gpu" / "distrib" / "result_1.distrib.csv"
+    # the reference's result folder (generate_directory_waveguide, src/waveguide.f90:1046-1067)
+    d = tmp_path / "out" / "dscat_2." / "dabso_0." / "wol_10.25" / "dirichlet" / "distrib"
+    assert str(d) + "/" == str(tmp_path) + "/" + generate_directory(
+        dict(dscat=2.0, dabso=0.0, wol=10.25, boundtype="dirichlet")) + "distrib/"
+    eig = d / "result_1.eigvals.csv"
+    dis = d / "result_1.distrib.csv"
     T = distrib.read_real_array(str(eig))
     ow = O.Waveguide("dirichlet", nxdiso=200, wol=10.25, aratio=10 / 10.25, dscat=2.0, ny=205, dos_model=1)
     To = ow.tdistrib_batch(1, 16, nthreads=4)

@@ -8,7 +8,7 @@ import pytest
 
 from recurgreen_amd import build as rgbuild
 from recurgreen_amd import tprofile
-from recurgreen_amd.waveguide import Waveguide
+from recurgreen_amd.waveguide import Waveguide, generate_directory
 
 WG = """&waveguide
     boundtype = '{bt}'
@@ -28,6 +28,11 @@ WG = """&waveguide
 /
 """
 PARAMS = dict(nxdiso=30, nxfree=2, dscat=1.5)
+
+
+def out_dir(bt, wol):
+    """The reference's result folder for the guides of WG (src/waveguide.f90:1046-1067)."""
+    return generate_directory(dict(dscat=1.5, dabso=0.0, wol=wol, boundtype=bt))
 
 
 def run(tmp_path, text):
@@ -55,7 +60,7 @@ def test_task_argument_checks(tmp_path, task, group, msg):
     assert os.path.exists(rgbuild.FEXE), "Fortran host not built"
     r = run(tmp_path, f"&settings\n    task = '{task}'\n/\n" + WG.format(bt="dirichlet", wol=4.3, na=1.0) + group)
     assert msg in r.stderr, r.stderr
-    assert not (tmp_path / "out_gpu").exists()
+    assert not (tmp_path / "out").exists()
 
 
 @pytest.mark.gpu
@@ -65,7 +70,7 @@ def test_fortran_wavefunction_matches_python(tmp_path):
         "&wavefunction\n    iseed = 3\n    ipos = 5\n    imode = 2\n/\n"
     r = run(tmp_path, text)
     assert r.returncode == 0 and "Result written" in r.stdout, r.stdout + r.stderr
-    data = rows(tmp_path / "out_gpu" / "wavefun" / "result_1.wavefun.csv")
+    data = rows(tmp_path / out_dir("periodic", 4.3) / "wavefun" / "result_1.wavefun.csv")
     wg = Waveguide("periodic", wol=4.3, **PARAMS, batch_max=1)
     ref = wg.compute_wavefunction(3, 5, 2)
     wg.delete()
@@ -79,8 +84,8 @@ def test_fortran_tprofile_matches_python(tmp_path):
         "&tprofile\n    tmin = 0.5\n    tmax = 1.0\n    nseed = 9\n    nthreads = 4\n/\n"
     r = run(tmp_path, text)
     assert r.returncode == 0 and "Average profile written" in r.stdout, r.stdout + r.stderr
-    raw = rows(tmp_path / "out_gpu" / "tprofile" / "result_1.raw.csv")
-    avg = rows(tmp_path / "out_gpu" / "tprofile" / "result_1.avg.csv")
+    raw = rows(tmp_path / out_dir("dirichlet", 10.25) / "tprofile" / "result_1.raw.csv")
+    avg = rows(tmp_path / out_dir("dirichlet", 10.25) / "tprofile" / "result_1.avg.csv")
     wg = Waveguide("dirichlet", wol=10.25, **PARAMS, batch_max=4)
     prof, nt = tprofile.compute_tprofile_batch(wg, 1, 9, 0.5, 1.0)
     ref_avg, ns = tprofile.average_profiles(prof, nt)
@@ -98,7 +103,7 @@ def test_fortran_tprofile_group_matches_python(tmp_path):
         "&tprofile_group\n    tm = 0.1, 0.4, 0.7, 0.95\n    dt = 0.1, 0.1, 0.1, 0.05\n    nseed = 10\n    nthreads = 4\n/\n"
     r = run(tmp_path, text)
     assert r.returncode == 0 and "Average profiles written" in r.stdout, r.stdout + r.stderr
-    path = tmp_path / "out_gpu" / "tprofile_group" / "result_1.csv"
+    path = tmp_path / out_dir("dirichlet", 10.25) / "tprofile_group" / "result_1.csv"
     text_lines = open(path).read().splitlines()
     assert text_lines[6].startswith("x, tprofile") and text_lines[6].count("tprofile") == 4
     data = rows(path)

@@ -20,12 +20,12 @@ pytestmark = pytest.mark.gpu
 def test_option_round_trip_and_rejection():
     say("options round trip")
     plan = binding.Plan(CASES["c1_N20_ny205"], batch_max=2)
-    for key, values in (("inverse_fp64", (0, 1, 2, 3, 4)), ("inverse_mixed", (1, 2, 3, 4)), ("dyson", (0, 1, 2)),
+    for key, values in (("inverse_fp64", (0, 1, 2, 3, 4)), ("inverse_mixed", (1, 2, 3, 4)), ("dyson", (0, 1, 2, 3)),
                         ("project_parts", (1, 3, 8)), ("lanes", (2, 1)), ("eigen", (1, 0, 2, 3))):
         for v in values:
             plan.set_option(key, v)
             assert plan.get_option(key) == v
-    for key, v in (("inverse_fp64", 5), ("inverse_mixed", 0), ("dyson", 3), ("project_parts", 9), ("lanes", 3),
+    for key, v in (("inverse_fp64", 5), ("inverse_mixed", 0), ("dyson", 4), ("project_parts", 9), ("lanes", 3),
                    ("eigen", 4), ("nonsense", 0)):
         with pytest.raises(binding.RecurGreenError):
             plan.set_option(key, v)
@@ -49,7 +49,7 @@ def test_every_option_value_matches_oracle(name):
             gram = key == "eigen" and v == 0  # the Gram eigensolver: ~ n eps T_max absolute
             for k in range(2):
                 assert_T_close(Tg.reshape(2, -1)[k], To[k], floor=GRAM_FLOOR if gram else 0.0)
-    for key, values in (("inverse_mixed", (1, 2, 3, 4)), ("dyson", (0, 1, 2)), ("eigen", (2, 1, 0, 3))):
+    for key, values in (("inverse_mixed", (1, 2, 3, 4)), ("dyson", (0, 1, 2, 3)), ("eigen", (2, 1, 0, 3))):
         for v in values:
             plan.set_option(key, v)
             Tg, st, code = plan.tdistrib_potential(U, "mixed_fp32")
@@ -136,26 +136,61 @@ def test_graph_replays_match_direct_launches(precision):
     plan.close()
 
 
+@pytest.mark.parametrize("dyson", [2, 3])
 @pytest.mark.parametrize("wol", [48.25, 64.25])
-def test_bf16x6_dyson_matches_oracle(wol):
-    """Option dyson = 2 (the complex64 Dyson product on the BF16 matrix cores with three-way split
-    FP32 operands) at both orders it is built for, NP = 96 and 128, within the complex64 bound."""
+def test_persistent_dyson_matches_oracle(wol, dyson):
+    """Options dyson = 2 (the complex64 Dyson product on the BF16 matrix cores with three-way split
+    FP32 operands) and 3 (three real FP32 products per complex product) at both orders they are built
+    for, NP = 96 and 128, within the complex64 bound."""
     p = dict(boundtype="dirichlet", nxdiso=40, wol=wol, aratio=2.0 / wol, dscat=1.0, ny=int(20 * wol), dos_model=1)
-    say(f"bf16x6 Dyson product N={int(2 * wol)}")
+    say(f"Dyson option {dyson} N={int(2 * wol)}")
     ow = O.Waveguide(**p)
     U = upot_batch(ow, [3, 8, 9])
     To = np.stack([ow.tdistrib_potential(U[:, :, k].T) for k in range(3)])
     plan = binding.Plan(p, batch_max=3)
-    plan.set_option("dyson", 2)
+    plan.set_option("dyson", dyson)
     Tg, st, code = plan.tdistrib_potential(U, "mixed_fp32")
     assert code == 0 and not st.any()
     err = np.max(np.abs(Tg.reshape(3, -1) - To))
-    print(f"[gpu-case] bf16x6 N={int(2 * wol)}: max |dT| = {err:.3e}", flush=True)
+    print(f"[gpu-case] dyson={dyson} N={int(2 * wol)}: max |dT| = {err:.3e}", flush=True)
     assert err <= MIXED_TOL["mixed_fp32"]
     plan.set_option("dyson", 1)
     T1, st1, code1 = plan.tdistrib_potential(U, "mixed_fp32")
     assert code1 == 0 and np.max(np.abs(T1 - Tg)) <= MIXED_TOL["mixed_fp32"]
     plan.close()
+
+
+@pytest.mark.parametrize("dyson", [2, 3])
+def test_persistent_dyson_loop(dyson):
+    """Options dyson = 2 and 3 launch one workgroup per CU, each looping over realizations blockIdx.x +
+    k gridDim.x and prefetching the next realization's first chunk under the current one's last.
+    With 2 ncu + 3 realizations every workgroup runs two or three of them: all must equal the FP32
+    kernel's (dyson = 1) within the complex64 bound, and realizations from the first, second and
+    third pass of the loop must match the oracle."""
+    import torch
+
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    n = 2 * ncu + 3
+    p = dict(CASES["N128"], nxdiso=6, aratio=0.3 / 64.25)
+    say(f"Dyson option {dyson} persistent loop, {n} realizations over {ncu} workgroups")
+    plan = binding.Plan(p, batch_max=n)
+    plan.set_option("dyson", dyson)
+    Tb, st, code = plan.tdistrib(1, n, "mixed_fp32")
+    assert code == 0 and not st.any()
+    plan.set_option("dyson", 1)
+    T1, st1, code1 = plan.tdistrib(1, n, "mixed_fp32")
+    assert code1 == 0 and not st1.any()
+    plan.close()
+    neigv = Tb.size // n
+    Tb, T1 = Tb.reshape(n, neigv), T1.reshape(n, neigv)
+    err = np.max(np.abs(Tb - T1), axis=1)
+    print(f"[gpu-case] dyson={dyson} loop: max |dT| against dyson=1 {err.max():.3e} (worst realization {err.argmax()})",
+          flush=True)
+    assert err.max() <= MIXED_TOL["mixed_fp32"]
+    ow = O.Waveguide(**p)
+    for r in (0, ncu + 1, 2 * ncu + 2):  # first, second and third realization of workgroups 0, 1, 2
+        To = ow.tdistrib_batch(1 + r, 1)
+        assert np.max(np.abs(Tb[r] - To)) <= MIXED_TOL["mixed_fp32"], r
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed_fp32"])

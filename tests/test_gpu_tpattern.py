@@ -122,7 +122,7 @@ def test_fortran_tpattern_matches_python_path(tmp_path, probatype):
     f.write_text(FORTRAN_NML.format(probatype=probatype))
     r = subprocess.run([rgbuild.FEXE, str(f)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and "Average profiles written" in r.stdout, r.stdout + r.stderr
-    out = tmp_path / "out_gpu" / "tpattern" / "result_1.csv"
+    out = tmp_path / "out" / "dscat_2." / "dabso_0." / "wol_20.3" / "periodic" / "tpattern" / "result_1.csv"
     lines = out.read_text().splitlines()
     assert lines[4].startswith(f"%% Radiation pattern parameters: probatype='{probatype}', nseed=7, imode=3, time=")
     data = tpattern.load_tpattern(str(out))

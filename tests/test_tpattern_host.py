@@ -119,7 +119,7 @@ def test_fortran_tpattern_argument_checks(tmp_path, nseed, probatype, msg):
     r = _run(tmp_path, NML.format(nseed=nseed, probatype=probatype))
     assert msg in r.stderr, r.stderr
     assert "Invalid arguments" in r.stderr + r.stdout
-    assert not (tmp_path / "out_gpu").exists()
+    assert not (tmp_path / "out").exists()
 
 
 class _OraclePlan:
