@@ -213,7 +213,9 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *                   4.7e-7 from complex128 against 2.0e-8 with 1, so not the default), 3 (default)
  *                   one workgroup per realization with three real products per complex product
  *                   (Gauss's form; NP 96 and 128, else 1);
- *   "dyson_fp64"    the same choice for the complex128 product (default 1);
+ *   "dyson_fp64"    complex128 Dyson product: 0 64x64 output tiles, 1 (default) one workgroup per
+ *                   realization (NP 80..128), 2 the same with three real products per complex
+ *                   product (Gauss's form);
  *   "project_parts" 1..8 partial sums of the projection over transverse cells;
  *   "lanes"         1 one stream, 2 the batch split in halves on two streams;
  *   "eigen"         transmission eigenvalues: 0 eigenvalues of t^H t (Householder tridiagonalisation

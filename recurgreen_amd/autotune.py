@@ -27,7 +27,7 @@ from .binding import Plan
 CACHE = os.environ.get("RG_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_cache.json"))
 
 CANDIDATES = {
-    "fp64": [("inverse_fp64", (4, 3, 0, 1, 2)), ("dyson_fp64", (1, 0)), ("project_parts", (1, 2, 4, 8))],
+    "fp64": [("inverse_fp64", (4, 3, 0, 1, 2)), ("dyson_fp64", (1, 2, 0)), ("project_parts", (1, 2, 4, 8))],
     "mixed_fp32": [("inverse_mixed", (4, 3, 1, 2)), ("dyson", (1, 3, 0)), ("project_parts", (1, 2, 4, 8))],
     "mixed_bf16x3": [("inverse_mixed", (4, 3, 1, 2)), ("project_parts", (1, 2, 4, 8))],
 }

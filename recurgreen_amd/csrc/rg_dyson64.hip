@@ -26,7 +26,10 @@ struct DysonShared64 {
     double b[2][2][16][S];  // [buffer][re, im][k][col]
 };
 
-template <int NP>
+// G3: Gauss's form of the complex product, P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi),
+// Re = P1 - P2, Im = P3 - P1 - P2: three real products on three accumulators instead of four
+// products on two (the error of Im is bounded by eps (|P1| + |P2| + |P3|) sums).
+template <int NP, bool G3>
 __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__restrict__ gss,
                                                             const double2 *__restrict__ gs1, double2 *__restrict__ out,
                                                             const double2 *__restrict__ bvec, int n, int cs)
@@ -74,14 +77,55 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
             }
         }
     };
+    // G3 stages the next chunk's A and B one after the other (A loaded before the first two k-steps
+    // and stored after them, then B), so that one operand's registers are in flight at a time; the
+    // other buffer was last read before the previous barrier.
+    auto load_a = [&](int kc) __attribute__((always_inline)) {
+#pragma unroll
+        for (int e = 0; e < LOADS; e++) {
+            const int idx = tid + 512 * e;
+            if (idx < NQ) ra[e] = A[(size_t)(idx >> 4) * NP + kc * 16 + (idx & 15)];
+        }
+    };
+    auto load_b = [&](int kc) __attribute__((always_inline)) {
+#pragma unroll
+        for (int e = 0; e < LOADS; e++) {
+            const int idx = tid + 512 * e;
+            if (idx < NQ) rb[e] = B[(size_t)(kc * 16 + idx / NP) * NP + idx % NP];
+        }
+    };
+    auto store_a = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int e = 0; e < LOADS; e++) {
+            const int idx = tid + 512 * e;
+            if (idx < NQ) {
+                sh.a[buf][0][idx & 15][idx >> 4] = ra[e].x;
+                sh.a[buf][1][idx & 15][idx >> 4] = ra[e].y;
+            }
+        }
+    };
+    auto store_b = [&](int kc, int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int e = 0; e < LOADS; e++) {
+            const int idx = tid + 512 * e;
+            if (idx < NQ) {
+                const int kk = idx / NP, col = idx % NP, gk = kc * 16 + kk;
+                const double2 bk = gk < n ? bvec[gk] : cmk(0.0, 0.0);
+                sh.b[buf][0][kk][col] = bk.x * rb[e].x - bk.y * rb[e].y;
+                sh.b[buf][1][kk][col] = bk.x * rb[e].y + bk.y * rb[e].x;
+            }
+        }
+    };
 
-    dy64_d4 cr[XR][YC], ci[XR][YC];
+    // cr, ci: the real and imaginary parts, or P1 and P2 with P3 in c3 (G3)
+    dy64_d4 cr[XR][YC], ci[XR][YC], c3[G3 ? XR : 1][G3 ? YC : 1];
 #pragma unroll
     for (int x = 0; x < XR; x++)
 #pragma unroll
         for (int y = 0; y < YC; y++) {
             cr[x][y] = dy64_d4{0.0, 0.0, 0.0, 0.0};
             ci[x][y] = dy64_d4{0.0, 0.0, 0.0, 0.0};
+            if constexpr (G3) c3[x][y] = dy64_d4{0.0, 0.0, 0.0, 0.0};
         }
 
     load(0);
@@ -89,9 +133,19 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
     __syncthreads();
     for (int kc = 0; kc < NK; kc++) {
         const int buf = kc & 1;
-        if (kc + 1 < NK) load(kc + 1);
+        if (kc + 1 < NK) {
+            if constexpr (G3) load_a(kc + 1);
+            else load(kc + 1);
+        }
 #pragma unroll
         for (int ks = 0; ks < 4; ks++) {
+            if constexpr (G3) {
+                if (ks == 2 && kc + 1 < NK) {
+                    store_a(buf ^ 1);
+                    load_b(kc + 1);
+                }
+                __builtin_amdgcn_sched_barrier(0);  // one k-step's fragments live at a time
+            }
             const int kr = 4 * ks + lg;
             double ar[XR], ai[XR], nai[XR], br[YC], bi[YC];
 #pragma unroll
@@ -100,6 +154,24 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
                 ar[x] = rt < T ? sh.a[buf][0][kr][16 * rt + li] : 0.0;
                 ai[x] = rt < T ? sh.a[buf][1][kr][16 * rt + li] : 0.0;
                 nai[x] = -ai[x];
+            }
+            if constexpr (G3) {
+                // one column tile's operands at a time
+#pragma unroll
+                for (int y = 0; y < YC; y++) {
+                    const int ct = ctile(y);
+                    if (ct >= T) continue;  // uniform per wavefront
+                    const double b0 = sh.b[buf][0][kr][16 * ct + li], b1 = sh.b[buf][1][kr][16 * ct + li];
+#pragma unroll
+                    for (int x = 0; x < XR; x++) {
+                        if (rtile(x) >= T) continue;
+                        cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], b0, cr[x][y], 0, 0, 0);
+                        ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], b1, ci[x][y], 0, 0, 0);
+                        c3[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x] + ai[x], b0 + b1, c3[x][y], 0, 0, 0);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                continue;
             }
 #pragma unroll
             for (int y = 0; y < YC; y++) {
@@ -118,7 +190,10 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
                     ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], ci[x][y], 0, 0, 0);
                 }
         }
-        if (kc + 1 < NK) store(kc + 1, buf ^ 1);
+        if (kc + 1 < NK) {
+            if constexpr (G3) store_b(kc + 1, buf ^ 1);
+            else store(kc + 1, buf ^ 1);
+        }
         __syncthreads();
     }
     double2 *C = out + base;
@@ -129,8 +204,14 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
             const int rt = rtile(x), ct = ctile(y);
             if (rt >= T || ct >= T) continue;
 #pragma unroll
-            for (int v = 0; v < 4; v++)
-                C[(size_t)(16 * rt + 4 * v + lg) * NP + 16 * ct + li] = cmk(-cr[x][y][v], -ci[x][y][v]);
+            for (int v = 0; v < 4; v++) {
+                double re = cr[x][y][v], im = ci[x][y][v];
+                if constexpr (G3) {
+                    re = cr[x][y][v] - ci[x][y][v];
+                    im = c3[x][y][v] - cr[x][y][v] - ci[x][y][v];
+                }
+                C[(size_t)(16 * rt + 4 * v + lg) * NP + 16 * ct + li] = cmk(-re, -im);
+            }
         }
 }
 
@@ -138,23 +219,22 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
 
 bool rg_launch_gemm_wg64(rg_plan *P, int nb, const double2 *gss, const double2 *gs1_old, double2 *gs1_new)
 {
+    const bool g3 = P->dyson64 == 2;
     switch (P->np) {
-        case 80:
-            hipLaunchKernelGGL(k_dyson_gemm_wg64<80>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b,
-                               P->nmu, P->cstride);
-            return true;
-        case 96:
-            hipLaunchKernelGGL(k_dyson_gemm_wg64<96>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b,
-                               P->nmu, P->cstride);
-            return true;
-        case 112:
-            hipLaunchKernelGGL(k_dyson_gemm_wg64<112>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b,
-                               P->nmu, P->cstride);
-            return true;
-        case 128:
-            hipLaunchKernelGGL(k_dyson_gemm_wg64<128>, dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b,
-                               P->nmu, P->cstride);
-            return true;
+#define RG_WG64(N)                                                                                                 \
+    case N:                                                                                                        \
+        if (g3)                                                                                                    \
+            hipLaunchKernelGGL((k_dyson_gemm_wg64<N, true>), dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, \
+                               P->d_b, P->nmu, P->cstride);                                                        \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_dyson_gemm_wg64<N, false>), dim3(nb), dim3(512), 0, P->stream, gss, gs1_old,        \
+                               gs1_new, P->d_b, P->nmu, P->cstride);                                               \
+        return true;
+        RG_WG64(80)
+        RG_WG64(96)
+        RG_WG64(112)
+        RG_WG64(128)
+#undef RG_WG64
         default: return false;
     }
 }

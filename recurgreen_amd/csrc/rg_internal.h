@@ -226,8 +226,8 @@ struct rg_plan {
     // Concurrent lanes: the batch split in halves, each swept on a stream of its own by a view of
     // this plan (same buffers, re-based pointers).  lanes = 1: one stream.
     int lanes = 1;
-    // complex128 Dyson product: 1 one workgroup per realization at NP 80..128 (default), 0 the
-    // 64 x 64-tiled kernel (option dyson_fp64)
+    // complex128 Dyson product: 1 one workgroup per realization at NP 80..128 (default), 2 the same
+    // with three real products per complex product, 0 the 64 x 64-tiled kernel (option dyson_fp64)
     int dyson64 = 1;
     rg_plan *lane[2] = {nullptr, nullptr};
     hipEvent_t lane_ev[3] = {nullptr, nullptr, nullptr};

@@ -1705,7 +1705,7 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
         if (value < 0 || value > 3) return RG_EINVAL;
         P->gemm_wg = value;
     } else if (std::strcmp(key, "dyson_fp64") == 0) {
-        if (value != 0 && value != 1) return RG_EINVAL;
+        if (value < 0 || value > 2) return RG_EINVAL;
         P->dyson64 = value;
     } else if (std::strcmp(key, "project_parts") == 0) {
         if (value < 1 || value > kMaxCqParts) return RG_EINVAL;
