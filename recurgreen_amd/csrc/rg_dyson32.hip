@@ -141,51 +141,59 @@ __global__ __launch_bounds__(512, 4) void k_dyson_gemm_wg(const float2 *__restri
 // operands.  The error of Im AB is bounded by eps32 (|P1| + |P2| + |P3|) sums, of the size of the
 // four-product form's for operands of random phase.  The three accumulators of a wavefront's 2 x 4
 // output tiles take 96 registers, so a workgroup of 8 wavefronts runs at two wavefronts per SIMD, one
-// workgroup per CU: the kernel is persistent (realizations blockIdx.x + k gridDim.x), and the chunk
-// after the current one -- the next realization's first after a realization's last -- is loaded into
-// registers while the current one is multiplied, as in the BF16 kernel.  k runs in chunks of 32
-// through a double-buffered LDS stage ([row][k] for A, [col][k] for B, row length 36 floats); k-step
-// s of a chunk takes k = 8 g + s in lane group g, the same relabelling for both operands, so each lane
-// reads its eight k values of a row (or column) as two 16-byte words.
-template <int NP>
+// workgroup per CU: the kernel is persistent, and the chunk after the current one -- the next work
+// item's first after an item's last -- is loaded into registers while the current one is multiplied,
+// as in the BF16 kernel.  A work item is an MB x MB block of one realization's output (MB = NP up to
+// 128; the four 128 x 128 quadrants at NP = 256, so that each operand element is read twice instead
+// of four times by 64 x 64 tiles); items blockIdx.x + k gridDim.x.  k runs in chunks of 32 through a
+// double-buffered LDS stage ([row][k] for A, [col][k] for B, row length 36 floats); k-step s of a
+// chunk takes k = 8 g + s in lane group g, the same relabelling for both operands, so each lane reads
+// its eight k values of a row (or column) as two 16-byte words.
+template <int MB>
 struct DysonShared3m {
     static constexpr int S = 36;
-    alignas(16) float a[2][2][NP][S];  // [buffer][re, im][row][k]
-    alignas(16) float b[2][2][NP][S];  // [buffer][re, im][col][k]
+    alignas(16) float a[2][2][MB][S];  // [buffer][re, im][row][k]
+    alignas(16) float b[2][2][MB][S];  // [buffer][re, im][col][k]
 };
 
-template <int NP>
+template <int NP, int MB>
 __global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restrict__ gss, const float2 *__restrict__ gs1,
                                                           float2 *__restrict__ out, const double2 *__restrict__ bvec,
                                                           int n, int cs, int nb)
 {
-    static_assert(NP % 32 == 0, "k chunks of 32");
-    constexpr int T = NP / 16, NK = NP / 32, NQ = NP * 16;  // tiles, k chunks, float4 per operand chunk
+    static_assert(NP % 32 == 0 && MB % 32 == 0 && NP % MB == 0 && MB <= 128, "k chunks of 32, 8 x 8 tiles at most");
+    constexpr int T = MB / 16, NK = NP / 32, NQ = MB * 16;  // tiles, k chunks, float4 per operand chunk
+    constexpr int QD = NP / MB, QI = QD * QD;               // blocks per dimension and per realization
     constexpr int LOADS = NQ / 512;
-    __shared__ DysonShared3m<NP> sh;
+    __shared__ DysonShared3m<MB> sh;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1, li = lane & 15, lg = lane >> 4;
-    if ((int)blockIdx.x >= nb) return;  // the grid never exceeds the batch; uniform per workgroup
+    const int items = nb * QI;
+    if ((int)blockIdx.x >= items) return;  // the grid never exceeds the work items; uniform per workgroup
 
-    // chunk kc of A: row idx >> 4, complex pair idx & 15 of the row's 32 k.  Of B: the 64 loads of
-    // wave slot s = idx >> 6 cover k rows 8 (s & 3) + (lane & 7) and column pairs 8 (s >> 2) + (lane >> 3).
-    // The next chunk's A and B are staged one after the other (A loaded before the first half of the
-    // products and stored after it, then B), so that only one operand's registers are in flight.
+    // Item t: realization t / QI, output rows MB ((t % QI) / QD) .., columns MB ((t % QI) % QD) ...
+    // Chunk kc of A: row idx >> 4 of the block, complex pair idx & 15 of the row's 32 k.  Of B: the 64
+    // loads of wave slot s = idx >> 6 cover k rows 8 (s & 3) + (lane & 7) and column pairs
+    // 8 (s >> 2) + (lane >> 3) of the block.  The next chunk's A and B are staged one after the other (A
+    // loaded before the first half of the products and stored after it, then B), so that only one
+    // operand's registers are in flight.
     float4 ra[LOADS], rb[LOADS];
-    auto load_a = [&](int r, int kc) __attribute__((always_inline)) {
+    auto load_a = [&](int t, int kc) __attribute__((always_inline)) {
+        const int r = t / QI, row0 = MB * ((t % QI) / QD);
         const float4 *A4 = reinterpret_cast<const float4 *>(gss + (size_t)r * NP * NP);
 #pragma unroll
         for (int e = 0; e < LOADS; e++) {
             const int idx = tid + 512 * e;
-            ra[e] = A4[(size_t)(idx >> 4) * (NP / 2) + kc * 16 + (idx & 15)];
+            ra[e] = A4[(size_t)(row0 + (idx >> 4)) * (NP / 2) + kc * 16 + (idx & 15)];
         }
     };
-    auto load_b = [&](int r, int kc) __attribute__((always_inline)) {
+    auto load_b = [&](int t, int kc) __attribute__((always_inline)) {
+        const int r = t / QI, col0 = MB * ((t % QI) % QD);
         const float4 *B4 = reinterpret_cast<const float4 *>(gs1 + (size_t)r * NP * NP);
 #pragma unroll
         for (int e = 0; e < LOADS; e++) {
             const int idx = tid + 512 * e, s = idx >> 6, l = idx & 63;
-            rb[e] = B4[(size_t)(kc * 32 + 8 * (s & 3) + (l & 7)) * (NP / 2) + 8 * (s >> 2) + (l >> 3)];
+            rb[e] = B4[(size_t)(kc * 32 + 8 * (s & 3) + (l & 7)) * (NP / 2) + col0 / 2 + 8 * (s >> 2) + (l >> 3)];
         }
     };
     auto store_a = [&](int buf) __attribute__((always_inline)) {
@@ -197,8 +205,8 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restri
             *reinterpret_cast<float2 *>(&sh.a[buf][1][row][k2]) = make_float2(ra[e].y, ra[e].w);
         }
     };
-    auto store_b = [&](int r, int kc, int buf) __attribute__((always_inline)) {
-        const double2 *bv = bvec + (size_t)r * cs;  // cs = np: constants of each realization, else shared
+    auto store_b = [&](int t, int kc, int buf) __attribute__((always_inline)) {
+        const double2 *bv = bvec + (size_t)(t / QI) * cs;  // cs = np: constants of each realization, else shared
 #pragma unroll
         for (int e = 0; e < LOADS; e++) {
             const int idx = tid + 512 * e, s = idx >> 6, l = idx & 63;
@@ -211,7 +219,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restri
         }
     };
     // the eight k values 8 lg .. 8 lg + 7 of row (or column) 16 tile + li of a plane
-    auto frag = [&](const float (*plane)[DysonShared3m<NP>::S], int tile, float (&v)[8]) __attribute__((always_inline)) {
+    auto frag = [&](const float (*plane)[DysonShared3m<MB>::S], int tile, float (&v)[8]) __attribute__((always_inline)) {
         const float4 *p = reinterpret_cast<const float4 *>(&plane[16 * tile + li][8 * lg]);
         const float4 x0 = p[0], x1 = p[1];
         v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w;
@@ -226,7 +234,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restri
     store_a(0);
     store_b(blockIdx.x, 0, 0);
     __syncthreads();
-    for (int r = blockIdx.x; r < nb; r += stride) {
+    for (int t = blockIdx.x; t < items; t += stride) {
 #pragma unroll
         for (int x = 0; x < 2; x++)
 #pragma unroll
@@ -237,10 +245,10 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restri
             }
 #pragma nounroll
         for (int kc = 0; kc < NK; kc++) {
-            // the chunk after this one: (r, kc + 1), or the first of the next realization
-            const int nr = kc + 1 < NK ? r : r + stride, nk = kc + 1 < NK ? kc + 1 : 0;
-            const bool more = nr < nb;
-            if (more) load_a(nr, nk);
+            // the chunk after this one: (t, kc + 1), or the first of the next item
+            const int nt = kc + 1 < NK ? t : t + stride, nk = kc + 1 < NK ? kc + 1 : 0;
+            const bool more = nt < items;
+            if (more) load_a(nt, nk);
             float ar[2][8], ai[2][8];
 #pragma unroll
             for (int x = 0; x < 2; x++) {
@@ -271,13 +279,14 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restri
                 __builtin_amdgcn_sched_barrier(0);
                 if (y == 1 && more) {  // the other buffer was last read before the previous barrier
                     store_a(buf ^ 1);
-                    load_b(nr, nk);
+                    load_b(nt, nk);
                 }
             }
-            if (more) store_b(nr, nk, buf ^ 1);
+            if (more) store_b(nt, nk, buf ^ 1);
             __syncthreads();
             buf ^= 1;
         }
+        const int r = t / QI, row0 = MB * ((t % QI) / QD), col0 = MB * ((t % QI) % QD);
         float2 *C = out + (size_t)r * NP * NP;
 #pragma unroll
         for (int x = 0; x < 2; x++)
@@ -288,7 +297,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restri
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const float re = p1[x][y][v] - p2[x][y][v], im = p3[x][y][v] - p1[x][y][v] - p2[x][y][v];
-                    C[(size_t)(16 * rt + 4 * lg + v) * NP + 16 * ct + li] = make_float2(-re, -im);
+                    C[(size_t)(row0 + 16 * rt + 4 * lg + v) * NP + col0 + 16 * ct + li] = make_float2(-re, -im);
                 }
             }
     }
@@ -315,19 +324,23 @@ bool rg_launch_gemm_wg(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_
     }
 }
 
-// Option dyson = 3: NP 96 and 128 (k chunks of 32); false otherwise.
+// Option dyson = 3: NP 96, 128 (the whole matrix per work item), 192 and 256 (four blocks of 96 or 128
+// per realization); false otherwise.
 bool rg_launch_gemm_3m(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new)
 {
-    const int grid = nb < P->ncu ? (nb < 1 ? 1 : nb) : P->ncu;
+    const int qi = P->np > 128 ? 4 : 1, items = nb * qi;
+    const int grid = items < P->ncu ? (items < 1 ? 1 : items) : P->ncu;
     switch (P->np) {
-        case 96:
-            hipLaunchKernelGGL(k_dyson_gemm_3m<96>, dim3(grid), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b,
-                               P->nmu, P->cstride, nb);
-            return true;
-        case 128:
-            hipLaunchKernelGGL(k_dyson_gemm_3m<128>, dim3(grid), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, P->d_b,
-                               P->nmu, P->cstride, nb);
-            return true;
+#define RG_3M(N, MB)                                                                                           \
+    case N:                                                                                                    \
+        hipLaunchKernelGGL((k_dyson_gemm_3m<N, MB>), dim3(grid), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, \
+                           P->d_b, P->nmu, P->cstride, nb);                                                    \
+        return true;
+        RG_3M(96, 96)
+        RG_3M(128, 128)
+        RG_3M(192, 96)
+        RG_3M(256, 128)
+#undef RG_3M
         default: return false;
     }
 }

@@ -29,15 +29,22 @@ struct DysonShared64 {
 // G3: Gauss's form of the complex product, P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi),
 // Re = P1 - P2, Im = P3 - P1 - P2: three real products on three accumulators instead of four
 // products on two (the error of Im is bounded by eps (|P1| + |P2| + |P3|) sums).
-template <int NP, bool G3>
+// A workgroup computes an MB x MB block of one realization's product: the whole of it (MB = NP) up to
+// NP = 128, one of the four 128 x 128 quadrants at NP = 256 (workgroup 4 r + q: rows 128 (q / 2) ..,
+// columns 128 (q % 2) ..), so that each operand element is read twice instead of four times by 64 x 64
+// tiles.
+template <int NP, int MB, bool G3>
 __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__restrict__ gss,
                                                             const double2 *__restrict__ gs1, double2 *__restrict__ out,
                                                             const double2 *__restrict__ bvec, int n, int cs)
 {
-    bvec += (size_t)blockIdx.x * cs;  // cs = np: constants of each realization, else shared
-    constexpr int T = NP / 16, NK = NP / 16, NQ = NP * 16;  // tiles, k chunks, complex per operand chunk
+    static_assert(NP % MB == 0 && MB % 16 == 0 && MB <= 128, "8 x 8 tiles at most");
+    constexpr int QD = NP / MB, QI = QD * QD;                // blocks per dimension and per realization
+    constexpr int T = MB / 16, NK = NP / 16, NQ = MB * 16;  // tiles, k chunks, complex per operand chunk
     constexpr int LOADS = (NQ + 511) / 512;
-    __shared__ DysonShared64<NP> sh;
+    const int rz = blockIdx.x / QI, row0 = MB * ((blockIdx.x % QI) / QD), col0 = MB * ((blockIdx.x % QI) % QD);
+    bvec += (size_t)rz * cs;  // cs = np: constants of each realization, else shared
+    __shared__ DysonShared64<MB> sh;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1, li = lane & 15, lg = lane >> 4;
     // ownership of the T x T output tiles: 2 row x 4 column tiles per wavefront, or at T = 7 (49
@@ -47,8 +54,8 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
     constexpr int XR = ROW1 ? 1 : 2, YC = ROW1 ? 7 : 4;
     auto rtile = [&](int x) { return ROW1 ? w : wr + 4 * x; };
     auto ctile = [&](int y) { return ROW1 ? y : wc + 2 * y; };
-    const size_t base = (size_t)blockIdx.x * NP * NP;
-    const double2 *A = gss + base, *B = gs1 + base;
+    const size_t base = (size_t)rz * NP * NP;
+    const double2 *A = gss + base + (size_t)row0 * NP, *B = gs1 + base + col0;
 
     double2 ra[LOADS], rb[LOADS];
     // chunk kc: A row idx >> 4, k idx & 15 (16 k of a row contiguous); B k row idx / NP, column idx % NP
@@ -58,7 +65,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
             const int idx = tid + 512 * e;
             if (idx < NQ) {
                 ra[e] = A[(size_t)(idx >> 4) * NP + kc * 16 + (idx & 15)];
-                rb[e] = B[(size_t)(kc * 16 + idx / NP) * NP + idx % NP];
+                rb[e] = B[(size_t)(kc * 16 + idx / MB) * NP + idx % MB];
             }
         }
     };
@@ -70,7 +77,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
                 const int row = idx >> 4, k = idx & 15;
                 sh.a[buf][0][k][row] = ra[e].x;
                 sh.a[buf][1][k][row] = ra[e].y;
-                const int kk = idx / NP, col = idx % NP, gk = kc * 16 + kk;
+                const int kk = idx / MB, col = idx % MB, gk = kc * 16 + kk;
                 const double2 bk = gk < n ? bvec[gk] : cmk(0.0, 0.0);
                 sh.b[buf][0][kk][col] = bk.x * rb[e].x - bk.y * rb[e].y;
                 sh.b[buf][1][kk][col] = bk.x * rb[e].y + bk.y * rb[e].x;
@@ -91,7 +98,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
 #pragma unroll
         for (int e = 0; e < LOADS; e++) {
             const int idx = tid + 512 * e;
-            if (idx < NQ) rb[e] = B[(size_t)(kc * 16 + idx / NP) * NP + idx % NP];
+            if (idx < NQ) rb[e] = B[(size_t)(kc * 16 + idx / MB) * NP + idx % MB];
         }
     };
     auto store_a = [&](int buf) __attribute__((always_inline)) {
@@ -109,7 +116,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
         for (int e = 0; e < LOADS; e++) {
             const int idx = tid + 512 * e;
             if (idx < NQ) {
-                const int kk = idx / NP, col = idx % NP, gk = kc * 16 + kk;
+                const int kk = idx / MB, col = idx % MB, gk = kc * 16 + kk;
                 const double2 bk = gk < n ? bvec[gk] : cmk(0.0, 0.0);
                 sh.b[buf][0][kk][col] = bk.x * rb[e].x - bk.y * rb[e].y;
                 sh.b[buf][1][kk][col] = bk.x * rb[e].y + bk.y * rb[e].x;
@@ -196,7 +203,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
         }
         __syncthreads();
     }
-    double2 *C = out + base;
+    double2 *C = out + base + (size_t)row0 * NP + col0;
 #pragma unroll
     for (int x = 0; x < XR; x++)
 #pragma unroll
@@ -221,19 +228,21 @@ bool rg_launch_gemm_wg64(rg_plan *P, int nb, const double2 *gss, const double2 *
 {
     const bool g3 = P->dyson64 == 2;
     switch (P->np) {
-#define RG_WG64(N)                                                                                                 \
+#define RG_WG64(N, MB)                                                                                             \
     case N:                                                                                                        \
         if (g3)                                                                                                    \
-            hipLaunchKernelGGL((k_dyson_gemm_wg64<N, true>), dim3(nb), dim3(512), 0, P->stream, gss, gs1_old, gs1_new, \
-                               P->d_b, P->nmu, P->cstride);                                                        \
+            hipLaunchKernelGGL((k_dyson_gemm_wg64<N, MB, true>), dim3(nb * (N / MB) * (N / MB)), dim3(512), 0,       \
+                               P->stream, gss, gs1_old, gs1_new, P->d_b, P->nmu, P->cstride);                      \
         else                                                                                                       \
-            hipLaunchKernelGGL((k_dyson_gemm_wg64<N, false>), dim3(nb), dim3(512), 0, P->stream, gss, gs1_old,        \
-                               gs1_new, P->d_b, P->nmu, P->cstride);                                               \
+            hipLaunchKernelGGL((k_dyson_gemm_wg64<N, MB, false>), dim3(nb * (N / MB) * (N / MB)), dim3(512), 0,      \
+                               P->stream, gss, gs1_old, gs1_new, P->d_b, P->nmu, P->cstride);                      \
         return true;
-        RG_WG64(80)
-        RG_WG64(96)
-        RG_WG64(112)
-        RG_WG64(128)
+        RG_WG64(80, 80)
+        RG_WG64(96, 96)
+        RG_WG64(112, 112)
+        RG_WG64(128, 128)
+        RG_WG64(192, 96)
+        RG_WG64(256, 128)
 #undef RG_WG64
         default: return false;
     }
