@@ -9,6 +9,16 @@
 // S) and six products on k_cgemm, the same 8 np^3 flops as a direct inversion.  The blocks live in
 // a per-realization work area; the result is assembled into Gss.  Both precisions: complex128
 // (FP64 MFMA) and complex64 (FP32 MFMA).
+//
+// Complex symmetric slice matrices (Dirichlet guides, complex128 with the symmetric inversion kernel):
+// C = B^T, so Y = C A^-1 = X^T and S^-1 Y = (X S^-1)^T, and A and S are symmetric.  Then
+//   A^-1 and S^-1 on the symmetric kernel (lower triangles), X = A^-1 B, S = D - C X, Z = -X S^-1,
+//   M^-1 = [[A^-1 - Z X^T, Z], [Z^T, S^-1]]:
+// four products instead of six.  The top-left block is written symmetrized, (T + T^T)/2, and the
+// bottom-left as the exact transpose of the top-right, so that Gss stays exactly symmetric (an
+// asymmetric rounding component of Gss is not damped by the recursion on absorbing guides).
+#include <type_traits>
+
 #include "rg_internal.h"
 
 namespace {
@@ -17,7 +27,8 @@ namespace {
 // lda/ldb/ldc, realization strides sA/sB/sC in elements).  64 x 64 output tile per workgroup,
 // 32 x 32 per wavefront, k in chunks of 16 staged through LDS as re/im planes.  beta == 0 does not
 // read C.
-template <class E>
+// TB: B is read transposed, B(kk, j) = B[j ldb + kk].
+template <class E, bool TB>
 __global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, double2 alpha, const E *__restrict__ A, int lda,
                                                size_t sA, const E *__restrict__ B, int ldb, size_t sB, double2 beta,
                                                E *__restrict__ C, int ldc, size_t sC)
@@ -50,7 +61,17 @@ __global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, double2 alph
                 AsI[kq + e][rr] = v.y;
             }
         }
-        {
+        if constexpr (TB) {  // 16 consecutive k of one column per 16 threads
+            const int kk = tid & 15, cq = (tid >> 4) * 4, gk = k0 + kk;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int gc = tj0 + cq + e;
+                E v{0, 0};
+                if (gk < k && gc < n) v = Bb[(size_t)gc * ldb + gk];
+                BsR[kk][cq + e] = v.x;
+                BsI[kk][cq + e] = v.y;
+            }
+        } else {
             const int kk = tid >> 4, cq = (tid & 15) * 4, gk = k0 + kk;
 #pragma unroll
             for (int e = 0; e < 4; e++) {
@@ -201,12 +222,37 @@ __global__ __launch_bounds__(256) void k_copy_block(const E *__restrict__ src, i
         d[(size_t)(e / n) * ldd + e % n] = s[(size_t)(e / n) * lds + e % n];
 }
 
-template <class E>
+template <class E, bool TB = false>
 void cgemm(rg_plan *P, int nb, int m, int n, int k, double2 alpha, const E *A, int lda, size_t sA, const E *B, int ldb,
            size_t sB, double2 beta, E *C, int ldc, size_t sC)
 {
-    hipLaunchKernelGGL(k_cgemm<E>, dim3((n + 63) / 64, (m + 63) / 64, nb), dim3(256), 0, P->stream, m, n, k, alpha, A, lda,
-                       sA, B, ldb, sB, beta, C, ldc, sC);
+    hipLaunchKernelGGL((k_cgemm<E, TB>), dim3((n + 63) / 64, (m + 63) / 64, nb), dim3(256), 0, P->stream, m, n, k, alpha, A,
+                       lda, sA, B, ldb, sB, beta, C, ldc, sC);
+}
+
+// dst (n x m, ldd) = src (m x n, lds)^T per realization.
+template <class E>
+__global__ __launch_bounds__(256) void k_copy_block_t(const E *__restrict__ src, int lds, size_t ss, E *__restrict__ dst,
+                                                      int ldd, size_t sd, int m, int n)
+{
+    const E *s = src + (size_t)blockIdx.y * ss;
+    E *d = dst + (size_t)blockIdx.y * sd;
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < m * n; e += gridDim.x * 256)
+        d[(size_t)(e % n) * ldd + e / n] = s[(size_t)(e / n) * lds + e % n];
+}
+
+// dst (m x m, ldd) = (src + src^T) / 2 per realization (src m x m, lds).
+template <class E>
+__global__ __launch_bounds__(256) void k_copy_block_sym(const E *__restrict__ src, int lds, size_t ss,
+                                                        E *__restrict__ dst, int ldd, size_t sd, int m)
+{
+    const E *s = src + (size_t)blockIdx.y * ss;
+    E *d = dst + (size_t)blockIdx.y * sd;
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < m * m; e += gridDim.x * 256) {
+        const int i = e / m, j = e % m;
+        const E u = s[(size_t)i * lds + j], v = s[(size_t)j * lds + i];
+        d[(size_t)i * ldd + j] = rg_mt<E>::make(0.5 * ((double)u.x + (double)v.x), 0.5 * ((double)u.y + (double)v.y));
+    }
 }
 
 template <class E>
@@ -252,6 +298,25 @@ void invert_block(const rg_plan *P, int nb, int no, E *mats, int precision)
     }
 }
 
+// Inverts, in place, symmetric complex128 blocks of order `no` (their lower triangles) on the symmetric
+// kernel through a view of the plan as invert_block does; flagged realizations go to the pivoted kernel.
+void invert_block_sym(const rg_plan *P, int nb, int no, double2 *mats)
+{
+    rg_plan V(*P);
+    V.graphs.clear();
+    for (auto &t : V.timers) { t.pending.clear(); t.pool.clear(); }
+    V.profiling = false;
+    V.np = no;
+    V.nmu = no;
+    V.cstride = 0;
+    V.d_nx_r = nullptr;
+    V.d_adiag = P->d_schur_zero;
+    V.d_b = P->d_schur_i;
+    V.d_gss = mats;
+    rg_launch_inverse_sym64(&V, nb, 0, 0, false, V.d_redo);
+    rg_launch_inverse_gj(&V, nb, 0, 0, false, RG_FP64, V.d_redo);
+}
+
 template <class E>
 void inverse_schur(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision, E *gss)
 {
@@ -262,6 +327,22 @@ void inverse_schur(rg_plan *P, int nb, int slot, int first, bool with_disorder, 
                        P->d_adiag + (size_t)slot * np, P->d_b, P->nmu, np, first, w, P->cstride, P->d_nx_r, P->cur_ix);
     const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
     const size_t gs = (size_t)np * np;
+    if constexpr (std::is_same<E, double2>::value) {
+        if (P->p.boundtype == RG_DIRICHLET && P->inv_fp64 == RG_INV_SYM && n2 >= 32) {
+            invert_block_sym(P, nb, n1, w.a);                                                      // A^-1
+            cgemm(P, nb, n1, n2, n1, one, w.a, n1, w.sa, w.b, n2, w.sb, zero, w.x, n2, w.sb);       // X = A^-1 B
+            cgemm(P, nb, n2, n2, n1, mone, w.c, n1, w.sb, w.x, n2, w.sb, one, w.d, n2, w.sd);       // S = D - C X
+            invert_block_sym(P, nb, n2, w.d);                                                      // S^-1
+            cgemm(P, nb, n1, n2, n2, mone, w.x, n2, w.sb, w.d, n2, w.sd, zero, gss + n1, np, gs);   // Z = -X S^-1
+            cgemm<E, true>(P, nb, n1, n1, n2, mone, gss + n1, np, gs, w.x, n2, w.sb, one, w.a, n1, w.sa);  // A^-1 - Z X^T
+            hipLaunchKernelGGL(k_copy_block_sym<E>, dim3((n1 * n1 + 255) / 256, nb), dim3(256), 0, P->stream, w.a, n1,
+                               w.sa, gss, np, gs, n1);
+            hipLaunchKernelGGL(k_copy_block_t<E>, dim3((n1 * n2 + 255) / 256, nb), dim3(256), 0, P->stream, gss + n1, np,
+                               gs, gss + (size_t)n1 * np, np, gs, n1, n2);                          // Z^T
+            copy_block(P, nb, w.d, n2, w.sd, gss + (size_t)n1 * np + n1, np, gs, n2, n2);
+            return;
+        }
+    }
     invert_block(P, nb, n1, w.a, precision);                                                    // A^-1
     cgemm(P, nb, n1, n2, n1, one, w.a, n1, w.sa, w.b, n2, w.sb, zero, w.x, n2, w.sb);           // X = A^-1 B
     cgemm(P, nb, n2, n2, n1, mone, w.c, n1, w.sb, w.x, n2, w.sb, one, w.d, n2, w.sd);           // S = D - C X
