@@ -52,13 +52,19 @@ def workload_params(args):
 def kernel_flops(plan, params, kernel, prec) -> float:
     """Algorithmic real flops per realization of one launch of `kernel` at N modes: 8 N^3 for the Dyson
     product and the general inversion (zgetrf + zgetri); for the symmetric inversion of Dirichlet
-    guides (N = 32..128) the lower-triangle tiles of 16: 4 N^2 (N + 16) (sweep-operator updates,
-    panel products and the diagonal-block inversions)."""
+    guides the lower-triangle tiles of 16: 4 N^2 (N + 16) (sweep-operator updates, panel products and the
+    diagonal-block inversions), directly up to N = 128 and for complex64 sweeps up to 256; complex128
+    sweeps above 128 modes invert through the Schur complement, whose symmetric form (blocks n1 = 128
+    and n2 = N - 128) does two symmetric block inversions and four products, 16 n1 n2 (n1 + n2)."""
     n = plan.nmu
     npad = 16 * ((n + 15) // 16)  # the kernels' padded order
-    if kernel == "slice_inverse" and params.get("boundtype", "dirichlet") == "dirichlet" and 32 <= npad <= 128:
+    if kernel == "slice_inverse" and params.get("boundtype", "dirichlet") == "dirichlet" and 32 <= npad:
         if plan.get_option("inverse_fp64" if prec == "fp64" else "inverse_mixed") == 4:
-            return 4.0 * n * n * (n + 16)
+            if npad <= 128 or prec != "fp64":
+                return 4.0 * n * n * (n + 16)
+            n1, n2 = 128, n - 128
+            if n2 >= 32:
+                return 4.0 * n1 * n1 * (n1 + 16) + 4.0 * n2 * n2 * (n2 + 16) + 16.0 * n1 * n2 * (n1 + n2)
     return 8.0 * n ** 3
 
 

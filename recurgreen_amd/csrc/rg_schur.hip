@@ -230,28 +230,42 @@ void cgemm(rg_plan *P, int nb, int m, int n, int k, double2 alpha, const E *A, i
                        lda, sA, B, ldb, sB, beta, C, ldc, sC);
 }
 
-// dst (n x m, ldd) = src (m x n, lds)^T per realization.
+// dst (n x m, ldd) = src (m x n, lds)^T per realization, through 32 x 32 tiles in LDS (coalesced reads and
+// writes); grid (ceil(n / 32), ceil(m / 32), realizations).
 template <class E>
 __global__ __launch_bounds__(256) void k_copy_block_t(const E *__restrict__ src, int lds, size_t ss, E *__restrict__ dst,
                                                       int ldd, size_t sd, int m, int n)
 {
-    const E *s = src + (size_t)blockIdx.y * ss;
-    E *d = dst + (size_t)blockIdx.y * sd;
-    for (int e = blockIdx.x * 256 + threadIdx.x; e < m * n; e += gridDim.x * 256)
-        d[(size_t)(e % n) * ldd + e / n] = s[(size_t)(e / n) * lds + e % n];
+    __shared__ E t[32][33];
+    const E *s = src + (size_t)blockIdx.z * ss;
+    E *d = dst + (size_t)blockIdx.z * sd;
+    const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8)
+        if (i0 + r < m && j0 + tx < n) t[r][tx] = s[(size_t)(i0 + r) * lds + j0 + tx];
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8)
+        if (j0 + r < n && i0 + tx < m) d[(size_t)(j0 + r) * ldd + i0 + tx] = t[tx][r];
 }
 
-// dst (m x m, ldd) = (src + src^T) / 2 per realization (src m x m, lds).
+// dst (m x m, ldd) = (src + src^T) / 2 per realization (src m x m, lds), the transposed tile through LDS;
+// grid (ceil(m / 32), ceil(m / 32), realizations).
 template <class E>
 __global__ __launch_bounds__(256) void k_copy_block_sym(const E *__restrict__ src, int lds, size_t ss,
                                                         E *__restrict__ dst, int ldd, size_t sd, int m)
 {
-    const E *s = src + (size_t)blockIdx.y * ss;
-    E *d = dst + (size_t)blockIdx.y * sd;
-    for (int e = blockIdx.x * 256 + threadIdx.x; e < m * m; e += gridDim.x * 256) {
-        const int i = e / m, j = e % m;
-        const E u = s[(size_t)i * lds + j], v = s[(size_t)j * lds + i];
-        d[(size_t)i * ldd + j] = rg_mt<E>::make(0.5 * ((double)u.x + (double)v.x), 0.5 * ((double)u.y + (double)v.y));
+    __shared__ E t[32][33];
+    const E *s = src + (size_t)blockIdx.z * ss;
+    E *d = dst + (size_t)blockIdx.z * sd;
+    const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8)  // t[r][c] = src[j0 + r][i0 + c]
+        if (j0 + r < m && i0 + tx < m) t[r][tx] = s[(size_t)(j0 + r) * lds + i0 + tx];
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, j = j0 + tx;
+        if (i < m && j < m) {
+            const E u = s[(size_t)i * lds + j], v = t[tx][r];  // src[i][j], src[j][i]
+            d[(size_t)i * ldd + j] = rg_mt<E>::make(0.5 * ((double)u.x + (double)v.x), 0.5 * ((double)u.y + (double)v.y));
+        }
     }
 }
 
@@ -335,10 +349,10 @@ void inverse_schur(rg_plan *P, int nb, int slot, int first, bool with_disorder, 
             invert_block_sym(P, nb, n2, w.d);                                                      // S^-1
             cgemm(P, nb, n1, n2, n2, mone, w.x, n2, w.sb, w.d, n2, w.sd, zero, gss + n1, np, gs);   // Z = -X S^-1
             cgemm<E, true>(P, nb, n1, n1, n2, mone, gss + n1, np, gs, w.x, n2, w.sb, one, w.a, n1, w.sa);  // A^-1 - Z X^T
-            hipLaunchKernelGGL(k_copy_block_sym<E>, dim3((n1 * n1 + 255) / 256, nb), dim3(256), 0, P->stream, w.a, n1,
-                               w.sa, gss, np, gs, n1);
-            hipLaunchKernelGGL(k_copy_block_t<E>, dim3((n1 * n2 + 255) / 256, nb), dim3(256), 0, P->stream, gss + n1, np,
-                               gs, gss + (size_t)n1 * np, np, gs, n1, n2);                          // Z^T
+            hipLaunchKernelGGL(k_copy_block_sym<E>, dim3(n1 / 32, n1 / 32, nb), dim3(256), 0, P->stream, w.a, n1, w.sa,
+                               gss, np, gs, n1);
+            hipLaunchKernelGGL(k_copy_block_t<E>, dim3((n2 + 31) / 32, n1 / 32, nb), dim3(256), 0, P->stream, gss + n1,
+                               np, gs, gss + (size_t)n1 * np, np, gs, n1, n2);                      // Z^T
             copy_block(P, nb, w.d, n2, w.sd, gss + (size_t)n1 * np + n1, np, gs, n2, n2);
             return;
         }
