@@ -161,6 +161,107 @@ Blocks<E> blocks_of(const rg_plan *P)
     return w;
 }
 
+// C = alpha A op(B) + beta C for complex128 128 x 128 blocks, one workgroup of 8 wavefronts per
+// realization (the layout of the one-workgroup Dyson product): each operand element is read once, against
+// twice by the 64 x 64 tiles of k_cgemm.  Wavefront w owns row tiles {w/2, w/2 + 4} and column tiles
+// {w%2 + 2y}; k runs in chunks of 16 through a double-buffered LDS stage ([k][row] for A, [k][col] for
+// B, row length 130 doubles), the next chunk loaded into registers while the current one is multiplied.
+// TB: op(B) = B^T, element (k, j) at B[j ldb + k] (16 consecutive k of a column per 16 threads).
+template <bool TB>
+__global__ __launch_bounds__(512, 2) void k_zgemm_wg128(double2 alpha, const double2 *__restrict__ A, int lda, size_t sA,
+                                                        const double2 *__restrict__ B, int ldb, size_t sB, double2 beta,
+                                                        double2 *__restrict__ C, int ldc, size_t sC)
+{
+    constexpr int NP = 128, S = NP + 2, NK = NP / 16, LOADS = NP * 16 / 512;
+    __shared__ double sa[2][2][16][S], sb[2][2][16][S];  // [buffer][re, im][k][row or col]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1, li = lane & 15, lg = lane >> 4;
+    const double2 *Ab = A + (size_t)blockIdx.x * sA, *Bb = B + (size_t)blockIdx.x * sB;
+    double2 ra[LOADS], rb[LOADS];
+    auto load = [&](int kc) __attribute__((always_inline)) {
+#pragma unroll
+        for (int e = 0; e < LOADS; e++) {
+            const int idx = tid + 512 * e;
+            ra[e] = Ab[(size_t)(idx >> 4) * lda + kc * 16 + (idx & 15)];
+            if constexpr (TB) rb[e] = Bb[(size_t)(idx >> 4) * ldb + kc * 16 + (idx & 15)];
+            else rb[e] = Bb[(size_t)(kc * 16 + idx / NP) * ldb + idx % NP];
+        }
+    };
+    auto store = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int e = 0; e < LOADS; e++) {
+            const int idx = tid + 512 * e;
+            sa[buf][0][idx & 15][idx >> 4] = ra[e].x;
+            sa[buf][1][idx & 15][idx >> 4] = ra[e].y;
+            const int kk = TB ? (idx & 15) : idx / NP, col = TB ? (idx >> 4) : idx % NP;
+            sb[buf][0][kk][col] = rb[e].x;
+            sb[buf][1][kk][col] = rb[e].y;
+        }
+    };
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    d4 cr[2][4], ci[2][4];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            cr[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+            ci[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+        }
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kc = 0; kc < NK; kc++) {
+        const int buf = kc & 1;
+        if (kc + 1 < NK) load(kc + 1);
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            const int kr = 4 * ks + lg;
+            double ar[2], ai[2], nai[2], br[4], bi[4];
+#pragma unroll
+            for (int x = 0; x < 2; x++) {
+                ar[x] = sa[buf][0][kr][16 * (wr + 4 * x) + li];
+                ai[x] = sa[buf][1][kr][16 * (wr + 4 * x) + li];
+                nai[x] = -ai[x];
+            }
+#pragma unroll
+            for (int y = 0; y < 4; y++) {
+                br[y] = sb[buf][0][kr][16 * (wc + 2 * y) + li];
+                bi[y] = sb[buf][1][kr][16 * (wc + 2 * y) + li];
+            }
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++) {
+                    cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], cr[x][y], 0, 0, 0);
+                    cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[x], bi[y], cr[x][y], 0, 0, 0);
+                    ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], ci[x][y], 0, 0, 0);
+                    ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], ci[x][y], 0, 0, 0);
+                }
+        }
+        if (kc + 1 < NK) store(buf ^ 1);
+        __syncthreads();
+    }
+    double2 *Cb = C + (size_t)blockIdx.x * sC;
+    const bool rd = beta.x != 0.0 || beta.y != 0.0;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                // v_mfma_f64_16x16x4_f64: register v of lane l holds D[4v + l/16][l%16]
+                const size_t o = (size_t)(16 * (wr + 4 * x) + 4 * v + lg) * ldc + 16 * (wc + 2 * y) + li;
+                const double xr = cr[x][y][v], xi = ci[x][y][v];
+                double outr = alpha.x * xr - alpha.y * xi, outi = alpha.x * xi + alpha.y * xr;
+                if (rd) {
+                    const double2 c = Cb[o];
+                    outr += beta.x * c.x - beta.y * c.y;
+                    outi += beta.x * c.y + beta.y * c.x;
+                }
+                Cb[o] = cmk(outr, outi);
+            }
+}
+
 // M = (diag(a) - V) - B Gss B in FP64 (as in the register-resident inversions), rounded to E and
 // written as the blocks A, B, C, D; identity on the padding rows/columns >= n.
 template <class E>
@@ -226,6 +327,13 @@ template <class E, bool TB = false>
 void cgemm(rg_plan *P, int nb, int m, int n, int k, double2 alpha, const E *A, int lda, size_t sA, const E *B, int ldb,
            size_t sB, double2 beta, E *C, int ldc, size_t sC)
 {
+    if constexpr (std::is_same<E, double2>::value) {
+        if (m == 128 && n == 128 && k == 128) {  // complex128 blocks of order 256: one workgroup per realization
+            hipLaunchKernelGGL(k_zgemm_wg128<TB>, dim3(nb), dim3(512), 0, P->stream, alpha, A, lda, sA, B, ldb, sB, beta, C,
+                               ldc, sC);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_cgemm<E, TB>), dim3((n + 63) / 64, (m + 63) / 64, nb), dim3(256), 0, P->stream, m, n, k, alpha, A,
                        lda, sA, B, ldb, sB, beta, C, ldc, sC);
 }
