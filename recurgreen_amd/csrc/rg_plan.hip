@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <list>
 #include <map>
 #include <new>
 #include <thread>
@@ -1236,8 +1237,27 @@ rg_status rg_tdistrib_uneven(const rg_params *params, const int64_t *seeds, int3
     // group whose launches do not fill the device (few or short realizations, or the tail of its sweep
     // where only the longest remain) leaves room to the others.  Every realization meets the same kernels
     // and data as in a group of its own, so the results do not depend on this schedule.
+    // A large group above 128 modes (the most costly per slice) is swept as two halves on two streams,
+    // its realizations dealt alternately by length: the halves fill each other's partly empty last
+    // rounds of workgroups and the gaps between the short kernels of a slice step (c5 distribution,
+    // 10 000 realizations: FP64 14.3 -> 13.5 s, mixed 6.1 -> 6.0 s; three or four parts gained nothing
+    // more).
     std::vector<std::vector<uneven_real> *> gl;
-    for (auto &kv : groups) gl.push_back(&kv.second);
+    std::list<std::vector<uneven_real>> halves;
+    for (auto &kv : groups) {
+        std::vector<uneven_real> &rs = kv.second;
+        if (rs[0].b.size() > 128 && rs[0].b.size() <= 256 && rs.size() >= 512) {
+            std::stable_sort(rs.begin(), rs.end(), [](const uneven_real &a, const uneven_real &b) { return a.nx > b.nx; });
+            halves.emplace_back();
+            halves.emplace_back();
+            std::vector<uneven_real> &h0 = *std::prev(halves.end(), 2), &h1 = halves.back();
+            for (size_t i = 0; i < rs.size(); i++) (i % 2 ? h1 : h0).push_back(std::move(rs[i]));
+            gl.push_back(&h0);
+            gl.push_back(&h1);
+        } else {
+            gl.push_back(&rs);
+        }
+    }
     std::vector<int32_t> gworst(gl.size(), 0);
     auto run_group = [&](size_t k) {
         std::vector<uneven_real> &rs = *gl[k];

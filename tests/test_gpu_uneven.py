@@ -65,3 +65,21 @@ def test_uneven_absorbing_realization_against_oracle():
         ow = O.Waveguide(**ps[r])
         assert st[r] == 0
         assert_T_close(vals[r], ow.tdistrib_potential(ow.potential(int(seeds[r]))))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed_fp32"])
+def test_split_group_equals_unsplit(precision):
+    """A group of 512 or more realizations above 128 modes is swept as two halves on two streams; each
+    realization must equal, bit for bit, the same realization in a call whose groups are not split
+    (the same realizations in two calls of 260)."""
+    rng = np.random.default_rng(7)
+    ps = [guide(128.25, int(rng.integers(2, 7)), float(rng.uniform(0.0, 0.5)), ny=257) for _ in range(520)]
+    seeds = np.arange(1000, 1520, dtype=np.int64)
+    say(f"split N=256 group of 520 ({precision})")
+    vals, st, code = binding.tdistrib_uneven(ps, seeds, precision, batch_max=520)
+    assert code == 0 and not st.any()
+    for lo in (0, 260):
+        ref, st2, code2 = binding.tdistrib_uneven(ps[lo:lo + 260], seeds[lo:lo + 260], precision, batch_max=260)
+        assert code2 == 0 and not st2.any()
+        for k in range(260):
+            assert np.array_equal(vals[lo + k], ref[k]), lo + k
