@@ -124,15 +124,15 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
         }
     };
 
-    // cr, ci: the real and imaginary parts, or P1 and P2 with P3 in c3 (G3)
-    dy64_d4 cr[XR][YC], ci[XR][YC], c3[G3 ? XR : 1][G3 ? YC : 1];
+    // cr, ci: the real and imaginary parts, or P1 and P2 with P3 in cp3 (G3)
+    dy64_d4 cr[XR][YC], ci[XR][YC], cp3[G3 ? XR : 1][G3 ? YC : 1];
 #pragma unroll
     for (int x = 0; x < XR; x++)
 #pragma unroll
         for (int y = 0; y < YC; y++) {
             cr[x][y] = dy64_d4{0.0, 0.0, 0.0, 0.0};
             ci[x][y] = dy64_d4{0.0, 0.0, 0.0, 0.0};
-            if constexpr (G3) c3[x][y] = dy64_d4{0.0, 0.0, 0.0, 0.0};
+            if constexpr (G3) cp3[x][y] = dy64_d4{0.0, 0.0, 0.0, 0.0};
         }
 
     load(0);
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
                         if (rtile(x) >= T) continue;
                         cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], b0, cr[x][y], 0, 0, 0);
                         ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], b1, ci[x][y], 0, 0, 0);
-                        c3[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x] + ai[x], b0 + b1, c3[x][y], 0, 0, 0);
+                        cp3[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x] + ai[x], b0 + b1, cp3[x][y], 0, 0, 0);
                     }
                     __builtin_amdgcn_sched_barrier(0);
                 }
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__res
                 double re = cr[x][y][v], im = ci[x][y][v];
                 if constexpr (G3) {
                     re = cr[x][y][v] - ci[x][y][v];
-                    im = c3[x][y][v] - cr[x][y][v] - ci[x][y][v];
+                    im = cp3[x][y][v] - cr[x][y][v] - ci[x][y][v];
                 }
                 C[(size_t)(16 * rt + 4 * v + lg) * NP + 16 * ct + li] = cmk(-re, -im);
             }

@@ -1239,9 +1239,9 @@ rg_status rg_tdistrib_uneven(const rg_params *params, const int64_t *seeds, int3
     // and data as in a group of its own, so the results do not depend on this schedule.
     // A large group above 128 modes (the most costly per slice) is swept as two halves on two streams,
     // its realizations dealt alternately by length: the halves fill each other's partly empty last
-    // rounds of workgroups and the gaps between the short kernels of a slice step (c5 distribution,
-    // 10 000 realizations: FP64 14.3 -> 13.5 s, mixed 6.1 -> 6.0 s; three or four parts gained nothing
-    // more).
+    // rounds of workgroups and the gaps between the short kernels of a slice step (10 000 realizations
+    // with N in {16, ..., 256}, L/l_s in [0.5, 10] and absorption: FP64 14.3 -> 13.5 s, mixed 6.1 -> 6.0 s;
+    // three or four parts gained nothing more).
     std::vector<std::vector<uneven_real> *> gl;
     std::list<std::vector<uneven_real>> halves;
     for (auto &kv : groups) {

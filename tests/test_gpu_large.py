@@ -129,7 +129,7 @@ def test_order_256_pivot_fallback():
     flag them and the pivoted kernel of order 256 invert them again.  Complex128 sweeps use the Schur
     complement, whose pivoting stays inside each 128-block where zgetrf pivots across them: forming
     D - C A^-1 B cancels terms of the spike's size, so this case is not held to 1e-10 (ordinary
-    potentials keep the 1e-10 bar: test_large_mode_counts_fp64_parity, and c3 and c5 in
+    potentials keep the 1e-10 bar: test_large_mode_counts_fp64_parity, and the N = 256 and uneven cases of
     tests/test_gpu_fullsize.py).  Measured on 1-3 of 256 eigenvalues per realization: up to 1.1e-9
     relative with the general form (inverse_fp64 = 3, six products), held to 2e-9; up to 2.3e-9 with the
     symmetric form of Dirichlet guides (the default, four products, S inverted from its lower
