@@ -68,14 +68,17 @@ __global__ __launch_bounds__(256) void k_noise_slice(int ixd, int64_t seed0, con
     if (iy < ny) us[(size_t)b * ny + iy] = noise_cell(b, ixd, iy, seed0, seedp, stdu, seeds, stdus);
 }
 
-// The projection of large batches: each workgroup (8 wavefronts) owns 128 realizations and one
-// contiguous range of 16-cell chunks; wavefront w computes rows 16w.. against all NT column tiles.  The
+// The projection of large batches: each workgroup (8 wavefronts) owns 128 realizations, one contiguous
+// range of 16-cell chunks and one group of NT column tiles (blockIdx.z: tiles NT z ..; above 128 modes
+// the table's 33 tiles form two groups); wavefront w computes rows 16w.. against the group's tiles.  The
 // table chunk (16 x 16 NT doubles) is staged once per workgroup in LDS, one chunk ahead, so the table
 // is read from L2 once per 128 realizations; the slice values of the next chunk are loaded (or, with
 // gen, generated here: the values of k_noise_slice) while the current chunk's products run.  MFMA
 // k-step s of a chunk takes cell 4 (lane / 16) + s of it in lane group lane / 16, for A and B alike.
+// Tiles past the table's last (the second group's 17th) are staged from a valid address and their sums
+// are not written.
 constexpr int PW_RB = 128;
-constexpr int PW_MAXNT = 17;  // ntab <= 272 (Dirichlet and periodic guides up to 128 modes)
+constexpr int PW_MAXNT = 17;  // column tiles per group (ntab <= 272 in one group: up to 128 modes)
 
 template <int NT>
 __global__ __launch_bounds__(512, 1) void k_project_wide(const double *__restrict__ us, int64_t sb, int si, int ny,
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(512, 1) void k_project_wide(const double *__restric
 {
     __shared__ double s_t[2][PR_KC][16 * NT];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
-    const int b0 = blockIdx.x * PW_RB, part = blockIdx.y;
+    const int b0 = blockIdx.x * PW_RB, part = blockIdx.y, tile0 = NT * blockIdx.z;
     const int c0 = (int)((int64_t)part * nchunk / parts), c1 = (int)((int64_t)(part + 1) * nchunk / parts);
     const int b = b0 + 16 * w + li;  // the realization of this lane's A operand rows
     const bool bok = b < nb;
@@ -106,15 +109,19 @@ __global__ __launch_bounds__(512, 1) void k_project_wide(const double *__restric
                 a[s] = gen ? noise_cell(b, ixd, iy, seed0, seedp, stdu, seeds, stdus) : us[(size_t)b * sb + (size_t)iy * si];
         }
     };
-    // Rows 16 c .. 16 c + 15 of the zero-padded table (ntp = 16 NT: one contiguous block here and in LDS),
-    // loaded straight into LDS: lane l of a wave writes the 16 bytes at the wave's LDS base + 16 l.
+    // Rows 16 c .. 16 c + 15, columns 16 tile0 .. 16 (tile0 + NT) - 1 of the zero-padded table (ntp columns),
+    // loaded straight into LDS: lane l of a wave writes the 16 bytes at the wave's LDS base + 16 l, so the
+    // staged rows are contiguous in LDS whatever their stride in the table.
     auto stage_tab = [&](int c, int buf) {
-        constexpr int UNITS = PR_KC * 16 * NT / 2;  // 16-byte units, a multiple of 64
-        const double *src = tab + (size_t)c * PR_KC * ntp;
+        constexpr int RU = 8 * NT, UNITS = PR_KC * RU;  // 16-byte units per staged row and per chunk (x 64)
         double *dst = &s_t[buf][0][0];
         for (int r = 0; r < UNITS; r += 512) {
             const int u0 = r + 64 * w;  // this wave's first unit (wave-uniform test)
-            if (u0 < UNITS) __builtin_amdgcn_global_load_lds(src + 2 * (u0 + lane), dst + 2 * u0, 16, 0, 0);
+            if (u0 < UNITS) {
+                const int u = u0 + lane, col = 16 * tile0 + 2 * (u % RU);
+                const double *src = tab + (size_t)(c * PR_KC + u / RU) * ntp + (col < ntp ? col : 0);
+                __builtin_amdgcn_global_load_lds(src, dst + 2 * u0, 16, 0, 0);
+            }
         }
     };
     if (c0 < c1) {
@@ -142,7 +149,7 @@ __global__ __launch_bounds__(512, 1) void k_project_wide(const double *__restric
     double *out = cq + (size_t)part * nb * ntab;
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-        const int q = 16 * t + li;
+        const int q = 16 * (tile0 + t) + li;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int bb = b0 + 16 * w + 4 * v + lk;
@@ -232,8 +239,12 @@ void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double 
         return e ? std::atoi(e) : 1;
     }();
     const int nti = P->ntp / 16;
-    if (nb >= 16 * PW_RB && nti <= PW_MAXNT) {
-        const int gen = !d_upot && gen_env;
+    // one group of column tiles (up to 128 modes) from 2048 realizations, two groups (up to 256 modes, the
+    // disorder from k_noise_slice rather than generated twice) from 1024: 0.52 -> 0.30 ms per slice of
+    // 4096 realizations at N = 256 (ny = 2565)
+    const int tgroups = (nti + PW_MAXNT - 1) / PW_MAXNT;
+    if ((tgroups == 1 && nb >= 16 * PW_RB) || (tgroups == 2 && nb >= 8 * PW_RB)) {
+        const int gen = !d_upot && gen_env && tgroups == 1;
         if (!d_upot && !gen) {
             hipLaunchKernelGGL(k_noise_slice, dim3((P->ny + 255) / 256, nb), dim3(256), 0, P->stream, ixd, seed0,
                                P->seed_dev, P->ny, P->stdu, P->d_seeds_r, P->d_stdu_r, P->d_us);
@@ -241,8 +252,8 @@ void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double 
             sb = P->ny;
             si = 1;
         }
-        const dim3 grid((nb + PW_RB - 1) / PW_RB, P->cq_parts);
-        switch (nti) {
+        const dim3 grid((nb + PW_RB - 1) / PW_RB, P->cq_parts, tgroups);
+        switch (tgroups == 1 ? nti : PW_MAXNT) {
 #define RG_PW(T)                                                                                                   \
     case T:                                                                                                        \
         hipLaunchKernelGGL(k_project_wide<T>, grid, dim3(512), 0, P->stream, us, sb, si, P->ny, gen, ixd, seed0,    \

@@ -193,24 +193,33 @@ def test_persistent_dyson_loop(dyson):
         assert np.max(np.abs(Tb[r] - To)) <= MIXED_TOL["mixed_fp32"], r
 
 
+WIDE_CASES = {
+    # one group of column tiles, seeded disorder generated in the kernel, from 2048 realizations
+    "N128": (dict(CASES["N128"], nxdiso=6), 2048),
+    # two groups of column tiles (513 table columns), disorder from k_noise_slice, from 1024 realizations
+    "N256": (dict(boundtype="dirichlet", nxdiso=4, wol=128.25, aratio=0.2 / 128.25, dscat=0.3), 1024),
+}
+
+
+@pytest.mark.parametrize("case", list(WIDE_CASES))
 @pytest.mark.parametrize("precision", ["fp64", "mixed_fp32"])
-def test_wide_projection_matches_small_batches(precision):
-    """Batches of 2048 and more take the wide projection kernel (128 realizations per workgroup, the
-    seeded disorder generated inside it; explicit potentials loaded); smaller batches the 32-realization
-    kernel fed by k_noise_slice.  The same realizations through both must agree: complex128 within the
-    FP64 bar, complex64 within the mixed bound (the two kernels add the cells in different orders)."""
-    p = dict(CASES["N128"], nxdiso=6)
-    say(f"wide projection against small batches, {precision}")
-    big = binding.Plan(p, batch_max=2048)
+def test_wide_projection_matches_small_batches(precision, case):
+    """Large batches take the wide projection kernel (128 realizations per workgroup, one or two groups
+    of column tiles; explicit potentials loaded); small batches the 32-realization kernel fed by
+    k_noise_slice.  The same realizations through both must agree: complex128 within the FP64 bar,
+    complex64 within the mixed bound (the two kernels add the cells in different orders)."""
+    p, nb = WIDE_CASES[case]
+    say(f"wide projection against small batches, {case} x {nb}, {precision}")
+    big = binding.Plan(p, batch_max=nb)
     small = binding.Plan(p, batch_max=4)
-    picks = [0, 1, 1023, 2047]
-    Tb, st, code = big.tdistrib(11, 2048, precision)
+    picks = [0, 1, nb // 2 - 1, nb - 1]
+    Tb, st, code = big.tdistrib(11, nb, precision)
     assert code == 0 and not st.any()
-    Tb = Tb.reshape(2048, -1)
-    U = big.generate_potential(11, 2048)  # (nxdiso, ny, 2048)
+    Tb = Tb.reshape(nb, -1)
+    U = big.generate_potential(11, nb)  # (nxdiso, ny, nb)
     Tbu, st, code = big.tdistrib_potential(U, precision)
     assert code == 0 and not st.any()
-    Tbu = Tbu.reshape(2048, -1)
+    Tbu = Tbu.reshape(nb, -1)
     for k in picks:
         Ts, st, code = small.tdistrib(11 + k, 1, precision)
         assert code == 0 and not st.any()
