@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librecurgreen_hip.so")
-SOURCES = ["rg_plan.hip", "rg_svd.hip", "rg_kernels_f64.hip", "rg_blocked.hip", "rg_gj.hip", "rg_project.hip", "rg_eig.hip", "rg_zgemm.hip", "rg_refl.hip", "rg_adjoint.hip", "rg_nopiv.hip", "rg_dyson32.hip", "rg_schur.hip", "rg_tprofile.hip", "rg_dyson64.hip", "rg_dyson_bf3.hip", "rg_noalloc.hip", "rg_nopiv64.hip"]
+SOURCES = ["rg_plan.hip", "rg_svd.hip", "rg_kernels_f64.hip", "rg_blocked.hip", "rg_gj.hip", "rg_project.hip", "rg_eig.hip", "rg_zgemm.hip", "rg_refl.hip", "rg_adjoint.hip", "rg_nopiv.hip", "rg_dyson32.hip", "rg_schur.hip", "rg_tprofile.hip", "rg_dyson64.hip", "rg_dyson_bf3.hip", "rg_noalloc.hip", "rg_nopiv64.hip", "rg_project_gen.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops is switched off for every HIP object: the v_pk_*_f32 forms are avoided.
@@ -24,6 +24,11 @@ FLAGS = [
 ]
 
 
+# Per-source additions: the generating projection keeps its generator's FP64 constants inside the chunk loop
+# (MachineLICM would hoist them into 88 registers that its accumulators need).
+EXTRA_FLAGS = {"rg_project_gen.hip": ["-mllvm", "-disable-machine-licm"]}
+
+
 def build(verbose=False, only_changed=False):
     """Compile every HIP source for gfx950 (one hipcc per file, in parallel) and link the
     shared library in-tree; raises on failure.  only_changed: recompile only the sources newer than
@@ -34,7 +39,8 @@ def build(verbose=False, only_changed=False):
     os.makedirs(odir, exist_ok=True)
     objs = [os.path.join(odir, os.path.splitext(f)[0] + ".o") for f in SOURCES]
 
-    hdrs = [os.path.join(CSRC, "rg_internal.h"), os.path.join(HERE, "..", "include", "recurgreen_hip.h")]
+    hdrs = [os.path.join(CSRC, "rg_internal.h"), os.path.join(CSRC, "rg_cos.h"),
+            os.path.join(HERE, "..", "include", "recurgreen_hip.h")]
     newest_hdr = max(os.path.getmtime(h) for h in hdrs)
 
     def stale(i):
@@ -45,7 +51,8 @@ def build(verbose=False, only_changed=False):
     def compile_one(i):
         if not stale(i):
             return None, None
-        cmd = [HIPCC, f"--offload-arch={ARCH}", *FLAGS, "-c", os.path.join(CSRC, SOURCES[i]), "-o", objs[i]]
+        cmd = [HIPCC, f"--offload-arch={ARCH}", *FLAGS, *EXTRA_FLAGS.get(SOURCES[i], []), "-c",
+               os.path.join(CSRC, SOURCES[i]), "-o", objs[i]]
         if verbose:
             print(" ".join(cmd))
         return cmd, subprocess.run(cmd, capture_output=True, text=True)

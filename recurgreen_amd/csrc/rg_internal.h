@@ -141,7 +141,8 @@ struct rg_plan {
     int ntab = 0;      // projection coefficients per slice (doubles)
     int ntp = 0;       // ntab padded to a multiple of 16 (table row length)
     int nchunk = 0;    // transverse cells in chunks of 16 (table rows = 16 * nchunk, zero padded)
-    int cq_parts = 1;  // partial sums of the projection (transverse-cell split)
+    int cq_parts = 1;  // partial sums of the projection (transverse-cell split), at most
+    int cq_parts_used = 1;  // partial sums written by the last projection launch (<= cq_parts)
     int device = 0;
     hipStream_t stream = nullptr;
     int batch_max = 0;
@@ -242,6 +243,9 @@ constexpr int kMaxCqParts = 8;
 void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double *d_upot, int upot_nreal);
 int rg_project_parts(int ny);
 bool rg_project_supported(int ntab);
+// rg_project_gen.hip: the seeded projection of one slice with the disorder generated in the kernel
+// (3 to 17 column tiles); false (nothing launched) for other table widths
+bool rg_launch_project_gen(rg_plan *P, int nb, int ixd, int64_t seed0);
 // Kernel launchers (rg_kernels_f64.hip).
 void rg_launch_inverse_f64(rg_plan *P, int nb, int slot, int first, bool with_disorder);
 void rg_launch_gemm_f64(rg_plan *P, int nb, const double2 *gss, const double2 *gs1_old, double2 *gs1_new);
@@ -339,6 +343,8 @@ rg_cqv rg_cq_view(const rg_plan *P, int nb, bool with_disorder);
 void rg_timer_begin(rg_plan *P, int id, hipEvent_t *stop_out);
 void rg_timer_end(rg_plan *P, int id, hipEvent_t stop);
 
+#include "rg_cos.h"
+
 // Host-side Philox4x32-10 (shared with the device generator: same constants and rounds).
 __host__ __device__ inline void rg_philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
 {
@@ -368,5 +374,5 @@ __host__ __device__ inline double rg_normal_cell(uint64_t seed, uint32_t stream,
     uint64_t b = ((uint64_t)(c[2] >> 5) << 26) | (uint64_t)(c[3] >> 6);
     double u1 = (double)(a + 1) * 0x1.0p-53;
     double u2 = (double)b * 0x1.0p-53;
-    return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+    return sqrt(-2.0 * log(u1)) * rg_cos_0_2pi(2.0 * 3.14159265358979323846 * u2);
 }

@@ -7,8 +7,9 @@
 // whole batch and one slice this is the product  C[b][q] = sum_i U[b][i] tab[i][q]:
 // M = realizations, N = table columns (2N+1 or 2N), K = ny transverse cells.
 //
-// Batches of 2048 realizations and more use k_project_wide (below: 128 realizations per workgroup, the
-// table staged straight into LDS, the seeded disorder generated in the kernel).  Smaller batches:
+// Batches of 2048 realizations and more use k_project_gen (rg_project_gen.hip: seeded, the disorder
+// generated in the kernel) or k_project_wide (below: 128 realizations per workgroup, the table staged
+// straight into LDS).  Smaller batches:
 // each workgroup (4 wavefronts) owns 32 realizations (two 16-row MFMA tiles), one contiguous range
 // of 16-cell chunks of K and one group of up to 20 table column tiles (blockIdx.z); wavefront w owns
 // the tiles {w, w+4, ...} of its group.  The slice
@@ -68,24 +69,20 @@ __global__ __launch_bounds__(256) void k_noise_slice(int ixd, int64_t seed0, con
     if (iy < ny) us[(size_t)b * ny + iy] = noise_cell(b, ixd, iy, seed0, seedp, stdu, seeds, stdus);
 }
 
-// The projection of large batches: each workgroup (8 wavefronts) owns 128 realizations, one contiguous
-// range of 16-cell chunks and one group of NT column tiles (blockIdx.z: tiles NT z ..; above 128 modes
-// the table's 33 tiles form two groups); wavefront w computes rows 16w.. against the group's tiles.  The
-// table chunk (16 x 16 NT doubles) is staged once per workgroup in LDS, one chunk ahead, so the table
-// is read from L2 once per 128 realizations; the slice values of the next chunk are loaded (or, with
-// gen, generated here: the values of k_noise_slice) while the current chunk's products run.  MFMA
-// k-step s of a chunk takes cell 4 (lane / 16) + s of it in lane group lane / 16, for A and B alike.
-// Tiles past the table's last (the second group's 17th) are staged from a valid address and their sums
-// are not written.
+// The projection of large batches from slice values in memory (explicit potentials, or the disorder of
+// k_noise_slice): each workgroup (8 wavefronts) owns 128 realizations, one contiguous range of 16-cell
+// chunks and one group of NT column tiles (blockIdx.z: tiles NT z ..; above 128 modes the table's 33 tiles
+// form two groups); wavefront w computes rows 16w.. against the group's tiles.  The table chunk
+// (16 x 16 NT doubles) is staged once per workgroup in LDS, one chunk ahead, so the table is read from L2
+// once per 128 realizations; the slice values of the next chunk are loaded while the current chunk's
+// products run.  MFMA k-step s of a chunk takes cell 4 (lane / 16) + s of it in lane group lane / 16, for A
+// and B alike.  Tiles past the table's last (the second group's 17th) are staged from a valid address and
+// their sums are not written.  (Seeded batches of one column group: k_project_gen, rg_project_gen.hip.)
 constexpr int PW_RB = 128;
 constexpr int PW_MAXNT = 17;  // column tiles per group (ntab <= 272 in one group: up to 128 modes)
 
 template <int NT>
 __global__ __launch_bounds__(512, 1) void k_project_wide(const double *__restrict__ us, int64_t sb, int si, int ny,
-                                                         int gen, int ixd, int64_t seed0,
-                                                         const int64_t *__restrict__ seedp, double stdu,
-                                                         const int64_t *__restrict__ seeds,
-                                                         const double *__restrict__ stdus,
                                                          const double *__restrict__ tab, int ntp, int ntab,
                                                          int nchunk, int parts, double *__restrict__ cq, int nb)
 {
@@ -105,8 +102,7 @@ __global__ __launch_bounds__(512, 1) void k_project_wide(const double *__restric
         for (int s = 0; s < 4; s++) {
             const int iy = c * PR_KC + 4 * lk + s;
             a[s] = 0.0;
-            if (bok && iy < ny)
-                a[s] = gen ? noise_cell(b, ixd, iy, seed0, seedp, stdu, seeds, stdus) : us[(size_t)b * sb + (size_t)iy * si];
+            if (bok && iy < ny) a[s] = us[(size_t)b * sb + (size_t)iy * si];
         }
     };
     // Rows 16 c .. 16 c + 15, columns 16 tile0 .. 16 (tile0 + NT) - 1 of the zero-padded table (ntp columns),
@@ -227,13 +223,14 @@ bool rg_project_supported(int ntab) { return (ntab + 15) / 16 <= 2 * PR_MAXNT; }
 void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double *d_upot, int upot_nreal)
 {
     (void)upot_nreal;
+    P->cq_parts_used = P->cq_parts;
     hipEvent_t stop = nullptr;
     rg_timer_begin(P, RGK_PROJECT, &stop);
     const double *us = d_upot ? d_upot + ixd : nullptr;  // explicit potentials upot(ix, iy, b)
     int64_t sb = (int64_t)P->nxdiso * P->ny;
     int si = P->nxdiso;
-    // Large batches (one 128-realization workgroup per CU and partial sum at least): the wide kernel,
-    // generating the seeded disorder itself unless RG_PROJECT_GEN=0.
+    // Large batches (one workgroup per CU and partial sum at least): seeded batches of one column group
+    // generate their disorder in k_project_gen unless RG_PROJECT_GEN=0; the others run k_project_wide.
     static const int gen_env = [] {
         const char *e = std::getenv("RG_PROJECT_GEN");
         return e ? std::atoi(e) : 1;
@@ -244,8 +241,11 @@ void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double 
     // 4096 realizations at N = 256 (ny = 2565)
     const int tgroups = (nti + PW_MAXNT - 1) / PW_MAXNT;
     if ((tgroups == 1 && nb >= 16 * PW_RB) || (tgroups == 2 && nb >= 8 * PW_RB)) {
-        const int gen = !d_upot && gen_env && tgroups == 1;
-        if (!d_upot && !gen) {
+        if (!d_upot && gen_env && tgroups == 1 && rg_launch_project_gen(P, nb, ixd, seed0)) {
+            rg_timer_end(P, RGK_PROJECT, stop);
+            return;
+        }
+        if (!d_upot) {
             hipLaunchKernelGGL(k_noise_slice, dim3((P->ny + 255) / 256, nb), dim3(256), 0, P->stream, ixd, seed0,
                                P->seed_dev, P->ny, P->stdu, P->d_seeds_r, P->d_stdu_r, P->d_us);
             us = P->d_us;
@@ -256,9 +256,8 @@ void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double 
         switch (tgroups == 1 ? nti : PW_MAXNT) {
 #define RG_PW(T)                                                                                                   \
     case T:                                                                                                        \
-        hipLaunchKernelGGL(k_project_wide<T>, grid, dim3(512), 0, P->stream, us, sb, si, P->ny, gen, ixd, seed0,    \
-                           P->seed_dev, P->stdu, P->d_seeds_r, P->d_stdu_r, P->d_tab, P->ntp, P->ntab, P->nchunk, \
-                           P->cq_parts, P->d_cq, nb);                                                              \
+        hipLaunchKernelGGL(k_project_wide<T>, grid, dim3(512), 0, P->stream, us, sb, si, P->ny, P->d_tab, P->ntp,   \
+                           P->ntab, P->nchunk, P->cq_parts, P->d_cq, nb);                                          \
         break;
             RG_PW(1) RG_PW(2) RG_PW(3) RG_PW(4) RG_PW(5) RG_PW(6) RG_PW(7) RG_PW(8) RG_PW(9)
             RG_PW(10) RG_PW(11) RG_PW(12) RG_PW(13) RG_PW(14) RG_PW(15) RG_PW(16) RG_PW(17)
@@ -287,7 +286,7 @@ rg_cqv rg_cq_view(const rg_plan *P, int nb, bool with_disorder)
     v.cq = with_disorder ? P->d_cq : nullptr;
     v.ntab = P->ntab;
     v.nb = nb;
-    v.parts = P->cq_parts;
+    v.parts = P->cq_parts_used;
     v.scale = 1.0 / P->ny;
     return v;
 }
