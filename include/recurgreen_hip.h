@@ -217,6 +217,10 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *                   realization (NP 80..128), 2 the same with three real products per complex
  *                   product (Gauss's form);
  *   "project_parts" 1..8 partial sums of the projection over transverse cells;
+ *   "project_wide_min" >= 1: batches of at least this many realizations (default 1: all) project
+ *                   on the wide kernels (128 or 64 realizations per workgroup, the table staged
+ *                   straight into LDS, seeded disorder generated in the kernel up to 128 modes);
+ *                   smaller ones on the 32-realization kernel after a separate noise kernel;
  *   "lanes"         1 one stream, 2 the batch split in halves on two streams;
  *   "eigen"         transmission eigenvalues: 0 eigenvalues of t^H t (Householder tridiagonalisation
  *                   and Sturm bisection); 1 one-sided Jacobi SVD of t; 2 Householder bidiagonalisation

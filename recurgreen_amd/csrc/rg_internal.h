@@ -143,6 +143,7 @@ struct rg_plan {
     int nchunk = 0;    // transverse cells in chunks of 16 (table rows = 16 * nchunk, zero padded)
     int cq_parts = 1;  // partial sums of the projection (transverse-cell split), at most
     int cq_parts_used = 1;  // partial sums written by the last projection launch (<= cq_parts)
+    int project_wide_min = 1;  // smallest batch projected by k_project_gen / k_project_wide (plan option)
     int device = 0;
     hipStream_t stream = nullptr;
     int batch_max = 0;

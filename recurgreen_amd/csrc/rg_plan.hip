@@ -357,6 +357,7 @@ static void lane_view(const rg_plan *P, rg_plan *L, int off, int nb)
     L->inv_mixed = P->inv_mixed;
     L->gemm_wg = P->gemm_wg;
     L->dyson64 = P->dyson64;
+    L->project_wide_min = P->project_wide_min;
     L->eigen = P->eigen;
     L->profiling = P->profiling;
     L->profile_mask = P->profile_mask;
@@ -370,8 +371,8 @@ __global__ void k_set_seed(int64_t *__restrict__ dst, int64_t v) { *dst = v; }
 // Options that change the kernels a sweep launches (part of a captured sweep's key).
 static int graph_opts(const rg_plan *P)
 {
-    return P->inv_fp64 | P->inv_mixed << 4 | P->cq_parts << 9 | P->eigen << 20 | P->dyson64 << 14 |
-           P->gemm_wg << 16;
+    return (P->inv_fp64 | P->inv_mixed << 4 | P->cq_parts << 9 | P->eigen << 20 | P->dyson64 << 14 |
+            P->gemm_wg << 16) ^ (int)((uint32_t)P->project_wide_min * 0x9E3779B1u & 0x7F000000u);
 }
 
 void free_graphs(rg_plan *P)
@@ -1122,6 +1123,9 @@ rg_status rg_plan_create(const rg_params *params, int32_t batch_max, int32_t dev
                  : dy && std::strcmp(dy, "bf16x6") == 0 ? 2
                  : dy && std::strcmp(dy, "gauss3") == 0 ? 3
                                                         : P->gemm_wg;
+    // Projection kernels: RG_PROJECT_WIDE_MIN = the smallest batch of the wide / generating kernels (default 1:
+    // every batch; larger values send smaller batches to k_project_mfma, for measurement).
+    if (const char *pw = std::getenv("RG_PROJECT_WIDE_MIN")) P->project_wide_min = std::max(1, std::atoi(pw));
     rg_status st = init_modes(P);  // argument checks first, like init_waveguide
     if (st) { delete P; return st; }
     int ndev = 0;
@@ -1751,6 +1755,9 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
     } else if (std::strcmp(key, "project_parts") == 0) {
         if (value < 1 || value > kMaxCqParts) return RG_EINVAL;
         P->cq_parts = value;
+    } else if (std::strcmp(key, "project_wide_min") == 0) {
+        if (value < 1) return RG_EINVAL;
+        P->project_wide_min = value;
     } else if (std::strcmp(key, "lanes") == 0) {
         if (value != 1 && value != 2) return RG_EINVAL;
         if (value == 2) {
@@ -1783,6 +1790,7 @@ rg_status rg_plan_get_option(const rg_plan *P, const char *key, int32_t *value)
     else if (std::strcmp(key, "dyson_fp64") == 0) *value = P->dyson64;
     else if (std::strcmp(key, "dyson") == 0) *value = P->gemm_wg;
     else if (std::strcmp(key, "project_parts") == 0) *value = P->cq_parts;
+    else if (std::strcmp(key, "project_wide_min") == 0) *value = P->project_wide_min;
     else if (std::strcmp(key, "lanes") == 0) *value = P->lanes;
     else if (std::strcmp(key, "eigen") == 0) *value = P->eigen;
     else if (std::strcmp(key, "profile_mask") == 0) *value = P->profile_mask;

@@ -236,11 +236,13 @@ void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double 
         return e ? std::atoi(e) : 1;
     }();
     const int nti = P->ntp / 16;
-    // one group of column tiles (up to 128 modes) from 2048 realizations, two groups (up to 256 modes, the
-    // disorder from k_noise_slice rather than generated twice) from 1024: 0.52 -> 0.30 ms per slice of
-    // 4096 realizations at N = 256 (ny = 2565)
+    // one group of column tiles (up to 128 modes), or two (up to 256 modes, the disorder from k_noise_slice
+    // rather than generated twice: 0.52 -> 0.30 ms per slice of 4096 realizations at N = 256, ny = 2565),
+    // for every batch of at least project_wide_min realizations (default 1): about twice as fast as
+    // k_noise_slice + k_project_mfma from 128 realizations up (N = 128, 256 realizations: 50 against 90 us
+    // per slice; N = 256, 512: 119 against 224 us)
     const int tgroups = (nti + PW_MAXNT - 1) / PW_MAXNT;
-    if ((tgroups == 1 && nb >= 16 * PW_RB) || (tgroups == 2 && nb >= 8 * PW_RB)) {
+    if (tgroups <= 2 && nb >= P->project_wide_min) {
         if (!d_upot && gen_env && tgroups == 1 && rg_launch_project_gen(P, nb, ixd, seed0)) {
             rg_timer_end(P, RGK_PROJECT, stop);
             return;

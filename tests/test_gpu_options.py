@@ -21,12 +21,13 @@ def test_option_round_trip_and_rejection():
     say("options round trip")
     plan = binding.Plan(CASES["c1_N20_ny205"], batch_max=2)
     for key, values in (("inverse_fp64", (0, 1, 2, 3, 4)), ("inverse_mixed", (1, 2, 3, 4)), ("dyson", (0, 1, 2, 3)),
-                        ("project_parts", (1, 3, 8)), ("lanes", (2, 1)), ("eigen", (1, 0, 2, 3))):
+                        ("project_parts", (1, 3, 8)), ("project_wide_min", (4096, 1)), ("lanes", (2, 1)),
+                        ("eigen", (1, 0, 2, 3))):
         for v in values:
             plan.set_option(key, v)
             assert plan.get_option(key) == v
-    for key, v in (("inverse_fp64", 5), ("inverse_mixed", 0), ("dyson", 4), ("project_parts", 9), ("lanes", 3),
-                   ("eigen", 4), ("nonsense", 0)):
+    for key, v in (("inverse_fp64", 5), ("inverse_mixed", 0), ("dyson", 4), ("project_parts", 9),
+                   ("project_wide_min", 0), ("lanes", 3), ("eigen", 4), ("nonsense", 0)):
         with pytest.raises(binding.RecurGreenError):
             plan.set_option(key, v)
     plan.close()
@@ -41,7 +42,7 @@ def test_every_option_value_matches_oracle(name):
     To = np.stack([ow.tdistrib_potential(U[:, :, k].T) for k in range(2)])
     plan = binding.Plan(p, batch_max=2)
     for key, values in (("inverse_fp64", (0, 1, 2, 3, 4)), ("dyson_fp64", (1, 2, 0)), ("project_parts", (1, 2, 4, 8)),
-                        ("eigen", (1, 0, 2, 3))):
+                        ("project_wide_min", (1 << 30, 1)), ("eigen", (1, 0, 2, 3))):
         for v in values:
             plan.set_option(key, v)
             Tg, st, code = plan.tdistrib_potential(U, "fp64")
@@ -194,9 +195,9 @@ def test_persistent_dyson_loop(dyson):
 
 
 WIDE_CASES = {
-    # one group of column tiles, seeded disorder generated in the kernel, from 2048 realizations
-    "N128": (dict(CASES["N128"], nxdiso=6), 2048),
-    # two groups of column tiles (513 table columns), disorder from k_noise_slice, from 1024 realizations
+    # one group of column tiles, seeded disorder generated in the kernel (four partial sums at 4096)
+    "N128": (dict(CASES["N128"], nxdiso=6), 4096),
+    # two groups of column tiles (513 table columns), disorder from k_noise_slice
     "N256": (dict(boundtype="dirichlet", nxdiso=4, wol=128.25, aratio=0.2 / 128.25, dscat=0.3), 1024),
 }
 
@@ -204,14 +205,17 @@ WIDE_CASES = {
 @pytest.mark.parametrize("case", list(WIDE_CASES))
 @pytest.mark.parametrize("precision", ["fp64", "mixed_fp32"])
 def test_wide_projection_matches_small_batches(precision, case):
-    """Large batches take the wide projection kernel (128 realizations per workgroup, one or two groups
-    of column tiles; explicit potentials loaded); small batches the 32-realization kernel fed by
-    k_noise_slice.  The same realizations through both must agree: complex128 within the FP64 bar,
-    complex64 within the mixed bound (the two kernels add the cells in different orders)."""
+    """The wide projection kernels (seeded: 64 realizations per workgroup with the disorder generated in
+    the kernel up to 128 modes, else 128 per workgroup in one or two groups of column tiles; explicit
+    potentials loaded) on a large batch, against one-realization sweeps on the 32-realization kernel fed
+    by k_noise_slice (project_wide_min raised past the batch).  The same realizations through both must
+    agree: complex128 within the FP64 bar, complex64 within the mixed bound (the kernels add the cells in
+    different orders and cell ranges)."""
     p, nb = WIDE_CASES[case]
     say(f"wide projection against small batches, {case} x {nb}, {precision}")
     big = binding.Plan(p, batch_max=nb)
     small = binding.Plan(p, batch_max=4)
+    small.set_option("project_wide_min", 1 << 30)
     picks = [0, 1, nb // 2 - 1, nb - 1]
     Tb, st, code = big.tdistrib(11, nb, precision)
     assert code == 0 and not st.any()
