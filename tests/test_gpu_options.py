@@ -202,6 +202,26 @@ WIDE_CASES = {
 }
 
 
+@pytest.mark.parametrize("nb", [64, 1024])
+@pytest.mark.parametrize("precision", ["fp64", "mixed_fp32"])
+def test_generated_disorder_equals_explicit_bit_for_bit(precision, nb):
+    """Seeded sweeps up to 128 modes generate each slice's disorder inside the projection kernel
+    (k_project_gen); explicit potentials load it (k_project_wide).  Fed the disorder that
+    rg_generate_potential writes, the two must give the same transmission eigenvalues bit for bit: the
+    generator's values equal the potential's (one Box-Muller cosine on both paths), and at these batch
+    sizes both kernels add the same cell ranges (8 partial sums) in the same order."""
+    p = dict(CASES["N128"], nxdiso=5)
+    say(f"generated against explicit disorder, N128 x {nb}, {precision}")
+    plan = binding.Plan(p, batch_max=nb)
+    assert plan.get_option("project_parts") == 8
+    T1, st1, c1 = plan.tdistrib(21, nb, precision)
+    U = plan.generate_potential(21, nb)
+    T2, st2, c2 = plan.tdistrib_potential(U, precision)
+    assert c1 == 0 and c2 == 0 and not st1.any() and not st2.any()
+    assert np.array_equal(T1, T2)
+    plan.close()
+
+
 @pytest.mark.parametrize("case", list(WIDE_CASES))
 @pytest.mark.parametrize("precision", ["fp64", "mixed_fp32"])
 def test_wide_projection_matches_small_batches(precision, case):
