@@ -7,12 +7,12 @@
 // whole batch and one slice this is the product  C[b][q] = sum_i U[b][i] tab[i][q]:
 // M = realizations, N = table columns (2N+1 or 2N), K = ny transverse cells.
 //
-// Batches of 2048 realizations and more use k_project_gen (rg_project_gen.hip: seeded, the disorder
-// generated in the kernel) or k_project_wide (below: 128 realizations per workgroup, the table staged
-// straight into LDS).  Smaller batches:
-// each workgroup (4 wavefronts) owns 32 realizations (two 16-row MFMA tiles), one contiguous range
-// of 16-cell chunks of K and one group of up to 20 table column tiles (blockIdx.z); wavefront w owns
-// the tiles {w, w+4, ...} of its group.  The slice
+// Batches of at least the plan's project_wide_min realizations (default 1: all) use k_project_gen
+// (rg_project_gen.hip: seeded, up to 128 modes, the disorder generated in the kernel) or k_project_wide
+// (below: 128 realizations per workgroup, the table staged straight into LDS).  Smaller ones take
+// k_project_mfma: each workgroup (4 wavefronts) owns 32 realizations (two 16-row MFMA tiles), one
+// contiguous range of 16-cell chunks of K and one group of up to 20 table column tiles (blockIdx.z);
+// wavefront w owns the tiles {w, w+4, ...} of its group.  The slice
 // values U (generated for the slice by k_noise_slice, or read from an explicit potential batch) are
 // staged through LDS one chunk ahead of the v_mfma_f64_16x16x4 products; table tiles stream from
 // L2.
