@@ -676,26 +676,31 @@ void rg_launch_inverse_sym32(rg_plan *P, int nb, int slot, int first, bool with_
     const rg_cqv cq = rg_cq_view(P, nb, with_disorder);
     const double2 *ad = P->d_adiag + (size_t)slot * P->np;
     switch (P->np / 16) {
-#define RG_SYM(W, NW)                                                                                        \
-    case W:                                                                                                  \
-        hipLaunchKernelGGL((k_inverse_sym32<W, NW, 16, 4>), dim3(nb), dim3(64 * NW), 0, P->stream, P->d_gss32, \
-                           cq, ad, P->d_b, P->nmu, first, redo, P->cstride, P->d_nx_r, P->cur_ix);           \
+// OCC: waves per SIMD the register budget is set for.  At 10 and 11 blocks one realization fills a CU at 3
+// or 4 waves per SIMD alike, and 3 removes the 59 spilled registers of the 4-wave budget (1.80 against 1.86 ms
+// per launch of 4096 at N = 150).  Kept at 4 where measured faster: order 112 with 59 spills (0.945 ms per
+// launch of 8192 at N = 100, against 1.03 at 3 waves and 1.29 with 7 waves per realization), and 224 and 256
+// with 15 and 6 (3.62 and 4.32 ms, against 4.21 and 4.97 ms with half the waves at 2 per SIMD).
+#define RG_SYM(W, NW, OCC)                                                                                      \
+    case W:                                                                                                     \
+        hipLaunchKernelGGL((k_inverse_sym32<W, NW, 16, OCC>), dim3(nb), dim3(64 * NW), 0, P->stream, P->d_gss32, \
+                           cq, ad, P->d_b, P->nmu, first, redo, P->cstride, P->d_nx_r, P->cur_ix);              \
         break;
-        RG_SYM(2, 2)
-        RG_SYM(3, 3)
-        RG_SYM(4, 4)
-        RG_SYM(5, 4)
-        RG_SYM(6, 4)
-        RG_SYM(7, 4)
-        RG_SYM(8, 4)
-        RG_SYM(9, 9)
-        RG_SYM(10, 10)
-        RG_SYM(11, 11)
-        RG_SYM(12, 12)
-        RG_SYM(13, 13)
-        RG_SYM(14, 14)
-        RG_SYM(15, 15)
-        RG_SYM(16, 16)
+        RG_SYM(2, 2, 4)
+        RG_SYM(3, 3, 4)
+        RG_SYM(4, 4, 4)
+        RG_SYM(5, 4, 4)
+        RG_SYM(6, 4, 4)
+        RG_SYM(7, 4, 4)
+        RG_SYM(8, 4, 4)
+        RG_SYM(9, 9, 4)
+        RG_SYM(10, 10, 3)
+        RG_SYM(11, 11, 3)
+        RG_SYM(12, 12, 4)
+        RG_SYM(13, 13, 4)
+        RG_SYM(14, 14, 4)
+        RG_SYM(15, 15, 4)
+        RG_SYM(16, 16, 4)
 #undef RG_SYM
         default: break;
     }

@@ -22,6 +22,8 @@ CASES = {
                         ny=400, dos_model=1),
     # periodic, N = 161 (np = 176)
     "periodic_N161": dict(boundtype="periodic", nxdiso=6, wol=80.3, aratio=0.3 / 80.3, dscat=0.2, ny=200),
+    # N = 150 (np = 160: the complex64 symmetric kernel of 10 blocks, set for 3 waves per SIMD)
+    "N150": dict(boundtype="dirichlet", nxdiso=8, wol=75.25, aratio=0.4 / 75.25, dscat=0.3, ny=300),
 }
 
 
