@@ -212,7 +212,8 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *                   product; NP 96 and 128, else 1; faster, but the N = 128 sweep's mean T moved
  *                   4.7e-7 from complex128 against 2.0e-8 with 1, so not the default), 3 (default)
  *                   one workgroup per realization with three real products per complex product
- *                   (Gauss's form; NP 96 and 128, else 1);
+ *                   (Gauss's form; NP 96 and 128 whole, 160, 192, 224, 256 as
+ *                   2 x 2 blocks, else 1);
  *   "dyson_fp64"    complex128 Dyson product: 0 64x64 output tiles, 1 (default) one workgroup per
  *                   realization (NP 80..128), 2 the same with three real products per complex
  *                   product (Gauss's form);

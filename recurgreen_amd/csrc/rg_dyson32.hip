@@ -161,10 +161,10 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restri
                                                           float2 *__restrict__ out, const double2 *__restrict__ bvec,
                                                           int n, int cs, int nb)
 {
-    static_assert(NP % 32 == 0 && MB % 32 == 0 && NP % MB == 0 && MB <= 128, "k chunks of 32, 8 x 8 tiles at most");
+    static_assert(NP % 32 == 0 && MB % 16 == 0 && NP % MB == 0 && MB <= 128, "k chunks of 32, 8 x 8 tiles at most");
     constexpr int T = MB / 16, NK = NP / 32, NQ = MB * 16;  // tiles, k chunks, float4 per operand chunk
     constexpr int QD = NP / MB, QI = QD * QD;               // blocks per dimension and per realization
-    constexpr int LOADS = NQ / 512;
+    constexpr int LOADS = (NQ + 511) / 512;                 // rounds of 512 float4 (the last partial at MB 80, 112)
     __shared__ DysonShared3m<MB> sh;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1, li = lane & 15, lg = lane >> 4;
@@ -184,6 +184,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restri
 #pragma unroll
         for (int e = 0; e < LOADS; e++) {
             const int idx = tid + 512 * e;
+            if (NQ % 512 != 0 && idx >= NQ) break;
             ra[e] = A4[(size_t)(row0 + (idx >> 4)) * (NP / 2) + kc * 16 + (idx & 15)];
         }
     };
@@ -193,6 +194,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restri
 #pragma unroll
         for (int e = 0; e < LOADS; e++) {
             const int idx = tid + 512 * e, s = idx >> 6, l = idx & 63;
+            if (NQ % 512 != 0 && idx >= NQ) break;
             rb[e] = B4[(size_t)(kc * 32 + 8 * (s & 3) + (l & 7)) * (NP / 2) + col0 / 2 + 8 * (s >> 2) + (l >> 3)];
         }
     };
@@ -200,6 +202,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restri
 #pragma unroll
         for (int e = 0; e < LOADS; e++) {
             const int idx = tid + 512 * e;
+            if (NQ % 512 != 0 && idx >= NQ) break;
             const int row = idx >> 4, k2 = 2 * (idx & 15);
             *reinterpret_cast<float2 *>(&sh.a[buf][0][row][k2]) = make_float2(ra[e].x, ra[e].z);
             *reinterpret_cast<float2 *>(&sh.a[buf][1][row][k2]) = make_float2(ra[e].y, ra[e].w);
@@ -210,6 +213,7 @@ __global__ __launch_bounds__(512, 2) void k_dyson_gemm_3m(const float2 *__restri
 #pragma unroll
         for (int e = 0; e < LOADS; e++) {
             const int idx = tid + 512 * e, s = idx >> 6, l = idx & 63;
+            if (NQ % 512 != 0 && idx >= NQ) break;
             const int k = 8 * (s & 3) + (l & 7), c = 2 * (8 * (s >> 2) + (l >> 3)), gk = kc * 32 + k;
             const double2 bk = gk < n ? bv[gk] : cmk(0.0, 0.0);
             sh.b[buf][0][c][k] = (float)(bk.x * rb[e].x - bk.y * rb[e].y);
@@ -324,11 +328,16 @@ bool rg_launch_gemm_wg(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_
     }
 }
 
-// Option dyson = 3: NP 96, 128 (the whole matrix per work item), 192 and 256 (four blocks of 96 or 128
-// per realization); false otherwise.
+// Option dyson = 3: NP 96, 128 (the whole matrix per work item), 160, 192, 224 and 256 (four blocks of 80 to 128
+// per realization); false otherwise.  Blocks of 80 and 112 rows (5 x 5 and 7 x 7 tiles over the 8 waves' 4 x 2
+// grid of up to 8 x 8) stage their operands in a last, partial round of 512 float4.  At NP 80 and 112 themselves
+// (k chunks of 32 would need zero padding) the four-product kernel with one workgroup per realization is
+// faster (measured with padding):
+// 0.42 against 0.55 ms and 0.96 against 1.16 ms per launch of 8192; the blocks of 80 and 112 beat the 64 x 64
+// tiles at NP 160 and 224 (1.52 against 1.89 ms and 3.21 against 4.50 ms per launch of 4096).
 bool rg_launch_gemm_3m(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_old, float2 *gs1_new)
 {
-    const int qi = P->np > 128 ? 4 : 1, items = nb * qi;
+    const int qi = P->np > 128 ? 4 : 1, items = nb * qi;  // the orders above 128 as 2 x 2 blocks
     const int grid = items < P->ncu ? (items < 1 ? 1 : items) : P->ncu;
     switch (P->np) {
 #define RG_3M(N, MB)                                                                                           \
@@ -338,7 +347,9 @@ bool rg_launch_gemm_3m(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_
         return true;
         RG_3M(96, 96)
         RG_3M(128, 128)
+        RG_3M(160, 80)
         RG_3M(192, 96)
+        RG_3M(224, 112)
         RG_3M(256, 128)
 #undef RG_3M
         default: return false;

@@ -22,8 +22,11 @@ CASES = {
                         ny=400, dos_model=1),
     # periodic, N = 161 (np = 176)
     "periodic_N161": dict(boundtype="periodic", nxdiso=6, wol=80.3, aratio=0.3 / 80.3, dscat=0.2, ny=200),
-    # N = 150 (np = 160: the complex64 symmetric kernel of 10 blocks, set for 3 waves per SIMD)
+    # N = 150 (np = 160: the complex64 symmetric kernel of 10 blocks, set for 3 waves per SIMD; the
+    # three-product Dyson kernel in 2 x 2 blocks of 80, its last operand load round partial)
     "N150": dict(boundtype="dirichlet", nxdiso=8, wol=75.25, aratio=0.4 / 75.25, dscat=0.3, ny=300),
+    # N = 220 (np = 224: the three-product Dyson kernel in 2 x 2 blocks of 112)
+    "N220": dict(boundtype="dirichlet", nxdiso=6, wol=110.25, aratio=0.3 / 110.25, dscat=0.3, ny=440),
 }
 
 

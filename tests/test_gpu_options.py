@@ -138,11 +138,12 @@ def test_graph_replays_match_direct_launches(precision):
 
 
 @pytest.mark.parametrize("dyson", [2, 3])
-@pytest.mark.parametrize("wol", [48.25, 64.25])
+@pytest.mark.parametrize("wol", [40.25, 48.25, 56.25, 64.25])
 def test_persistent_dyson_matches_oracle(wol, dyson):
     """Options dyson = 2 (the complex64 Dyson product on the BF16 matrix cores with three-way split
-    FP32 operands) and 3 (three real FP32 products per complex product) at both orders they are built
-    for, NP = 96 and 128, within the complex64 bound."""
+    FP32 operands) and 3 (three real FP32 products per complex product), both built for NP = 96 and 128
+    (other orders up to 128 take dyson = 1), at NP = 80, 96, 112, 128, within the complex64 bound.
+    (NP 160 and 224, the three-product kernel in blocks of 80 and 112: tests/test_gpu_large.py.)"""
     p = dict(boundtype="dirichlet", nxdiso=40, wol=wol, aratio=2.0 / wol, dscat=1.0, ny=int(20 * wol), dos_model=1)
     say(f"Dyson option {dyson} N={int(2 * wol)}")
     ow = O.Waveguide(**p)
