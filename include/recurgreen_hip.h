@@ -215,8 +215,8 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *                   (Gauss's form; NP 96 and 128 whole, 160, 192, 224, 256 as
  *                   2 x 2 blocks, else 1);
  *   "dyson_fp64"    complex128 Dyson product: 0 64x64 output tiles, 1 (default) one workgroup per
- *                   realization (NP 80..128), 2 the same with three real products per complex
- *                   product (Gauss's form);
+ *                   realization (NP 80..128) or per quadrant (NP 160, 192, 224, 256), 2 the same
+ *                   with three real products per complex product (Gauss's form);
  *   "project_parts" 1..8 partial sums of the projection over transverse cells;
  *   "project_wide_min" >= 1: batches of at least this many realizations (default 1: all) project
  *                   on the wide kernels (128 or 64 realizations per workgroup, the table staged

@@ -30,9 +30,9 @@ struct DysonShared64 {
 // Re = P1 - P2, Im = P3 - P1 - P2: three real products on three accumulators instead of four
 // products on two (the error of Im is bounded by eps (|P1| + |P2| + |P3|) sums).
 // A workgroup computes an MB x MB block of one realization's product: the whole of it (MB = NP) up to
-// NP = 128, one of the four 128 x 128 quadrants at NP = 256 (workgroup 4 r + q: rows 128 (q / 2) ..,
-// columns 128 (q % 2) ..), so that each operand element is read twice instead of four times by 64 x 64
-// tiles.
+// NP = 128, one of the four NP/2 x NP/2 quadrants at NP = 160, 192, 224, 256 (workgroup 4 r + q: rows
+// NP/2 (q / 2) .., columns NP/2 (q % 2) ..), so that each operand element is read twice instead of four times
+// by 64 x 64 tiles (NP 160 and 224: 3.38 and 7.24 ms per launch of 4096 against 4.20 and 10.13 ms).
 template <int NP, int MB, bool G3>
 __global__ __launch_bounds__(512, 2) void k_dyson_gemm_wg64(const double2 *__restrict__ gss,
                                                             const double2 *__restrict__ gs1, double2 *__restrict__ out,
@@ -241,7 +241,9 @@ bool rg_launch_gemm_wg64(rg_plan *P, int nb, const double2 *gss, const double2 *
         RG_WG64(96, 96)
         RG_WG64(112, 112)
         RG_WG64(128, 128)
+        RG_WG64(160, 80)
         RG_WG64(192, 96)
+        RG_WG64(224, 112)
         RG_WG64(256, 128)
 #undef RG_WG64
         default: return false;
