@@ -76,8 +76,8 @@ rg_status rg_plan_dims(const rg_plan *plan, int32_t *dims, double *vals);
 /* Mirrors compute_tdistrib (src/waveguide.f90:392-462) over a batch of seeds, the loop of
  * src/main.f90:148-171: realization r uses seed seed0 + r; tallvals (host) receives
  * nseed*neigv values, status (host, may be NULL) one rg_status per realization.  The disorder
- * is generated on the device (Philox4x32-10 keyed by seed, ix, iy; Box-Muller cosine branch of
- * src/random_normal.f90:64). */
+ * is generated on the device (Philox4x32-10 keyed by seed, ix, iy / 2; both outputs of the
+ * Box-Muller transform of src/random_normal.f90:64, cosine for even iy, sine for odd iy). */
 rg_status rg_tdistrib(rg_plan *plan, int64_t seed0, int32_t nseed, int32_t precision,
                       double *tallvals, int32_t *status);
 /* Same with device-resident output (tallvals_dev: nseed*neigv doubles, status_dev: nseed

@@ -24,8 +24,9 @@
  * The disorder generator is NOT the reference's (gfortran's random_number
  * stream, src/random_normal.f90:40-67, is compiler-runtime specific); both the
  * oracle and the GPU library use the counter-based Philox4x32-10 generator of
- * Salmon et al. (SC'11) keyed by (seed, ix, iy), followed by the reference's
- * Box-Muller cosine branch (src/random_normal.f90:63-65).
+ * Salmon et al. (SC'11) keyed by (seed, ix, iy / 2), followed by the reference's
+ * Box-Muller transform (src/random_normal.f90:63-65), both of its outputs: the
+ * cosine one for even iy, the sine one for odd iy.
  *
  * Conventions: matrices are row-major, element (i, j) of an m x n matrix at
  * [i*n + j]; indices inside the code are 0-based, comments cite the
@@ -187,31 +188,34 @@ void rgo_philox4x32_10(const uint32_t *ctr_in, const uint32_t *key_in, uint32_t 
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-/* Key and counter of the disorder cell (ix, iy) of realization `seed` (0-based ix, iy).
+/* Key and counter of the disorder cells (ix, 2p) and (ix, 2p + 1) of realization `seed` (0-based ix, p).
  * stream 0 = disorder potential. */
-static void rgo_cell_key(uint64_t seed, uint32_t stream, int ix, int iy, uint32_t *ctr, uint32_t *key)
+static void rgo_cell_key(uint64_t seed, uint32_t stream, int ix, int p, uint32_t *ctr, uint32_t *key)
 {
-    ctr[0] = (uint32_t)iy; ctr[1] = (uint32_t)ix; ctr[2] = stream; ctr[3] = 0u;
+    ctr[0] = (uint32_t)p; ctr[1] = (uint32_t)ix; ctr[2] = stream; ctr[3] = 0u;
     key[0] = (uint32_t)seed; key[1] = (uint32_t)(seed >> 32) ^ 0x52474E31u;
 }
 
-/* Two 53-bit uniforms from one Philox block, then the Box-Muller cosine branch of
- * src/random_normal.f90:64: sqrt(-2 log u1) cos(2 pi u2), u1 in (0,1], u2 in [0,1). */
-double rgo_normal_from_bits(const uint32_t *r)
+/* Two 53-bit uniforms from one Philox block, then the Box-Muller transform (the reference keeps its cosine
+ * output only, src/random_normal.f90:64; both outputs are used here, for two cells):
+ * sqrt(-2 log u1) cos(2 pi u2) (odd = 0) or sqrt(-2 log u1) sin(2 pi u2) (odd = 1), u1 in (0,1], u2 in [0,1). */
+double rgo_normal_from_bits(const uint32_t *r, int odd)
 {
     uint64_t a = ((uint64_t)(r[0] >> 5) << 26) | (uint64_t)(r[1] >> 6);
     uint64_t b = ((uint64_t)(r[2] >> 5) << 26) | (uint64_t)(r[3] >> 6);
     double u1 = (double)(a + 1) * 0x1.0p-53;
     double u2 = (double)b * 0x1.0p-53;
-    return sqrt(-2.0 * log(u1)) * cos(2.0 * RGO_PI * u2);
+    double th = 2.0 * RGO_PI * u2;
+    return sqrt(-2.0 * log(u1)) * (odd ? sin(th) : cos(th));
 }
 
+/* Cell (ix, iy): the Philox block of the pair p = iy / 2, the cosine output for even iy, the sine for odd. */
 double rgo_normal_cell(uint64_t seed, uint32_t stream, int ix, int iy)
 {
     uint32_t ctr[4], key[2], r[4];
-    rgo_cell_key(seed, stream, ix, iy, ctr, key);
+    rgo_cell_key(seed, stream, ix, iy >> 1, ctr, key);
     rgo_philox4x32_10(ctr, key, r);
-    return rgo_normal_from_bits(r);
+    return rgo_normal_from_bits(r, iy & 1);
 }
 
 /* upot(ix, iy) = stdu * xi(seed, ix, iy): src/waveguide.f90:366-372 with the Philox stream. */

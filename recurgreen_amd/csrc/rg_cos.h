@@ -1,4 +1,4 @@
-// rg_cos.h -- cos x for 0 <= x < 2 pi with no lane branches: the angle 2 pi u2 of the Box-Muller
+// rg_cos.h -- sin x and cos x for 0 <= x < 2 pi with no lane branches: the angle 2 pi u2 of the Box-Muller
 // transform of the disorder generator.  The library's cos carries a large-argument reduction behind a
 // branch that never runs for this range but splits the code into basic blocks, so that the generator
 // cannot be interleaved with matrix-core products.  RG_HD defaults to __host__ __device__; a host-only
@@ -11,14 +11,14 @@
 // over doubles below 2 pi is 6.1e-17 (x = RN(pi/2)); the rounding of k C2 and the omitted remainder of pi/2
 // are below 2^-159.  Kernels on |rh| <= pi/4 + 2^-48: the Taylor series of sin to r^17 and of cos to r^18
 // (first omitted terms below 2^-62 relative), with the first-order terms of rl and the exact error of rh^2.  Then
-// cos x = cos r, -sin r, -cos r, sin r for k mod 4 = 0, 1, 2, 3, selected without branching.
+// (sin x, cos x) from (sin r, cos r) by the quadrant k mod 4, selected without branching.
 #pragma once
 
 #ifndef RG_HD
 #define RG_HD __host__ __device__
 #endif
 
-RG_HD inline double rg_cos_0_2pi(double x)
+RG_HD inline void rg_sincos_0_2pi(double x, double *sinx, double *cosx)
 {
 #pragma clang fp contract(off)  // the two-sums must round each operation as written
     const double C0 = 0x1.921fb54442d18p+0, C1 = 0x1.1a62633145c07p-54, C2 = -0x1.f1976b7ed8fbcp-110;
@@ -32,14 +32,14 @@ RG_HD inline double rg_cos_0_2pi(double x)
 
     const double z = rh * rh, ze = fma(rh, rh, -z);  // rh^2 = z + ze
     // sin r = rh + rh z S(z) + rl (1 - z/2), S(z) = sum_{n>=1} (-1)^n z^(n-1) / (2n+1)!
-    double sp = 0x1.952c77030ad4ap-49;                                    // 1/17!
-    sp = fma(sp, z, -0x1.ae7f3e733b81fp-41);                             // -1/15!
-    sp = fma(sp, z, 0x1.6124613a86d09p-33);                              // 1/13!
-    sp = fma(sp, z, -0x1.ae64567f544e4p-26);                             // -1/11!
-    sp = fma(sp, z, 0x1.71de3a556c734p-19);                              // 1/9!
-    sp = fma(sp, z, -0x1.a01a01a01a01ap-13);                             // -1/7!
-    sp = fma(sp, z, 0x1.1111111111111p-7);                               // 1/5!
-    sp = fma(sp, z, -0x1.5555555555555p-3);                              // -1/3!
+    double sp = 0x1.952c77030ad4ap-49;           // 1/17!
+    sp = fma(sp, z, -0x1.ae7f3e733b81fp-41);    // -1/15!
+    sp = fma(sp, z, 0x1.6124613a86d09p-33);     // 1/13!
+    sp = fma(sp, z, -0x1.ae64567f544e4p-26);    // -1/11!
+    sp = fma(sp, z, 0x1.71de3a556c734p-19);     // 1/9!
+    sp = fma(sp, z, -0x1.a01a01a01a01ap-13);    // -1/7!
+    sp = fma(sp, z, 0x1.1111111111111p-7);      // 1/5!
+    sp = fma(sp, z, -0x1.5555555555555p-3);     // -1/3!
     const double sinr = rh + fma(rh * z, sp, rl * fma(-0.5, z, 1.0));
     // cos r = 1 - z/2 + z^2 C(z) - rh rl - ze/2, C(z) = sum_{n>=2} (-1)^n z^(n-2) / (2n)!
     double cp = -0x1.6827863b97d97p-53;          // -1/18!
@@ -54,7 +54,9 @@ RG_HD inline double rg_cos_0_2pi(double x)
     const double tail = ((1.0 - w) - hz) + (fma(z * z, cp, -rh * rl) - 0.5 * ze);
     const double cosr = w + tail;
 
+    // x = r + q pi/2: (sin x, cos x) = (sin r, cos r), (cos r, -sin r), (-sin r, -cos r), (-cos r, sin r)
     const int q = (int)k & 3;
-    const double v = (q & 1) ? sinr : cosr;
-    return (q == 1 || q == 2) ? -v : v;
+    const double a = (q & 1) ? cosr : sinr, c = (q & 1) ? sinr : cosr;
+    *sinx = (q == 2 || q == 3) ? -a : a;
+    *cosx = (q == 1 || q == 2) ? -c : c;
 }

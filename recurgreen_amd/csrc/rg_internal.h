@@ -364,16 +364,30 @@ __host__ __device__ inline void rg_philox4x32_10(uint32_t c[4], uint32_t k0, uin
     }
 }
 
-// Standard normal of disorder cell (ix, iy) of realization `seed`: two 53-bit uniforms from one
-// Philox block, u1 in (0,1], u2 in [0,1), then sqrt(-2 log u1) cos(2 pi u2) -- the cosine branch
-// of the Box-Muller transform used by src/random_normal.f90:64.
-__host__ __device__ inline double rg_normal_cell(uint64_t seed, uint32_t stream, int ix, int iy)
+// Standard normals of disorder cells (ix, 2p) and (ix, 2p + 1) of realization `seed`: two 53-bit uniforms
+// from the Philox block of counter (p, ix, stream, 0), u1 in (0,1], u2 in [0,1), then both outputs of the
+// Box-Muller transform, z0 = sqrt(-2 log u1) cos(2 pi u2) and z1 = sqrt(-2 log u1) sin(2 pi u2) (the
+// reference keeps the cosine output only, src/random_normal.f90:64): one block, logarithm and square root
+// per two cells.
+__host__ __device__ inline void rg_normal_pair(uint64_t seed, uint32_t stream, int ix, int p, double &z0, double &z1)
 {
-    uint32_t c[4] = {(uint32_t)iy, (uint32_t)ix, stream, 0u};
+    uint32_t c[4] = {(uint32_t)p, (uint32_t)ix, stream, 0u};
     rg_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32) ^ 0x52474E31u);
     uint64_t a = ((uint64_t)(c[0] >> 5) << 26) | (uint64_t)(c[1] >> 6);
     uint64_t b = ((uint64_t)(c[2] >> 5) << 26) | (uint64_t)(c[3] >> 6);
     double u1 = (double)(a + 1) * 0x1.0p-53;
     double u2 = (double)b * 0x1.0p-53;
-    return sqrt(-2.0 * log(u1)) * rg_cos_0_2pi(2.0 * 3.14159265358979323846 * u2);
+    const double r = sqrt(-2.0 * log(u1));
+    double s, co;
+    rg_sincos_0_2pi(2.0 * 3.14159265358979323846 * u2, &s, &co);
+    z0 = r * co;
+    z1 = r * s;
+}
+
+// Cell (ix, iy): the cosine output of pair iy / 2 for even iy, the sine output for odd iy.
+__host__ __device__ inline double rg_normal_cell(uint64_t seed, uint32_t stream, int ix, int iy)
+{
+    double z0, z1;
+    rg_normal_pair(seed, stream, ix, iy >> 1, z0, z1);
+    return (iy & 1) ? z1 : z0;
 }

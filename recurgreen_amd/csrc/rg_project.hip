@@ -32,15 +32,17 @@ constexpr int PR_TPW = (PR_MAXNT + 3) / 4;
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-// Disorder of realization b, cell iy of slice ixd: stdu N(seed, ixd, iy) with the seed seed0 + b (or
-// *seedp + seed0 + b in a captured sweep, or seeds[b] and stdus[b] in an uneven batch).
-__device__ __forceinline__ double noise_cell(int b, int ixd, int iy, int64_t seed0, const int64_t *__restrict__ seedp,
-                                             double stdu, const int64_t *__restrict__ seeds,
-                                             const double *__restrict__ stdus)
+// Disorder of realization b, cells 2 p and 2 p + 1 of slice ixd: stdu N(seed, ixd, 2 p + {0, 1}) with the seed
+// seed0 + b (or *seedp + seed0 + b in a captured sweep, or seeds[b] and stdus[b] in an uneven batch).
+__device__ __forceinline__ void noise_pair(int b, int ixd, int p, int64_t seed0, const int64_t *__restrict__ seedp,
+                                           double stdu, const int64_t *__restrict__ seeds,
+                                           const double *__restrict__ stdus, double &z0, double &z1)
 {
     const int64_t s = seeds ? seeds[b] : (seedp ? *seedp + seed0 : seed0) + b;
     const double sd = stdus ? stdus[b] : stdu;
-    return sd * rg_normal_cell((uint64_t)s, 0u, ixd, iy);
+    rg_normal_pair((uint64_t)s, 0u, ixd, p, z0, z1);
+    z0 *= sd;
+    z1 *= sd;
 }
 
 // One chunk of slice values into s_u[KC][RB]: two cells per thread, u(b, iy) = us[b sb + iy si].
@@ -57,16 +59,21 @@ __device__ inline void stage_chunk(double (*s_u)[PR_RB + 1], int chunk, int b0, 
 }
 
 // Disorder of one slice for a batch of seeds: us[b ny + iy] = stdu * N(seed0 + b, ixd, iy), the
-// Philox4x32-10 / Box-Muller normal of the cell (generated here, at full occupancy, so that the
-// projection kernel's matrix cores are not held up by the transcendental functions).  With
-// seedp != null the first seed is *seedp + seed0 (a captured sweep reads it from device memory).
-// An uneven batch gives every realization its own seed and strength: seeds[b], stdus[b].
+// Philox4x32-10 / Box-Muller normals of the cells, two per thread (one block and Box-Muller pair; generated
+// here, at full occupancy, so that the projection kernel's matrix cores are not held up by the
+// transcendental functions).  With seedp != null the first seed is *seedp + seed0 (a captured sweep reads it
+// from device memory).  An uneven batch gives every realization its own seed and strength: seeds[b],
+// stdus[b].  Grid: (ceil(ny / 512), nb).
 __global__ __launch_bounds__(256) void k_noise_slice(int ixd, int64_t seed0, const int64_t *__restrict__ seedp, int ny,
                                                      double stdu, const int64_t *__restrict__ seeds,
                                                      const double *__restrict__ stdus, double *__restrict__ us)
 {
-    const int b = blockIdx.y, iy = blockIdx.x * 256 + threadIdx.x;
-    if (iy < ny) us[(size_t)b * ny + iy] = noise_cell(b, ixd, iy, seed0, seedp, stdu, seeds, stdus);
+    const int b = blockIdx.y, p = blockIdx.x * 256 + threadIdx.x, iy = 2 * p;
+    if (iy >= ny) return;
+    double z0, z1;
+    noise_pair(b, ixd, p, seed0, seedp, stdu, seeds, stdus, z0, z1);
+    us[(size_t)b * ny + iy] = z0;
+    if (iy + 1 < ny) us[(size_t)b * ny + iy + 1] = z1;
 }
 
 // The projection of large batches from slice values in memory (explicit potentials, or the disorder of
@@ -248,7 +255,7 @@ void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double 
             return;
         }
         if (!d_upot) {
-            hipLaunchKernelGGL(k_noise_slice, dim3((P->ny + 255) / 256, nb), dim3(256), 0, P->stream, ixd, seed0,
+            hipLaunchKernelGGL(k_noise_slice, dim3((P->ny + 511) / 512, nb), dim3(256), 0, P->stream, ixd, seed0,
                                P->seed_dev, P->ny, P->stdu, P->d_seeds_r, P->d_stdu_r, P->d_us);
             us = P->d_us;
             sb = P->ny;
@@ -270,7 +277,7 @@ void rg_launch_project(rg_plan *P, int nb, int ixd, int64_t seed0, const double 
         return;
     }
     if (!d_upot) {  // seeded: this slice's disorder first
-        hipLaunchKernelGGL(k_noise_slice, dim3((P->ny + 255) / 256, nb), dim3(256), 0, P->stream, ixd, seed0,
+        hipLaunchKernelGGL(k_noise_slice, dim3((P->ny + 511) / 512, nb), dim3(256), 0, P->stream, ixd, seed0,
                            P->seed_dev, P->ny, P->stdu, P->d_seeds_r, P->d_stdu_r, P->d_us);
         us = P->d_us;
         sb = P->ny;

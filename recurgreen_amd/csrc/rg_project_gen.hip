@@ -16,8 +16,8 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // The projection of large seeded batches with the disorder generated in the kernel: 12 waves per
 // workgroup of 64 realizations, three per SIMD.  Wave w multiplies rows 16 (w mod 4).. against column third
 // w / 4 of the table (tiles NT g / 3 .. NT (g + 1) / 3 - 1); waves generate the next chunk's 16 cells of the
-// 64 realizations into LDS (wave w: cell w, waves 0..3 also cell 12 + w: four cells per SIMD), which the
-// three thirds read, so each cell is generated once.  The generator's FP64 arithmetic does not overlap the
+// 64 realizations into LDS (waves 0..7: the two cells 2 w, 2 w + 1 of one Philox block and Box-Muller pair;
+// two pairs per SIMD), which the three thirds read, so each cell is generated once.  The generator's FP64 arithmetic does not overlap the
 // FP64 products (they share the FP64 datapath: the vector and matrix FP64 peaks are equal): at 4096 x N=128
 // (ny = 1285) the products alone take 62 us per slice, the generator 37-41 us more, in this kernel or with
 // the generator inside the 128-realization one alike; this kernel's gain (97 against 101 us) comes from
@@ -63,20 +63,19 @@ __global__ __launch_bounds__(768, 1) void k_project_gen(int ny, int ixd, int64_t
     const int gbc = gok ? gb : nb - 1;
     const uint64_t seed = (uint64_t)(seeds ? seeds[gbc] : (seedp ? *seedp + seed0 : seed0) + gbc);
     const double sd = stdus ? stdus[gbc] : stdu;
-    auto generate_cell = [&](int c, int buf, int cell) {
-        const int iy = c * PR_KC + cell;
-        // the key schedule derived from the seed is recomputed per cell rather than hoisted out of the chunk
+    // wave w < 8: the cells 2 w, 2 w + 1 of chunk c (one Philox block and Box-Muller pair)
+    auto generate = [&](int c, int buf) {
+        if (w >= PR_KC / 2) return;  // uniform per wavefront
+        const int iy = c * PR_KC + 2 * w;
+        // the key schedule derived from the seed is recomputed per chunk rather than hoisted out of the chunk
         // loop, where its registers would stay live beside the accumulators
         uint64_t sj = seed;
         asm volatile("" : "+v"(sj));
-        // a product with the 0/1 mask rather than a select, which the compiler turns into a branch
-        s_a[buf][cell][lane] = (sd * rg_normal_cell(sj, 0u, ixd, iy)) * ((gok && iy < ny) ? 1.0 : 0.0);
-    };
-    auto generate = [&](int c, int buf) {
-        generate_cell(c, buf, w);
-        __builtin_amdgcn_sched_barrier(0);  // one cell's registers at a time
-        if (w < PR_KC - 12) generate_cell(c, buf, 12 + w);
-        __builtin_amdgcn_sched_barrier(0);
+        double z0, z1;
+        rg_normal_pair(sj, 0u, ixd, iy >> 1, z0, z1);
+        // products with the 0/1 masks rather than selects, which the compiler turns into branches
+        s_a[buf][2 * w][lane] = (sd * z0) * ((gok && iy < ny) ? 1.0 : 0.0);
+        s_a[buf][2 * w + 1][lane] = (sd * z1) * ((gok && iy + 1 < ny) ? 1.0 : 0.0);
     };
     // table rows 16 c .. 16 c + 15 straight into LDS (the 16 bytes of lane l at the wave's LDS base
     // + 16 l, so the staged rows are contiguous), over the 12 waves
