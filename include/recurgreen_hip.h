@@ -222,6 +222,13 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *                   on the wide kernels (128 or 64 realizations per workgroup, the table staged
  *                   straight into LDS, seeded disorder generated in the kernel up to 128 modes);
  *                   smaller ones on the 32-realization kernel after a separate noise kernel;
+ *   "refine"        0..8 (default 0): complex64 sweeps of uniform batches refine G(L;0) in FP64, the
+ *                   iterative-refinement correction of the mixed-precision sweep: every slice's
+ *                   complex64 inverse is kept, and X = H^-1 E_1 (H the block-tridiagonal matrix of
+ *                   src/waveguide.f90:424-440) is corrected `refine` times by the FP64 residual
+ *                   E_1 - H X solved with those inverses; the FP64 eigensolver follows.  Passes hold
+ *                   3 nx np^2 complex128 per realization (the pass size follows from the free device
+ *                   memory); slower than a complex128 sweep at N = 128 (see rg_refine.hip);
  *   "lanes"         1 one stream, 2 the batch split in halves on two streams;
  *   "eigen"         transmission eigenvalues: 0 eigenvalues of t^H t (Householder tridiagonalisation
  *                   and Sturm bisection); 1 one-sided Jacobi SVD of t; 2 Householder bidiagonalisation

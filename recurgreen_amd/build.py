@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librecurgreen_hip.so")
-SOURCES = ["rg_plan.hip", "rg_svd.hip", "rg_kernels_f64.hip", "rg_blocked.hip", "rg_gj.hip", "rg_project.hip", "rg_eig.hip", "rg_zgemm.hip", "rg_refl.hip", "rg_adjoint.hip", "rg_nopiv.hip", "rg_dyson32.hip", "rg_schur.hip", "rg_tprofile.hip", "rg_dyson64.hip", "rg_dyson_bf3.hip", "rg_noalloc.hip", "rg_nopiv64.hip", "rg_project_gen.hip"]
+SOURCES = ["rg_plan.hip", "rg_svd.hip", "rg_kernels_f64.hip", "rg_blocked.hip", "rg_gj.hip", "rg_project.hip", "rg_eig.hip", "rg_zgemm.hip", "rg_refl.hip", "rg_adjoint.hip", "rg_nopiv.hip", "rg_dyson32.hip", "rg_schur.hip", "rg_tprofile.hip", "rg_dyson64.hip", "rg_dyson_bf3.hip", "rg_noalloc.hip", "rg_nopiv64.hip", "rg_project_gen.hip", "rg_refine.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops is switched off for every HIP object: the v_pk_*_f32 forms are avoided.

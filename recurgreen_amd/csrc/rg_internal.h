@@ -166,6 +166,11 @@ struct rg_plan {
     // of recomputed states, five adjoint matrices, the gradient staging, for adj_cap realizations
     int adj_cap = 0, adj_seg = 0, adj_nck = 0;
     int adj_prec = 0;  // RG_FP64: complex128 adjoint; RG_MIXED_FP32: complex64 state and products
+    // FP64 refinement of the complex64 sweeps (plan option refine: passes after the first): for passes of
+    // ref_cap realizations, every slice's g_x, X and R (complex128, np x np) and coefficients; ref_vcap V blocks
+    int refine = 0, ref_cap = 0, ref_vcap = 0;
+    double2 *d_ref_g = nullptr, *d_ref_x = nullptr, *d_ref_r = nullptr, *d_ref_w = nullptr, *d_ref_v = nullptr;
+    double *d_ref_cq = nullptr;
     double2 *d_ck = nullptr, *d_seg = nullptr, *d_adj = nullptr;
     double *d_loss = nullptr, *d_grad = nullptr;
     double2 *d_bconj = nullptr;  // conj(b) (np)
@@ -321,6 +326,14 @@ void rg_launch_gemm_sp(rg_plan *P, int nb, const float2 *gss, const float2 *gs1_
 // rg_schur.hip: slice inversion for 128 < np <= 256 (2 x 2 Schur complement on the smaller kernels);
 // rg_schur_elems: complex128 elements of block work area per realization.
 void rg_launch_inverse_schur(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision);
+// C = alpha A B + beta C per realization, complex128 (rg_schur.hip)
+void rg_zgemm(rg_plan *P, int nb, int m, int n, int k, double2 alpha, const double2 *A, int lda, size_t sA,
+              const double2 *B, int ldb, size_t sB, double2 beta, double2 *C, int ldc, size_t sC);
+// FP64 refinement of the complex64 sweeps (rg_refine.hip): buffers (sets ref_cap), the store of slice ix
+// after its step, and the refinement passes, returning G(L;0) (nb x np x np complex128)
+rg_status rg_refine_reserve(rg_plan *P);
+void rg_refine_store(rg_plan *P, int nb, int ix, bool with_disorder);
+const double2 *rg_refine_solve(rg_plan *P, int nb);
 size_t rg_schur_elems(int np);
 // rg_dyson32.hip: the FP32 Dyson product with one realization per workgroup; false if NP is not
 // one of 80, 96, 112, 128

@@ -55,7 +55,8 @@ void free_plan(rg_plan *P)
                     P->d_gss32, P->d_gs1_32[0], P->d_gs1_32[1], P->d_g11, P->d_g1s[0], P->d_g1s[1],
                     P->d_sqrtremu, P->d_redo, P->d_us, P->d_ck, P->d_seg, P->d_adj, P->d_loss, P->d_grad, P->d_bconj,
                     P->d_schur, P->d_schur_zero, P->d_schur_i, P->d_tp_g, P->d_tp_z, P->d_tp_x, P->d_tp_v,
-                    P->d_tp_w, P->d_tp_y[0], P->d_tp_y[1], P->d_tp_t, P->d_tp_i, P->d_tp_psi, P->d_adj_work};
+                    P->d_tp_w, P->d_tp_y[0], P->d_tp_y[1], P->d_tp_t, P->d_tp_i, P->d_tp_psi, P->d_adj_work,
+                    P->d_ref_g, P->d_ref_x, P->d_ref_r, P->d_ref_w, P->d_ref_v, P->d_ref_cq};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_graphs(P);
@@ -428,9 +429,43 @@ bool run_sweep_graph(rg_plan *P, int nb, int64_t seed0, const double *d_upot, in
 // split in halves on two streams, slice launches interleaved and the second lane started one slice
 // inversion behind the first, so that one half's latency-bound inversion shares the compute units
 // with the other half's Dyson products.
+// A complex64 sweep with FP64 refinement: the sweep keeps every slice's inverse and coefficients, the
+// refinement passes (rg_refine.hip) give G(L;0) to FP64 accuracy, and the FP64 Fisher-Lee relation and
+// eigenvalues follow.  nb <= ref_cap.
+static void run_sweep_refined(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int precision, double *teig,
+                       double2 *tmat_out)
+{
+    int cur = 0;
+    for (int ix = 1; ix <= P->nx; ix++) {
+        sweep_slice(P, nb, ix, seed0, d_upot, precision, &cur);
+        const int id = ix - P->nxfree;
+        rg_refine_store(P, nb, ix, id >= 1 && id <= P->nxdiso);
+    }
+    const double2 *gl0 = rg_refine_solve(P, nb);
+    if (tmat_out) rg_launch_fisher_lee_svd_f64(P, nb, gl0, teig, P->d_status, tmat_out);
+    else rg_launch_teig_f64(P, nb, gl0, teig, P->d_status);
+}
+
+static bool refined(const rg_plan *P, int precision) { return P->refine > 0 && precision != RG_FP64; }
+
+// Realizations per pass: `base`, or fewer where the refinement's per-slice store sets the limit.
+static rg_status pass_cap(rg_plan *P, int precision, int base, int *cap)
+{
+    *cap = base;
+    if (!refined(P, precision)) return RG_OK;
+    rg_status st = rg_refine_reserve(P);
+    if (st) return st;
+    *cap = std::min(base, P->ref_cap);
+    return RG_OK;
+}
+
 void run_sweep(rg_plan *P, int nb, int64_t seed0, const double *d_upot, int precision, double *teig,
                double2 *tmat_out)
 {
+    if (refined(P, precision)) {
+        run_sweep_refined(P, nb, seed0, d_upot, precision, teig, tmat_out);
+        return;
+    }
     if (P->use_graph && !P->profiling && !tmat_out && (P->lanes < 2 || nb < 2 * kLaneMin) &&
         run_sweep_graph(P, nb, seed0, d_upot, precision, teig))
         return;
@@ -1170,8 +1205,10 @@ rg_status rg_tdistrib_device(rg_plan *P, int64_t seed0, int32_t nseed, int32_t p
     if (st) return st;
     if (!tallvals_dev && nseed > 0) return RG_EINVAL;
     DeviceGuard g(P->device);
-    for (int32_t off = 0; off < nseed; off += P->batch_max) {
-        const int nb = std::min(P->batch_max, nseed - off);
+    int cap = 0;
+    if ((st = pass_cap(P, precision, P->batch_max, &cap))) return st;
+    for (int32_t off = 0; off < nseed; off += cap) {
+        const int nb = std::min(cap, nseed - off);
         HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
         run_sweep(P, nb, seed0 + off, nullptr, precision, tallvals_dev + (size_t)off * P->neigv, nullptr);
         if (status_dev)
@@ -1187,9 +1224,10 @@ rg_status rg_tdistrib(rg_plan *P, int64_t seed0, int32_t nseed, int32_t precisio
     if (st) return st;
     if (!tallvals && nseed > 0) return RG_EINVAL;
     DeviceGuard g(P->device);
-    int32_t worst = 0;
-    for (int32_t off = 0; off < nseed; off += P->batch_max) {
-        const int nb = std::min(P->batch_max, nseed - off);
+    int32_t worst = 0, cap = 0;
+    if ((st = pass_cap(P, precision, P->batch_max, &cap))) return st;
+    for (int32_t off = 0; off < nseed; off += cap) {
+        const int nb = std::min(cap, nseed - off);
         HIPCHK(hipMemsetAsync(P->d_status, 0, sizeof(int32_t) * nb, P->stream));
         run_sweep(P, nb, seed0 + off, nullptr, precision, P->d_teig, nullptr);
         HIPCHK(hipGetLastError());
@@ -1324,7 +1362,8 @@ static rg_status potential_pass(rg_plan *P, const double *upot, int32_t nreal, i
     DeviceGuard g(P->device);
     const size_t per = (size_t)P->nxdiso * P->ny;
     const int m = P->imax - P->imin + 1, nn = P->jmax - P->jmin + 1;
-    const int cap = std::min(P->upot_cap, P->batch_max);
+    int cap = 0;
+    if ((st = pass_cap(P, precision, std::min(P->upot_cap, P->batch_max), &cap))) return st;
     int32_t worst = 0;
     for (int32_t off = 0; off < nreal; off += cap) {
         const int nb = std::min(cap, nreal - off);
@@ -1758,6 +1797,9 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
     } else if (std::strcmp(key, "project_wide_min") == 0) {
         if (value < 1) return RG_EINVAL;
         P->project_wide_min = value;
+    } else if (std::strcmp(key, "refine") == 0) {
+        if (value < 0 || value > 8) return RG_EINVAL;
+        P->refine = value;
     } else if (std::strcmp(key, "lanes") == 0) {
         if (value != 1 && value != 2) return RG_EINVAL;
         if (value == 2) {
@@ -1791,6 +1833,7 @@ rg_status rg_plan_get_option(const rg_plan *P, const char *key, int32_t *value)
     else if (std::strcmp(key, "dyson") == 0) *value = P->gemm_wg;
     else if (std::strcmp(key, "project_parts") == 0) *value = P->cq_parts;
     else if (std::strcmp(key, "project_wide_min") == 0) *value = P->project_wide_min;
+    else if (std::strcmp(key, "refine") == 0) *value = P->refine;
     else if (std::strcmp(key, "lanes") == 0) *value = P->lanes;
     else if (std::strcmp(key, "eigen") == 0) *value = P->eigen;
     else if (std::strcmp(key, "profile_mask") == 0) *value = P->profile_mask;

@@ -485,6 +485,12 @@ size_t rg_schur_elems(int np)
     return n1 * n1 + 4 * n1 * n2 + n2 * n2;  // A, B, C, D, X, Y (complex128 elements per realization)
 }
 
+void rg_zgemm(rg_plan *P, int nb, int m, int n, int k, double2 alpha, const double2 *A, int lda, size_t sA,
+              const double2 *B, int ldb, size_t sB, double2 beta, double2 *C, int ldc, size_t sC)
+{
+    cgemm<double2>(P, nb, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+}
+
 void rg_launch_inverse_schur(rg_plan *P, int nb, int slot, int first, bool with_disorder, int precision)
 {
     hipEvent_t stop = nullptr;

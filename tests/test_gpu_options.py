@@ -21,13 +21,14 @@ def test_option_round_trip_and_rejection():
     say("options round trip")
     plan = binding.Plan(CASES["c1_N20_ny205"], batch_max=2)
     for key, values in (("inverse_fp64", (0, 1, 2, 3, 4)), ("inverse_mixed", (1, 2, 3, 4)), ("dyson", (0, 1, 2, 3)),
-                        ("project_parts", (1, 3, 8)), ("project_wide_min", (4096, 1)), ("lanes", (2, 1)),
+                        ("project_parts", (1, 3, 8)), ("project_wide_min", (4096, 1)), ("refine", (2, 8, 0)),
+                        ("lanes", (2, 1)),
                         ("eigen", (1, 0, 2, 3))):
         for v in values:
             plan.set_option(key, v)
             assert plan.get_option(key) == v
     for key, v in (("inverse_fp64", 5), ("inverse_mixed", 0), ("dyson", 4), ("project_parts", 9),
-                   ("project_wide_min", 0), ("lanes", 3), ("eigen", 4), ("nonsense", 0)):
+                   ("project_wide_min", 0), ("refine", 9), ("refine", -1), ("lanes", 3), ("eigen", 4), ("nonsense", 0)):
         with pytest.raises(binding.RecurGreenError):
             plan.set_option(key, v)
     plan.close()
