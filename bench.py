@@ -314,6 +314,51 @@ def tprofile_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def refine_bench(args):
+    """The complex64 sweep with `--refine K` FP64 refinement passes of G(L;0) (plan option refine) on the
+    N=128 workload, beside the complex128 and plain complex64 sweeps of the same batch (192 realizations by
+    default: the refinement keeps 3 nx N^2 complex128 per realization, 0.79 GB at nx = 1000, so passes hold
+    about 200); each precision timed over one call after a warmup call, host copies included.  The refined
+    eigenvalues are compared with the complex128 sweep's."""
+    import numpy as np
+
+    from recurgreen_amd import binding
+
+    params = workload_params(args)
+    B = args.batch
+    plan = binding.Plan(params, batch_max=B)
+    res = {}
+    for label, prec, passes in (("fp64", "fp64", 0), ("mixed_fp32", "mixed_fp32", 0),
+                                (f"mixed_fp32_refine{args.refine}", "mixed_fp32", args.refine)):
+        plan.set_option("refine", passes)
+        plan.tdistrib(1, B, prec)
+        t0 = time.perf_counter()
+        T, st, code = plan.tdistrib(1, B, prec)
+        dt = time.perf_counter() - t0
+        res[label] = (B / dt, T.reshape(B, -1), int((st != 0).sum()))
+    plan.close()
+    T64 = res["fp64"][1]
+    err = {}
+    for label in res:
+        if label == "fp64":
+            continue
+        T = res[label][1]
+        tmax = T64.max(axis=1, keepdims=True)
+        ok = T64 >= 1e-11 * tmax
+        bound = 1e-10 * T64 + 2e-13 * np.sqrt(T64 * tmax) + 1e-26 * tmax
+        err[label] = {"max_rel_err_resolved": float(np.max(np.abs(T - T64)[ok] / T64[ok])),
+                      "max_err_over_fp64_bound": float(np.max(np.abs(T - T64) / bound)),
+                      "mean_T_shift": float(T.mean() - T64.mean())}
+    label = f"mixed_fp32_refine{args.refine}"
+    line = {"metric": "realizations/s of the complex64 sweep with FP64 refinement of G(L;0)",
+            "value": res[label][0], "unit": "realizations/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": f"{args.config}: W={params['wol']} nx={params['nxdiso']} ny={params['ny']} batch {B}"},
+            "realizations_per_s": {k: v[0] for k, v in res.items()},
+            "failed_realizations": {k: v[2] for k, v in res.items()}, "against_fp64_sweep": err}
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(mib):
     """Plain device-to-device copies of `mib` MiB (torch's copy kernel): the reference kernel whose
     FETCH_SIZE / WRITE_SIZE counters scale the sweep kernels' counters to bytes."""
@@ -344,6 +389,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uneven", type=int, default=0, metavar="K",
                     help="measure an uneven batch of K realizations (rg_tdistrib_uneven) instead")
+    ap.add_argument("--refine", type=int, default=0, metavar="K",
+                    help="time the complex64 sweep with K FP64 refinement passes (batch 192 by default)")
     ap.add_argument("--adjoint", action="store_true", help="time the adjoint (gradient) sweep instead (c4 by default)")
     ap.add_argument("--tprofile", action="store_true",
                     help="time the transmission-eigenstate pass (rg_tprofile) instead (c4 by default)")
@@ -359,7 +406,10 @@ def main():
     if (args.adjoint or args.tprofile) and args.config == "n128":
         args.config = "c4"
     if args.batch is None:
-        args.batch = CONFIGS[args.config][3]
+        args.batch = 192 if args.refine else CONFIGS[args.config][3]
+    if args.refine:
+        refine_bench(args)
+        return
     if args.adjoint:
         adjoint_bench(args)
         return

@@ -332,7 +332,7 @@ void rg_zgemm(rg_plan *P, int nb, int m, int n, int k, double2 alpha, const doub
 // FP64 refinement of the complex64 sweeps (rg_refine.hip): buffers (sets ref_cap), the store of slice ix
 // after its step, and the refinement passes, returning G(L;0) (nb x np x np complex128)
 rg_status rg_refine_reserve(rg_plan *P);
-void rg_refine_store(rg_plan *P, int nb, int ix, bool with_disorder);
+void rg_refine_store(rg_plan *P, int nb, int ix, bool with_disorder, const float2 *gs1);
 const double2 *rg_refine_solve(rg_plan *P, int nb);
 size_t rg_schur_elems(int np);
 // rg_dyson32.hip: the FP32 Dyson product with one realization per workgroup; false if NP is not

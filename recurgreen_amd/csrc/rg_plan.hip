@@ -439,7 +439,7 @@ static void run_sweep_refined(rg_plan *P, int nb, int64_t seed0, const double *d
     for (int ix = 1; ix <= P->nx; ix++) {
         sweep_slice(P, nb, ix, seed0, d_upot, precision, &cur);
         const int id = ix - P->nxfree;
-        rg_refine_store(P, nb, ix, id >= 1 && id <= P->nxdiso);
+        rg_refine_store(P, nb, ix, id >= 1 && id <= P->nxdiso, P->d_gs1_32[cur]);
     }
     const double2 *gl0 = rg_refine_solve(P, nb);
     if (tmat_out) rg_launch_fisher_lee_svd_f64(P, nb, gl0, teig, P->d_status, tmat_out);

@@ -9,7 +9,8 @@
 //   y_1 = g_1 R_1,   y_x = g_x (R_x - b y_{x-1})       (forward, x = 2..nx)
 //   d_nx = y_nx,     d_x = y_x - g_x b d_{x+1}         (backward)
 //   X <- X + d
-// with the complex64 g_x taken into FP64 products.  The first pass is the complex64 sweep's own solution;
+// with the complex64 g_x taken into FP64 products.  The first pass is the complex64 sweep's own solution
+// (its forward half, y = Gs1 at every slice, kept from the sweep);
 // each further pass multiplies the error by about eps32 times the growth of the elimination, so X, and
 // G(L;0) = X_nx, reach the accuracy of an FP64 sweep of the same H in one or two passes.  The FP64
 // Fisher-Lee relation and eigenvalues follow (rg_plan.hip).
@@ -27,16 +28,19 @@ constexpr int kEw = 256;  // threads of the element-wise kernels
 
 int ew_blocks(size_t elems) { return (int)std::min<size_t>(64, (elems + kEw - 1) / kEw); }
 
-// g_x: the sweep's complex64 inverse to complex128; the slice's summed coefficients (zeros on slices
-// without disorder).  Grid (blocks over np^2, realizations).
-__global__ __launch_bounds__(kEw) void k_ref_store(const float2 *__restrict__ g32, double2 *__restrict__ g, const rg_cqv cqv,
-                                                   double *__restrict__ cq, int np2, int ntab)
+// g_x: the sweep's complex64 inverse to complex128; y_x = Gs1_x, the sweep's forward solution of
+// H X = E_1 (the first pass's forward half), to complex128 on the n x n block (zero padding); the
+// slice's summed coefficients (zeros on slices without disorder).  Grid (blocks over np^2, realizations).
+__global__ __launch_bounds__(kEw) void k_ref_store(const float2 *__restrict__ g32, double2 *__restrict__ g,
+                                                   const float2 *__restrict__ y32, double2 *__restrict__ y,
+                                                   const rg_cqv cqv, double *__restrict__ cq, int n, int np, int ntab)
 {
-    const int b = blockIdx.y, tid = threadIdx.x;
+    const int b = blockIdx.y, tid = threadIdx.x, np2 = np * np;
     const size_t o = (size_t)b * np2;
     for (int e = blockIdx.x * kEw + tid; e < np2; e += gridDim.x * kEw) {
-        const float2 v = g32[o + e];
+        const float2 v = g32[o + e], w = y32[o + e];
         g[o + e] = cmk((double)v.x, (double)v.y);
+        y[o + e] = (e / np < n && e % np < n) ? cmk((double)w.x, (double)w.y) : cmk(0.0, 0.0);
     }
     if (blockIdx.x == 0) {
         double *dst = cq + (size_t)b * ntab;
@@ -145,12 +149,13 @@ rg_status rg_refine_reserve(rg_plan *P)
     return RG_OK;
 }
 
-void rg_refine_store(rg_plan *P, int nb, int ix, bool with_disorder)
+void rg_refine_store(rg_plan *P, int nb, int ix, bool with_disorder, const float2 *gs1)
 {
     const size_t np2 = (size_t)P->np * P->np, x = (size_t)ix - 1;
     const rg_cqv cq = rg_cq_view(P, nb, with_disorder);
     hipLaunchKernelGGL(k_ref_store, dim3(ew_blocks(np2), nb), dim3(kEw), 0, P->stream, P->d_gss32,
-                       P->d_ref_g + x * nb * np2, cq, P->d_ref_cq + x * nb * P->ntab, (int)np2, P->ntab);
+                       P->d_ref_g + x * nb * np2, gs1, P->d_ref_r + x * nb * np2, cq, P->d_ref_cq + x * nb * P->ntab,
+                       P->nmu, P->np, P->ntab);
 }
 
 const double2 *rg_refine_solve(rg_plan *P, int nb)
@@ -162,22 +167,21 @@ const double2 *rg_refine_solve(rg_plan *P, int nb)
     const double2 *G = P->d_ref_g;
     const int gx = ew_blocks(np2), chunk = std::max(1, P->ref_vcap / nb);
     (void)hipMemsetAsync(X, 0, all * sizeof(double2), P->stream);
+    // pass 0: R holds the sweep's forward solution y (stored slice by slice); X = 0
     for (int pass = 0; pass <= P->refine; pass++) {
-        // R = E_1 - H X
-        for (int x0 = 0; x0 < nx; x0 += chunk) {
-            const int cnt = std::min(chunk, nx - x0), items = cnt * nb;
-            if (pass == 0) {
-                (void)hipMemsetAsync(R + x0 * slab, 0, (size_t)cnt * slab * sizeof(double2), P->stream);
-            } else {
+        if (pass > 0) {
+            // R = E_1 - H X
+            for (int x0 = 0; x0 < nx; x0 += chunk) {
+                const int cnt = std::min(chunk, nx - x0), items = cnt * nb;
                 hipLaunchKernelGGL(k_ref_vmat, dim3(gx, items), dim3(kEw), 0, P->stream,
                                    P->d_ref_cq + (size_t)x0 * nb * P->ntab, P->ntab, P->p.boundtype, n, np, P->d_ref_v);
                 rg_zgemm(P, items, n, n, n, one, P->d_ref_v, np, np2, X + x0 * slab, np, np2, zero, R + x0 * slab, np, np2);
+                hipLaunchKernelGGL(k_ref_resid, dim3(gx, items), dim3(kEw), 0, P->stream, R, X, P->d_adiag, P->d_b,
+                                   P->cstride, nb, x0, nx, n, np);
             }
-            hipLaunchKernelGGL(k_ref_resid, dim3(gx, items), dim3(kEw), 0, P->stream, R, X, P->d_adiag, P->d_b,
-                               P->cstride, nb, x0, nx, n, np);
         }
         // forward: R_x <- y_x
-        for (int x = 0; x < nx; x++) {
+        for (int x = 0; x < nx && pass > 0; x++) {
             hipLaunchKernelGGL(k_ref_rhs, dim3(gx, nb), dim3(kEw), 0, P->stream, R + x * slab,
                                x > 0 ? R + (x - 1) * slab : nullptr, P->d_b, P->cstride, W, n, np);
             rg_zgemm(P, nb, n, n, n, one, G + x * slab, np, np2, W, np, np2, zero, R + x * slab, np, np2);
