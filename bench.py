@@ -319,7 +319,8 @@ def refine_bench(args):
     N=128 workload, beside the complex128 and plain complex64 sweeps of the same batch (192 realizations by
     default: the refinement keeps 3 nx N^2 complex128 per realization, 0.79 GB at nx = 1000, so passes hold
     about 200); each precision timed over one call after a warmup call, host copies included.  The refined
-    eigenvalues are compared with the complex128 sweep's."""
+    eigenvalues are compared with the complex128 sweep's, with the rho(T) bin errors of accuracy() on the
+    linear and Chebyshev bins."""
     import numpy as np
 
     from recurgreen_amd import binding
@@ -336,6 +337,11 @@ def refine_bench(args):
         T, st, code = plan.tdistrib(1, B, prec)
         dt = time.perf_counter() - t0
         res[label] = (B / dt, T.reshape(B, -1), int((st != 0).sum()))
+    import torch
+
+    t64 = torch.from_numpy(np.ascontiguousarray(res["fp64"][1]).ravel()).cuda()
+    acc = {label: accuracy(plan, t64, torch.from_numpy(np.ascontiguousarray(v[1]).ravel()).cuda())
+           for label, v in res.items() if label != "fp64"}
     plan.close()
     T64 = res["fp64"][1]
     err = {}
@@ -355,7 +361,8 @@ def refine_bench(args):
             "dtype": "f32+f64", "data": "synthetic",
             "config": {"workload": f"{args.config}: W={params['wol']} nx={params['nxdiso']} ny={params['ny']} batch {B}"},
             "realizations_per_s": {k: v[0] for k, v in res.items()},
-            "failed_realizations": {k: v[2] for k, v in res.items()}, "against_fp64_sweep": err}
+            "failed_realizations": {k: v[2] for k, v in res.items()}, "against_fp64_sweep": err,
+            "accuracy_against_fp64_sweep": acc}
     print(json.dumps(line), flush=True)
 
 
