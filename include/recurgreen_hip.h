@@ -229,6 +229,8 @@ rg_status rg_plan_kernel_times(rg_plan *plan, int32_t cap, char *names, double *
  *                   E_1 - H X solved with those inverses; the FP64 eigensolver follows.  Passes hold
  *                   3 nx np^2 complex128 per realization (the pass size follows from the free device
  *                   memory); slower than a complex128 sweep at N = 128 (see rg_refine.hip);
+ *   "refine_cap"    >= 0 (default 0): at most this many realizations per refined pass (0: as many as
+ *                   60% of the free device memory holds); setting it frees the refinement's buffers;
  *   "lanes"         1 one stream, 2 the batch split in halves on two streams;
  *   "eigen"         transmission eigenvalues: 0 eigenvalues of t^H t (Householder tridiagonalisation
  *                   and Sturm bisection); 1 one-sided Jacobi SVD of t; 2 Householder bidiagonalisation

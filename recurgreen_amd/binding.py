@@ -442,7 +442,7 @@ class Plan:
         check(lib().rg_plan_synchronize(self._h))
 
     # -- launch options (the autotuner's choices) -------------------------------------------
-    OPTIONS = ("inverse_fp64", "inverse_mixed", "dyson", "dyson_fp64", "project_parts", "project_wide_min", "refine", "lanes",
+    OPTIONS = ("inverse_fp64", "inverse_mixed", "dyson", "dyson_fp64", "project_parts", "project_wide_min", "refine", "refine_cap", "lanes",
                "eigen", "graph", "adjoint_precision")
 
     def set_option(self, key: str, value: int) -> None:

@@ -169,6 +169,7 @@ struct rg_plan {
     // FP64 refinement of the complex64 sweeps (plan option refine: passes after the first): for passes of
     // ref_cap realizations, every slice's g_x, X and R (complex128, np x np) and coefficients; ref_vcap V blocks
     int refine = 0, ref_cap = 0, ref_vcap = 0;
+    int refine_cap = 0;  // plan option: at most this many realizations per refined pass (0: from the free memory)
     double2 *d_ref_g = nullptr, *d_ref_x = nullptr, *d_ref_r = nullptr, *d_ref_w = nullptr, *d_ref_v = nullptr;
     double *d_ref_cq = nullptr;
     double2 *d_ck = nullptr, *d_seg = nullptr, *d_adj = nullptr;
@@ -332,6 +333,7 @@ void rg_zgemm(rg_plan *P, int nb, int m, int n, int k, double2 alpha, const doub
 // FP64 refinement of the complex64 sweeps (rg_refine.hip): buffers (sets ref_cap), the store of slice ix
 // after its step, and the refinement passes, returning G(L;0) (nb x np x np complex128)
 rg_status rg_refine_reserve(rg_plan *P);
+void rg_refine_release(rg_plan *P);
 void rg_refine_store(rg_plan *P, int nb, int ix, bool with_disorder, const float2 *gs1);
 const double2 *rg_refine_solve(rg_plan *P, int nb);
 size_t rg_schur_elems(int np);

@@ -1800,6 +1800,10 @@ rg_status rg_plan_set_option(rg_plan *P, const char *key, int32_t value)
     } else if (std::strcmp(key, "refine") == 0) {
         if (value < 0 || value > 8) return RG_EINVAL;
         P->refine = value;
+    } else if (std::strcmp(key, "refine_cap") == 0) {
+        if (value < 0) return RG_EINVAL;
+        P->refine_cap = value;
+        rg_refine_release(P);  // the next refined call sizes its buffers again
     } else if (std::strcmp(key, "lanes") == 0) {
         if (value != 1 && value != 2) return RG_EINVAL;
         if (value == 2) {
@@ -1834,6 +1838,7 @@ rg_status rg_plan_get_option(const rg_plan *P, const char *key, int32_t *value)
     else if (std::strcmp(key, "project_parts") == 0) *value = P->cq_parts;
     else if (std::strcmp(key, "project_wide_min") == 0) *value = P->project_wide_min;
     else if (std::strcmp(key, "refine") == 0) *value = P->refine;
+    else if (std::strcmp(key, "refine_cap") == 0) *value = P->refine_cap;
     else if (std::strcmp(key, "lanes") == 0) *value = P->lanes;
     else if (std::strcmp(key, "eigen") == 0) *value = P->eigen;
     else if (std::strcmp(key, "profile_mask") == 0) *value = P->profile_mask;

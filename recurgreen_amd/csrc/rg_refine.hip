@@ -131,7 +131,8 @@ rg_status rg_refine_reserve(rg_plan *P)
     const size_t per = 3 * nx * mat + nx * (size_t)P->ntab * sizeof(double) + 2 * mat;  // g, X, R, cq, W, V
     const size_t budget = fr / 10 * 6;
     if (budget <= vcap0 * mat + per) return RG_ENOMEM;
-    const size_t cap = std::min<size_t>({(size_t)P->batch_max, 65535, (budget - vcap0 * mat) / per});
+    size_t cap = std::min<size_t>({(size_t)P->batch_max, 65535, (budget - vcap0 * mat) / per});
+    if (P->refine_cap > 0) cap = std::min<size_t>(cap, (size_t)P->refine_cap);
     const size_t vcap = std::max(vcap0, cap);
     auto alloc = [](auto **p, size_t count) {
         return hipMalloc((void **)p, count * sizeof(**p)) == hipSuccess;
@@ -147,6 +148,16 @@ rg_status rg_refine_reserve(rg_plan *P)
     P->ref_cap = (int)cap;
     P->ref_vcap = (int)vcap;
     return RG_OK;
+}
+
+void rg_refine_release(rg_plan *P)
+{
+    for (void *p : {(void *)P->d_ref_g, (void *)P->d_ref_x, (void *)P->d_ref_r, (void *)P->d_ref_w, (void *)P->d_ref_v,
+                    (void *)P->d_ref_cq})
+        if (p) (void)hipFree(p);
+    P->d_ref_g = P->d_ref_x = P->d_ref_r = P->d_ref_w = P->d_ref_v = nullptr;
+    P->d_ref_cq = nullptr;
+    P->ref_cap = P->ref_vcap = 0;
 }
 
 void rg_refine_store(rg_plan *P, int nb, int ix, bool with_disorder, const float2 *gs1)

@@ -78,6 +78,32 @@ def test_refined_transmission_matrix(passes):
     plan.close()
 
 
+def test_refined_passes_split_by_refine_cap():
+    """Five realizations through passes of 4 + 1 (batch_max) and of 2 + 2 + 1 (refine_cap = 2): each within
+    the FP64 bar of the oracle either way."""
+    p = CASES["N64"]
+    say("refined passes split by refine_cap, N64")
+    ow = O.Waveguide(**p)
+    seeds = [21, 22, 23, 24, 25]
+    U = upot_batch(ow, seeds)
+    To = [ow.tdistrib_potential(U[:, :, k].T) for k in range(len(seeds))]
+    plan = binding.Plan(p, batch_max=4)
+    plan.set_option("refine", 2)
+    out = {}
+    for cap in (0, 2):
+        plan.set_option("refine_cap", cap)
+        assert plan.get_option("refine_cap") == cap
+        T, st, code = plan.tdistrib_potential(U, "mixed_fp32")
+        assert code == 0 and not st.any()
+        out[cap] = T.reshape(len(seeds), -1)
+        for k in range(len(seeds)):
+            assert_T_close(out[cap][k], To[k])
+    with pytest.raises(binding.RecurGreenError):
+        plan.set_option("refine_cap", -1)
+    plan.close()
+    say(f"N64 passes 4 + 1 against 2 + 2 + 1: max |dT| {np.max(np.abs(out[0] - out[2])):.2e}")
+
+
 @pytest.mark.parametrize("name", ["n128", "c3_nx200"])
 def test_refined_fullsize_against_oracle(name):
     ent = FIX[name]
