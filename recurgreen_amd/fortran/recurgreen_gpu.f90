@@ -5,7 +5,8 @@
 !! Usage: recurgreen_gpu settings.nml [more.nml ...]
 !! Reads the reference's groups unchanged: &settings task (src/main.f90:45), &waveguide and
 !! &tdistrib (src/main.f90:83-84), and the optional group
-!!   &gpu  device, precision ('fp64' | 'mixed_fp32' | 'mixed_bf16x3'), batch, ny, dos_model, bintype /
+!!   &gpu  device, precision ('fp64' | 'mixed_fp32' | 'mixed_bf16x3'), batch, ny, dos_model, bintype,
+!!         refine (tdistrib: FP64 refinement passes of G(L;0) for the mixed precisions, default 0) /
 !! that has no counterpart in the reference (a file without it parses as before).  The OpenMP loop of
 !! compute_eigvals_omp (src/main.f90:134-179) becomes one batched device pass; the histogram chain
 !! follows prepare_distrib (src/main.f90:223-251) and the CSV layout of save_distrib / save_raw_eigvals
@@ -82,18 +83,18 @@ contains
         type(waveguide_gpu_t) :: wguide
         character(len=50) :: boundtype, precision, bintype
         real(wp) :: wol, aratio, exfac, dscat, dabso, napera, naperb, time, speed, missing
-        integer :: nxdiso, nxfree, nseed, nbins, nthreads, istat, device, batch, ny, dos_model, prec, k
+        integer :: nxdiso, nxfree, nseed, nbins, nthreads, istat, device, batch, ny, dos_model, prec, k, refine
         logical :: save_eigvals
         integer(8) :: c0, c1, crate
         real(wp), allocatable :: tallvals(:), bins(:), histo(:), rho(:, :)
         namelist /waveguide/ boundtype, wol, exfac, dscat, dabso, aratio, nxdiso, nxfree, napera, naperb
         namelist /tdistrib/ nseed, nbins, nthreads, save_eigvals
-        namelist /gpu/ device, precision, batch, ny, dos_model, bintype
+        namelist /gpu/ device, precision, batch, ny, dos_model, bintype, refine
 
         boundtype = "dirichlet"; wol = 4.3_wp; exfac = 1.0_wp; aratio = 1.0_wp; dscat = 1.0_wp; dabso = 0.0_wp
         nxdiso = 100; nxfree = 0; napera = 1.0_wp; naperb = 1.0_wp
         nseed = 1; nbins = 128; nthreads = 1; save_eigvals = .false.
-        device = 0; precision = "fp64"; batch = 1024; ny = 0; dos_model = 0; bintype = "chebyshev"
+        device = 0; precision = "fp64"; batch = 1024; ny = 0; dos_model = 0; bintype = "chebyshev"; refine = 0
         read(unit=funit, nml=waveguide, iostat=istat)
         call rewind_or_stop(funit, istat, .false.)
         read(unit=funit, nml=tdistrib, iostat=istat)
@@ -116,6 +117,7 @@ contains
         call wguide%set_options(ny, dos_model, prec)
         call wguide%init(boundtype, nxdiso, nxfree, wol, exfac, aratio, dscat, dabso, napera, naperb, &
                          batch_max=min(batch, nseed), device=device)
+        if (refine > 0) call wguide%set_plan_option("refine", refine)
         call wguide%print_parameters(output_unit, "[INFO]")
         write (output_unit, '(3(a,i0),2a)') "[INFO] Nseed=", nseed, ", Nbins=", nbins, ", Device=", device, &
             ", Precision=", trim(precision)

@@ -37,6 +37,12 @@ module recurgreen_hip
             import :: c_int, c_ptr
             type(c_ptr), value :: plan
         end function
+        integer(c_int) function rg_plan_set_option(plan, key, value) bind(C, name="rg_plan_set_option")
+            import :: c_int, c_int32_t, c_ptr, c_char
+            type(c_ptr), value :: plan
+            character(kind=c_char), dimension(*), intent(in) :: key
+            integer(c_int32_t), value :: value
+        end function
         integer(c_int) function rg_plan_dims(plan, dims, vals) bind(C, name="rg_plan_dims")
             import :: c_int, c_int32_t, c_double, c_ptr
             type(c_ptr), value :: plan
@@ -125,6 +131,7 @@ module recurgreen_hip
     contains
         procedure :: init => init_waveguide_gpu
         procedure :: set_options => set_options_waveguide_gpu
+        procedure :: set_plan_option => set_plan_option_waveguide_gpu
         procedure :: get_nmu => get_nmu_waveguide_gpu
         procedure :: get_neigv => get_neigv_waveguide_gpu
         procedure :: get_nx => get_nx_waveguide_gpu
@@ -174,6 +181,14 @@ contains
 
     !! Extra settings with no counterpart in the reference (ny > nmu config mode, DOS model,
     !! precision, device, batch size); must be called before init.
+    !! A launch option of the plan (rg_plan_set_option, keys in include/recurgreen_hip.h), after init.
+    subroutine set_plan_option_waveguide_gpu(self, key, value)
+        class(waveguide_gpu_t), intent(inout) :: self
+        character(len=*), intent(in) :: key
+        integer, intent(in) :: value
+        call check_status(rg_plan_set_option(self%plan, trim(key) // c_null_char, int(value, c_int32_t)))
+    end subroutine
+
     subroutine set_options_waveguide_gpu(self, ny, dos_model, precision)
         class(waveguide_gpu_t), intent(inout) :: self
         integer, intent(in) :: ny, dos_model, precision

@@ -90,3 +90,24 @@ def test_fortran_tdistrib_matches_python_path(tmp_path):
     rho = distrib.load_distrib(str(dis))
     ref, miss = O.prepare_distrib(T, 64, "chebyshev")
     np.testing.assert_allclose(rho, ref, rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_fortran_tdistrib_refined_mixed(tmp_path):
+    """precision = 'mixed_fp32' with refine = 2 in the &gpu group: the eigenvalues meet the FP64 bar
+    against the oracle's (the plain complex64 sweep does not)."""
+    from oracle import oracle as O
+    from recurgreen_amd import distrib
+
+    from test_gpu_parity import assert_T_close
+
+    print("[gpu-case] fortran host tdistrib, mixed_fp32 refined", flush=True)
+    grp = GPU_GROUP.replace("precision = 'fp64'", "precision = 'mixed_fp32'").replace("/\n", "    refine = 2\n/\n")
+    r = run(tmp_path, NML.format(wol=10.25, aratio=10 / 10.25) + grp)
+    assert r.returncode == 0 and "Result written" in r.stdout, r.stdout + r.stderr
+    d = tmp_path / "out" / "dscat_2." / "dabso_0." / "wol_10.25" / "dirichlet" / "distrib"
+    T = distrib.read_real_array(str(d / "result_1.eigvals.csv")).reshape(16, -1)
+    ow = O.Waveguide("dirichlet", nxdiso=200, wol=10.25, aratio=10 / 10.25, dscat=2.0, ny=205, dos_model=1)
+    To = ow.tdistrib_batch(1, 16, nthreads=4).reshape(16, -1)
+    for k in range(16):
+        assert_T_close(T[k], To[k])
