@@ -96,10 +96,11 @@ __global__ __launch_bounds__(kEw) void k_ref_resid(double2 *__restrict__ R, cons
     }
 }
 
-// W = Pm - diag(b) Q per realization (Pm or Q null: zero).  Grid (blocks, realizations).
+// W = Pm - diag(b) Q per realization (Pm or Q null: zero); with Xq, also Xq += Q (the backward half adds
+// each finished d_{x+1} to X as it reads it).  Grid (blocks, realizations).
 __global__ __launch_bounds__(kEw) void k_ref_rhs(const double2 *__restrict__ Pm, const double2 *__restrict__ Q,
                                                  const double2 *__restrict__ bvec, int cs, double2 *__restrict__ W, int n,
-                                                 int np)
+                                                 int np, double2 *__restrict__ Xq)
 {
     const int b = blockIdx.y;
     const size_t o = (size_t)b * np * np;
@@ -108,7 +109,11 @@ __global__ __launch_bounds__(kEw) void k_ref_rhs(const double2 *__restrict__ Pm,
         const int r = e / np, col = e % np;
         if (r >= n || col >= n) continue;
         double2 v = Pm ? Pm[o + e] : cmk(0.0, 0.0);
-        if (Q) v = csub(v, cmul(bv[r], Q[o + e]));
+        if (Q) {
+            const double2 q = Q[o + e];
+            v = csub(v, cmul(bv[r], q));
+            if (Xq) Xq[o + e] = cadd(Xq[o + e], q);
+        }
         W[o + e] = v;
     }
 }
@@ -194,17 +199,17 @@ const double2 *rg_refine_solve(rg_plan *P, int nb)
         // forward: R_x <- y_x
         for (int x = 0; x < nx && pass > 0; x++) {
             hipLaunchKernelGGL(k_ref_rhs, dim3(gx, nb), dim3(kEw), 0, P->stream, R + x * slab,
-                               x > 0 ? R + (x - 1) * slab : nullptr, P->d_b, P->cstride, W, n, np);
+                               x > 0 ? R + (x - 1) * slab : nullptr, P->d_b, P->cstride, W, n, np, nullptr);
             rg_zgemm(P, nb, n, n, n, one, G + x * slab, np, np2, W, np, np2, zero, R + x * slab, np, np2);
         }
-        // backward: R_x <- d_x = y_x + g_x (-b d_{x+1})
+        // backward: R_x <- d_x = y_x + g_x (-b d_{x+1}), and X_{x+1} += d_{x+1}; then X_1 += d_1
         for (int x = nx - 2; x >= 0; x--) {
             hipLaunchKernelGGL(k_ref_rhs, dim3(gx, nb), dim3(kEw), 0, P->stream, nullptr, R + (x + 1) * slab, P->d_b,
-                               P->cstride, W, n, np);
+                               P->cstride, W, n, np, X + (x + 1) * slab);
             rg_zgemm(P, nb, n, n, n, one, G + x * slab, np, np2, W, np, np2, one, R + x * slab, np, np2);
         }
-        hipLaunchKernelGGL(k_ref_add, dim3((unsigned)std::min<size_t>(4096, (all + kEw - 1) / kEw)), dim3(kEw), 0,
-                           P->stream, X, (const double2 *)R, all);
+        hipLaunchKernelGGL(k_ref_add, dim3((unsigned)std::min<size_t>(4096, (slab + kEw - 1) / kEw)), dim3(kEw), 0,
+                           P->stream, X, (const double2 *)R, slab);
     }
     return X + (size_t)(nx - 1) * slab;
 }
